@@ -1,0 +1,118 @@
+"""Benchmark of the quasi-static step on one MI355X: plasma particle-slices/s and beam pushes/s, FP64.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
+
+A step is one quasi-static time step of the workload (quarz_amd/configs.py): beam deposit, the plasma sweep
+over every slice (one persistent kernel), the beam push.  W untimed steps, then exactly K timed steps between
+device synchronisations.  Prints one JSON line:
+  value          plasma particle-slices per second of the whole timed steps (wall clock)
+  beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
+  roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops
+                 679 per particle-slice + 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP
+                 events on the simulation's stream
+  cpu_baseline   the oracle's OpenMP build of the same step (the reference's loop structure) timed on this
+                 host's cores on one full sweep of the same deck
+The inputs (beam particles, grid, operators) are resident in HBM before the timed region; the plasma is
+reloaded at the head of the box on the host every step, as in the reference.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "plasma particle-slices/s & beam pushes/s (FP64, 1 GPU); batched vs plain"
+FP64_PEAK = 78.6e12
+FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
+
+
+def cpu_baseline(deck, nthreads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    oracle_lib.set_threads(nthreads)
+    s = oracle_lib.Sim(deck, parallel=True)
+    t0 = time.perf_counter()
+    s.step_fields()
+    wall = time.perf_counter() - t0
+    sweep_s, _, psl = s.timing()
+    return {"value": psl / wall, "unit": "particle-slices/s", "cores": nthreads, "kind": "port",
+            "sample": f"one full sweep of the same deck ({s.K} slices, {int(psl)} particle-slices, "
+                      f"{wall:.2f} s, OpenMP atomics, g++ -O2 -ffp-contract=off)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
+              f"see DESIGN.md for how the path shards)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import quarz_amd
+    from quarz_amd import configs
+    deck = configs.deck(args.config)
+    total = args.warmup + args.steps + 1
+    sim = quarz_amd.Simulation(deck, {"time.steps": total})
+    modes = sim.m1
+    for _ in range(args.warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    psl = pushes = 0.0
+    sweep_ms = beam_ms = dep_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step()
+        st = sim.stats()
+        psl += st["particle_slices"]
+        pushes += st["beam_pushed"]
+        sweep_ms += st["sweep_ms"]
+        beam_ms += st["beam_push_ms"]
+        dep_ms += st["beam_dep_ms"]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fp, fn = FLOPS[modes]
+    flops_per_sweep = fp * psl / args.steps + fn * sim.M * sim.K
+    t_kernel = sweep_ms / args.steps / 1e3
+    achieved = flops_per_sweep / t_kernel
+    out = {
+        "metric": METRIC,
+        "value": psl / wall,
+        "unit": "particle-slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: beams sampled from the deck's seeds, plasma loaded at the box head",
+        "config": {"workload": f"{args.config}: {configs.DESCRIPTION[args.config]}", "M": sim.M, "slices": sim.K,
+                   "plasma_particles_per_slice": int(round(psl / args.steps / max(sim.K, 1))),
+                   "beam_particles": int(round(pushes / args.steps)), "parallelism": "one run per GPU"},
+        "beam_pushes_per_s": pushes / wall,
+        "sweep_ms": sweep_ms / args.steps,
+        "beam_deposit_ms": dep_ms / args.steps,
+        "beam_push_ms": beam_ms / args.steps,
+        "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel",
+                     "flops_per_launch": flops_per_sweep},
+    }
+    if not args.no_cpu:
+        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(deck, min(16, ncpu))
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

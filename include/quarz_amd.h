@@ -1,0 +1,162 @@
+/*
+ * quarz_amd -- C-ABI of the MI355X (gfx950) quasi-static step of QUARZ.
+ *
+ * The reference has no plugin/FFI layer: its hot path is a set of C++ member functions dispatched
+ * through Kokkos.  Each entry point below replaces one of them (reference file:line cited); the C++ host
+ * classes of this library (Simulation, PlasmaSpecies, Beam, FieldSolver, RadialGrid, Config) call these
+ * exactly where the reference calls the member functions.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Device pointers are hipMalloc'd memory of the current device;
+ *     `stream` is a hipStream_t (NULL: the default stream).  The library never frees caller memory.
+ *   - Arrays are FP64, layouts as the reference: (xi, r, component) arrays LayoutRight with the component
+ *     fastest (src/FieldStore.hpp:9-21), particle data SoA.
+ *   - Every function returns 0 on success and a negative code on failure (bad extent, unsupported size,
+ *     HIP error); qz_last_error() returns the message.  Kernels never fail: removed particles are data
+ *     (w = 0), as in the reference.
+ *   - Results are bit-identical to a serial (single-thread) CPU build of the reference arithmetic:
+ *     deposits are summed per node in particle-index order, no multiply-add is fused, and divisions and
+ *     square roots are correctly rounded.
+ */
+#ifndef QUARZ_AMD_H
+#define QUARZ_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes */
+#define QZ_OK 0
+#define QZ_EINVAL (-1)    /* bad argument or extent */
+#define QZ_ERANGE (-2)    /* size beyond what the kernels support */
+#define QZ_EHIP (-3)      /* HIP runtime error */
+
+/* Device view of the radial grid (GridD, src/RadialGrid.hpp:26-55). */
+typedef struct qz_radial_grid {
+    const double* r;      /* N+1 nodes */
+    const double* inv_h;  /* N: 1 / h_j */
+    const double* inv_V;  /* N+1: 1 / V_j (hat-function volumes) */
+    int N;                /* cells */
+    double R;             /* outer radius */
+} qz_radial_grid;
+
+/* Longitudinal box of the beams (BeamGrid, src/Beam.hpp:25-30). */
+typedef struct qz_beam_grid {
+    double xi_min, dxi;
+    int nxi, k0, nloc;
+    int ngp;              /* 1: nearest-slice shape in xi, 0: linear */
+} qz_beam_grid;
+
+/* Beam particles, SoA x y px py pz xi w (Beam::x_ ... w_, src/Beam.hpp:168). */
+typedef struct qz_beam_soa {
+    double *x, *y, *px, *py, *pz, *xi, *w;
+} qz_beam_soa;
+
+/* Mobile plasma species state (PlasmaSpecies, src/PlasmaSpecies.hpp:132-177). */
+typedef struct qz_species_state {
+    double *x, *y, *px, *py, *dl, *w;   /* np live slots; dl = Delta = gamma - p_z */
+    double* hist;                       /* AB history (ab_order, 5, cap), particle fastest */
+    int np, cap;
+    double q, m;
+} qz_species_state;
+
+#define QZ_MAX_SPECIES 2
+
+/* Everything one quasi-static sweep reads and writes (Simulation::step_fields, src/Simulation.cpp:636-795). */
+typedef struct qz_sweep_args {
+    qz_radial_grid grid;
+    int modes;                 /* 0: m = 0; 1: m = 0 and 1 */
+    int picard;                /* Picard iterations of the chi_1 coupling (m = 1) */
+    int nslices;               /* K slices, head first */
+    int ab_order;              /* Adams-Bashforth order 1..5 */
+    int ngp;                   /* beams.xi_shape = ngp */
+    double dxi, delta_min, max_qsa;
+    double ab_c[5];            /* AB weights of every push of the sweep (uniform steps) */
+    /* operators L0, L1D, L1F, L2D: tridiagonal rows a, b, c (FieldSolver, src/FieldSolver.cpp:43-109) */
+    const double* op_a[4];
+    const double* op_b[4];
+    const double* op_c[4];
+    double flux_coef;          /* g coefficient of the flux row of L1F */
+    const double *grad_m, *grad_0, *grad_p, *inv_r;   /* gradient stencils, 1/r (FieldSolver.cpp:157-176) */
+    /* Thomas factors of the chi-free L0 and L1D: m (M) and c' (M) as Thomas forms them (Tridiag.cpp:114-121) */
+    const double *fac_l0_m, *fac_l0_cp, *fac_l1_m, *fac_l1_cp;
+    const double* bsrc;        /* (K, M, ncb) beam sources */
+    int ncb;
+    const double* bg;          /* background rho - J_z (3 M: mode 0, 1r, 1i) then rho (3 M) */
+    double* fld;               /* (K, M, ncf) fields for the beams */
+    int ncf;
+    double* diag;              /* (K, M, ncd) psi, n_e, n_i */
+    int ncd;
+    int nspecies;              /* mobile species (<= QZ_MAX_SPECIES) */
+    qz_species_state sp[QZ_MAX_SPECIES];
+    int* lost;                 /* one counter per species (particles removed as trapped) */
+    double* ws;                /* per-run workspace of qz_sweep_workspace_doubles(M) doubles */
+    unsigned long long* stats; /* optional: stats[0] += live particles of every slice (particle-slices) */
+} qz_sweep_args;
+
+const char* qz_last_error(void);
+
+/* Library / device information: gcnArchName and the number of CUs of the current device. */
+int qz_device_info(char* arch, int arch_len, int* n_cu);
+
+/* Batched Thomas solve of `batch` systems of size M, system s at offset s*M of every array
+ * (TridiagSolver::solve, Thomas branch, src/Tridiag.cpp:111-126).  a[0] and c[M-1] are not used. */
+int qz_tridiag_thomas_batched(const double* a, const double* b, const double* c, const double* d, double* x, int M,
+                              int batch, void* stream);
+
+/* Beam deposit: adds the sources of one beam to bsrc (nloc, N+1, nc)
+ * (Beam::deposit, src/Beam.cpp:725-785).  Particles are summed per node in index order.
+ * work: device scratch of qz_beam_deposit_workspace_bytes(np, nloc, N) bytes. */
+size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N);
+int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
+                    const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream);
+
+/* Beam push with the stored fields fld (nloc, N+1, nc): gather, pusher (0 vay, 1 boris, 2 hc, 3 imp),
+ * drift; move = 0 is the explicit leapfrog start-up kick (Beam::advance, src/Beam.cpp:948-1131). */
+int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher, int i0,
+                    int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt, int move,
+                    void* stream);
+
+/* The 17 weighted beam sums relative to gref and xiref (Beam::sums, src/Beam.cpp:1133-1155). */
+int qz_beam_sums(int np, const qz_beam_soa* p, double gref, double xiref, double* out17_host, void* stream);
+
+/* One quasi-static sweep over all slices of one run: per slice the mode deposit, the psi / E_z / B_z and
+ * B+ solves, the field store and the Adams-Bashforth push, in one persistent kernel
+ * (Simulation::step_fields slice loop, src/Simulation.cpp:727-783).  args points to host memory. */
+size_t qz_sweep_workspace_doubles(int M);
+int qz_sweep(const qz_sweep_args* args, void* stream);
+
+/* Many independent runs in one persistent launch, one workgroup per run (not in the reference). */
+int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream);
+
+/* ---- whole simulation (Simulation, src/Simulation.hpp:67-103) on a deck (src/Config.cpp format) ---- */
+typedef struct qz_sim qz_sim;
+/* deck: text of an input deck; overrides: "key=value" lines (src/main.cpp:82-93). */
+int qz_sim_create(const char* deck, const char* overrides, qz_sim** out);
+void qz_sim_destroy(qz_sim* s);
+/* info: M K modes NCF NCB NCD nspecies nbeams nsteps next_step */
+int qz_sim_info(const qz_sim* s, int* info10);
+/* one step of Simulation::run (src/Simulation.cpp:1376-1459): sweep, then the beam push unless last */
+int qz_sim_step(qz_sim* s);
+int qz_sim_step_fields(qz_sim* s);   /* the sweep only */
+int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
+/* what: fld diag bsrc nodes, beam<b>.<x|y|px|py|pz|xi|w>, sp<s>.<x|y|px|py|dl|w>; returns the count */
+long qz_sim_get(qz_sim* s, const char* what, double* host_out, long cap);
+int qz_sim_beam_diag(qz_sim* s, int b, double* out15);
+int qz_sim_beam_sums(qz_sim* s, int b, double* out17, double* refs2);
+/* times of the last sweep and beam push (ms, HIP events on the simulation's stream) */
+int qz_sim_timing(const qz_sim* s, double* out2);
+/* work of the last step: particle-slices of the sweep, beam particles pushed, beam particles deposited,
+ * sweep kernel ms, beam deposit ms, beam push ms */
+int qz_sim_stats(const qz_sim* s, double* out6);
+long qz_sim_lost(const qz_sim* s);
+double qz_sim_time(const qz_sim* s);
+int qz_sim_write_fields(qz_sim* s, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

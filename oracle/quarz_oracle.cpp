@@ -1415,6 +1415,7 @@ struct Sim {
     std::vector<Real> fld, bsrc, diag;
     long lost_last = 0;
     double t_sweep = 0, t_beam = 0;
+    double particle_slices = 0;   // live mobile particles summed over the slices of the last sweep
 
     void init(const std::string& deck, const std::string& ov, bool parallel) {
         par = parallel;
@@ -1720,8 +1721,12 @@ struct Sim {
     template <bool PAR> void sweep_slices() {
         const int Kg = box.nxi, k0 = box.k0, KL = box.nloc;
         const Real dxi = box.dxi;
+        particle_slices = 0;
         for (int kl = 0; kl < KL; ++kl) {
             const int k = k0 + kl;
+            for (auto& sp : species)
+                if (sp.mobile)
+                    for (int i = 0; i < sp.Np; ++i) particle_slices += sp.w[i] != Real(0) ? 1.0 : 0.0;
             zero_sources();
             for (auto& sp : species)
                 if (sp.mobile) sp.template deposit<PAR>(src);
@@ -2053,7 +2058,7 @@ int or_sim_sweep_partial(void* h, int nk) {
 }
 int or_sim_timing(void* h, double* out) {
     Sim& s = *static_cast<Sim*>(h);
-    out[0] = s.t_sweep; out[1] = s.t_beam;
+    out[0] = s.t_sweep; out[1] = s.t_beam; out[2] = s.particle_slices;
     return 0;
 }
 long or_sim_lost(void* h) { return static_cast<Sim*>(h)->lost_last; }

@@ -1,0 +1,228 @@
+"""quarz_amd: the quasi-static time step of QUARZ on MI355X (gfx950).
+
+The product is libquarz_amd.so (C++ host + gfx950 HIP kernels behind the C-ABI of include/quarz_amd.h).
+This module is a thin ctypes driver over it: `Simulation` mirrors the reference's Simulation on a deck
+(src/Simulation.hpp:67-103), and the kernel-level entry points take device pointers (e.g. of torch tensors).
+There is no CPU fallback: without the built library every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libquarz_amd.so")
+
+_lib = None
+
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+class QuarzError(RuntimeError):
+    pass
+
+
+class RadialGridC(C.Structure):
+    _fields_ = [("r", C.c_void_p), ("inv_h", C.c_void_p), ("inv_V", C.c_void_p), ("N", C.c_int), ("R", C.c_double)]
+
+
+class BeamGridC(C.Structure):
+    _fields_ = [("xi_min", C.c_double), ("dxi", C.c_double), ("nxi", C.c_int), ("k0", C.c_int), ("nloc", C.c_int),
+                ("ngp", C.c_int)]
+
+
+class BeamSoAC(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("x", "y", "px", "py", "pz", "xi", "w")]
+
+
+def lib():
+    """Load libquarz_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise QuarzError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        L.qz_last_error.restype = C.c_char_p
+        L.qz_device_info.argtypes = [C.c_char_p, C.c_int, IP]
+        L.qz_tridiag_thomas_batched.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_void_p]
+        L.qz_beam_deposit_workspace_bytes.restype = C.c_size_t
+        L.qz_beam_deposit_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.qz_beam_deposit.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
+                                      C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.qz_beam_advance.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
+                                      C.c_int, C.c_int, C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double,
+                                      C.c_double, C.c_int, C.c_void_p]
+        L.qz_beam_sums.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, DP, C.c_void_p]
+        L.qz_sweep_workspace_doubles.restype = C.c_size_t
+        L.qz_sweep_workspace_doubles.argtypes = [C.c_int]
+        L.qz_sim_create.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+        L.qz_sim_destroy.argtypes = [C.c_void_p]
+        L.qz_sim_info.argtypes = [C.c_void_p, IP]
+        L.qz_sim_step.argtypes = [C.c_void_p]
+        L.qz_sim_step_fields.argtypes = [C.c_void_p]
+        L.qz_sim_push_beams.argtypes = [C.c_void_p]
+        L.qz_sim_get.restype = C.c_long
+        L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
+        L.qz_sim_beam_diag.argtypes = [C.c_void_p, C.c_int, DP]
+        L.qz_sim_beam_sums.argtypes = [C.c_void_p, C.c_int, DP, DP]
+        L.qz_sim_timing.argtypes = [C.c_void_p, DP]
+        L.qz_sim_stats.argtypes = [C.c_void_p, DP]
+        L.qz_sim_lost.restype = C.c_long
+        L.qz_sim_lost.argtypes = [C.c_void_p]
+        L.qz_sim_time.restype = C.c_double
+        L.qz_sim_time.argtypes = [C.c_void_p]
+        L.qz_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().qz_last_error().decode()
+
+
+def check(rc):
+    if rc != 0:
+        raise QuarzError(last_error())
+    return rc
+
+
+def device_info():
+    buf = C.create_string_buffer(64)
+    ncu = C.c_int(0)
+    check(lib().qz_device_info(buf, 64, C.byref(ncu)))
+    return buf.value.decode(), ncu.value
+
+
+BEAM_ARRAYS = ("x", "y", "px", "py", "pz", "xi", "w")
+DIAG_KEYS = ["t", "npart", "charge", "gamma_mean", "gamma_rms", "xi_mean", "xi_rms", "r_rms", "emit_nx", "x_mean",
+             "y_mean", "emit_ny", "sigma_x", "sigma_y", "alive"]
+
+
+class Simulation:
+    """One QUARZ run on a deck (text or path) with key=value overrides; the step runs on the GPU."""
+
+    def __init__(self, deck, overrides=None):
+        if os.path.exists(deck):
+            deck = open(deck).read()
+        ov = "\n".join(f"{k}={v}" for k, v in (overrides or {}).items())
+        h = C.c_void_p()
+        check(lib().qz_sim_create(deck.encode(), ov.encode(), C.byref(h)))
+        self.h = h
+        info = (C.c_int * 10)()
+        check(lib().qz_sim_info(self.h, info))
+        self.M, self.K, self.m1, self.NCF, self.NCB, self.NCD, self.nspecies, self.nbeams, self.nsteps = info[:9]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qz_sim_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self):
+        check(lib().qz_sim_step(self.h))
+
+    def step_fields(self):
+        check(lib().qz_sim_step_fields(self.h))
+
+    def push_beams(self):
+        check(lib().qz_sim_push_beams(self.h))
+
+    def timing(self):
+        """(sweep ms, beam push ms) of the last step, from HIP events on the run's stream."""
+        out = np.zeros(2)
+        check(lib().qz_sim_timing(self.h, out.ctypes.data_as(DP)))
+        return out
+
+    def stats(self):
+        """Work of the last step: particle_slices, beam_pushed, beam_deposited, sweep_ms, beam_dep_ms, beam_push_ms."""
+        out = np.zeros(6)
+        check(lib().qz_sim_stats(self.h, out.ctypes.data_as(DP)))
+        return dict(zip(["particle_slices", "beam_pushed", "beam_deposited", "sweep_ms", "beam_dep_ms", "beam_push_ms"], out))
+
+    def lost(self):
+        return lib().qz_sim_lost(self.h)
+
+    @property
+    def t(self):
+        return lib().qz_sim_time(self.h)
+
+    def get(self, what):
+        n = lib().qz_sim_get(self.h, what.encode(), None, 0)
+        if n < 0:
+            raise KeyError(f"{what}: {last_error()}")
+        out = np.zeros(n)
+        lib().qz_sim_get(self.h, what.encode(), out.ctypes.data_as(DP), n)
+        return out
+
+    @property
+    def fld(self):
+        return self.get("fld").reshape(self.K, self.M, self.NCF)
+
+    @property
+    def diag(self):
+        return self.get("diag").reshape(self.K, self.M, self.NCD)
+
+    @property
+    def bsrc(self):
+        return self.get("bsrc").reshape(self.K, self.M, self.NCB)
+
+    @property
+    def nodes(self):
+        return self.get("nodes")
+
+    def beam(self, b):
+        return {k: self.get(f"beam{b}.{k}") for k in BEAM_ARRAYS}
+
+    def beam_sums(self, b):
+        out, refs = np.zeros(17), np.zeros(2)
+        check(lib().qz_sim_beam_sums(self.h, b, out.ctypes.data_as(DP), refs.ctypes.data_as(DP)))
+        return out, refs
+
+    def beam_diag(self, b):
+        out = np.zeros(15)
+        check(lib().qz_sim_beam_diag(self.h, b, out.ctypes.data_as(DP)))
+        return dict(zip(DIAG_KEYS, out))
+
+    def write_fields(self, path):
+        check(lib().qz_sim_write_fields(self.h, path.encode()))
+
+
+# ---------------------------------------------------------------------------- kernel-level entry points
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def thomas_batched(a, b, c, d, x, stream=None):
+    """Batched Thomas solve on device tensors (batch, M), float64."""
+    batch, M = d.shape
+    check(lib().qz_tridiag_thomas_batched(_ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), M, batch, stream))
+
+
+def radial_grid(r, inv_h, inv_V):
+    """qz_radial_grid over device tensors (r: N+1, inv_h: N, inv_V: N+1)."""
+    return RadialGridC(r.data_ptr(), inv_h.data_ptr(), inv_V.data_ptr(), r.numel() - 1, float(r[-1].item()))
+
+
+def beam_soa(p):
+    return BeamSoAC(*[p[k].data_ptr() for k in BEAM_ARRAYS])
+
+
+def beam_deposit(grid, box, modes, q, p, bsrc, work, stream=None):
+    soa = beam_soa(p)
+    nc = bsrc.shape[-1]
+    check(lib().qz_beam_deposit(C.byref(grid), C.byref(box), modes, q, p["x"].numel(), C.byref(soa), _ptr(bsrc), nc,
+                                _ptr(work), stream))
+
+
+def beam_deposit_workspace_bytes(np_, nloc, N):
+    return lib().qz_beam_deposit_workspace_bytes(np_, nloc, N)
+
+
+def beam_advance(grid, box, modes, qm, pusher, p, fld, dt_kick, dt, move, i0=0, stream=None):
+    soa = beam_soa(p)
+    nc = fld.shape[-1]
+    check(lib().qz_beam_advance(C.byref(grid), C.byref(box), modes, qm, pusher, i0, p["x"].numel(), C.byref(soa),
+                                _ptr(fld), nc, dt_kick, dt, int(move), stream))

@@ -1,0 +1,963 @@
+#include "simulation.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <functional>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+
+namespace qz {
+
+constexpr Real PI = 3.14159265358979323846;   // src/Types.hpp:22
+
+// C-library pow taken at run time, never folded at build time, so that the host constants do not depend
+// on the compiler's constant folding.
+static double (*volatile c_pow)(double, double) = ::pow;
+
+inline Real xi_from_user(Real xu) { return Real(0) - xu; }   // src/Types.hpp:30-31
+inline Real xi_to_user(Real xi) { return Real(0) - xi; }
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+static void qz_check(int rc) {
+    if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
+}
+
+// ---------------------------------------------------------------------------- DevBuf
+template <class T> void DevBuf<T>::alloc(size_t count) {
+    release();
+    n = count;
+    if (count) hip_check(hipMalloc(reinterpret_cast<void**>(&p), sizeof(T) * count), "hipMalloc");
+}
+template <class T> void DevBuf<T>::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+template <class T> void DevBuf<T>::upload(const std::vector<T>& h, hipStream_t st) {
+    if (h.size() > n) alloc(h.size());
+    if (!h.empty()) hip_check(hipMemcpyAsync(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st), "upload");
+}
+template <class T> void DevBuf<T>::download(std::vector<T>& h, hipStream_t st) const {
+    h.resize(n);
+    if (n) hip_check(hipMemcpyAsync(h.data(), p, sizeof(T) * n, hipMemcpyDeviceToHost, st), "download");
+    hip_check(hipStreamSynchronize(st), "sync");
+}
+template struct DevBuf<Real>;
+template struct DevBuf<int>;
+template struct DevBuf<char>;
+template struct DevBuf<unsigned long long>;
+
+// ---------------------------------------------------------------------------- RadialGrid (src/RadialGrid.cpp)
+std::vector<Real> RadialGrid::make_uniform(Real R, Real dr) {   // :14-20
+    if (R <= 0 || dr <= 0) throw std::runtime_error("grid: R and dr must be positive");
+    const int n = std::max(1, static_cast<int>(std::lround(R / dr)));
+    std::vector<Real> nodes(n + 1);
+    for (int j = 0; j <= n; ++j) nodes[j] = R * Real(j) / Real(n);
+    return nodes;
+}
+
+std::vector<Real> RadialGrid::make_regions(Real R, const std::vector<std::pair<double, double>>& regions,
+                                           Real max_ratio) {   // :27-71
+    if (regions.empty()) throw std::runtime_error("grid: grid.regions is empty");
+    if (max_ratio < 1.0) throw std::runtime_error("grid: grid.max_ratio must be >= 1");
+    for (size_t i = 0; i < regions.size(); ++i) {
+        if (regions[i].second <= 0) throw std::runtime_error("grid: region spacing must be > 0");
+        if (i > 0 && regions[i].first <= regions[i - 1].first) throw std::runtime_error("grid: region end radii must increase");
+    }
+    const Real grow = max_ratio - 1.0;
+    auto target = [&](Real x) {
+        Real ht = regions.back().second;
+        for (const auto& reg : regions)
+            if (x < reg.first) { ht = reg.second; break; }
+        Real start = 0;   // arrive at a finer region with its spacing
+        for (const auto& reg : regions) {
+            if (start > x && grow > 0) ht = std::min(ht, Real(reg.second + grow * (start - x)));
+            if (start > x && grow == 0) ht = std::min(ht, Real(reg.second));
+            start = reg.first;
+        }
+        return ht;
+    };
+    std::vector<Real> nodes{0.0};
+    Real x = 0, hprev = target(0);
+    while (x < R * (1 - 1e-12)) {
+        const Real ht = target(x + 0.5 * hprev);
+        Real hnew = std::clamp(ht, hprev / max_ratio, hprev * max_ratio);
+        if (nodes.size() == 1) hnew = target(0);
+        x += hnew;
+        nodes.push_back(x);
+        hprev = hnew;
+        if (nodes.size() > 10000000) throw std::runtime_error("grid: too many nodes");
+    }
+    const size_t n = nodes.size();
+    if (n >= 3 && (nodes[n - 1] - R) > (R - nodes[n - 2])) nodes.pop_back();
+    const Real scale = R / nodes.back();
+    for (auto& v : nodes) v *= scale;
+    nodes.back() = R;
+    return nodes;
+}
+
+std::vector<Real> RadialGrid::read_file(const std::string& filename) {   // :73-85
+    std::ifstream in(filename);
+    if (!in) throw std::runtime_error("grid: cannot open grid file '" + filename + "'");
+    std::vector<Real> nodes;
+    std::string line;
+    while (std::getline(in, line)) {
+        const auto hash = line.find('#');
+        if (hash != std::string::npos) line.erase(hash);
+        if (line.find_first_not_of(" \t\r\n") == std::string::npos) continue;
+        nodes.push_back(std::stod(line));
+    }
+    return nodes;
+}
+
+void RadialGrid::finalize(const std::vector<Real>& nodes, hipStream_t st) {   // :106-148
+    if (nodes.size() < 3) throw std::runtime_error("grid: need at least 2 cells");
+    if (std::abs(nodes[0]) > 0) throw std::runtime_error("grid: first node must be r=0");
+    for (size_t j = 1; j < nodes.size(); ++j)
+        if (!(nodes[j] > nodes[j - 1])) throw std::runtime_error("grid: nodes must be strictly increasing");
+    r = nodes;
+    N = static_cast<int>(r.size()) - 1;
+    R = r[N];
+    h.resize(N);
+    rmid.resize(N);
+    for (int j = 0; j < N; ++j) {
+        h[j] = r[j + 1] - r[j];
+        rmid[j] = 0.5 * (r[j + 1] + r[j]);
+    }
+    // hat volumes: left part r_{j-1} h/2 + h^2/3, right part r_j h/2 + h^2/6
+    V.assign(N + 1, 0.0);
+    for (int j = 0; j <= N; ++j) {
+        Real v = 0;
+        if (j > 0) v += r[j - 1] * h[j - 1] / 2 + h[j - 1] * h[j - 1] / 3;
+        if (j < N) v += r[j] * h[j] / 2 + h[j] * h[j] / 6;
+        V[j] = 2 * PI * v;
+    }
+    inv_h.resize(N);
+    inv_V.resize(N + 1);
+    for (int j = 0; j <= N; ++j) inv_V[j] = 1.0 / V[j];
+    for (int j = 0; j < N; ++j) inv_h[j] = 1.0 / h[j];
+    d_r.upload(r, st);
+    d_inv_h.upload(inv_h, st);
+    d_inv_V.upload(inv_V, st);
+}
+
+// ---------------------------------------------------------------------------- FieldSolver (src/FieldSolver.cpp)
+// rows of the finite-volume L_n (n >= 1) with Dirichlet rows 0 and N (:20-33)
+static void fv_rows(const RadialGrid& g, int n, std::vector<Real>& a, std::vector<Real>& b, std::vector<Real>& c) {
+    const int N = g.N, M = N + 1;
+    a.assign(M, 0); b.assign(M, 0); c.assign(M, 0);
+    b[0] = 1.0;
+    b[N] = 1.0;
+    auto kp = [&](int j) { return 1.0 / (g.h[j] * c_pow(g.rmid[j], 2 * n - 1)); };
+    for (int j = 1; j < N; ++j) {
+        const Real dV = 0.5 * (g.h[j] + g.h[j - 1]);
+        const Real pre = c_pow(g.r[j], n - 1) / dV;
+        c[j] = pre * c_pow(g.r[j + 1], n) * kp(j);
+        a[j] = pre * c_pow(g.r[j - 1], n) * kp(j - 1);
+        b[j] = -pre * c_pow(g.r[j], n) * (kp(j) + kp(j - 1));
+    }
+}
+
+// Thomas factors c' and m of a chi-free operator, formed exactly as TridiagSolver's loop forms them
+// (src/Tridiag.cpp:114-121)
+static void thomas_factors(const std::vector<Real>& a, const std::vector<Real>& b, const std::vector<Real>& c,
+                           std::vector<Real>& m, std::vector<Real>& cp) {
+    const size_t M = b.size();
+    m.assign(M, 0.0);
+    cp.assign(M, 0.0);
+    cp[0] = c[0] / b[0];
+    for (size_t i = 1; i < M; ++i) {
+        m[i] = Real(1) / (b[i] - a[i] * cp[i - 1]);
+        cp[i] = c[i] * m[i];
+    }
+}
+
+void FieldSolver::init(const RadialGrid& g, hipStream_t st) {   // :43-109, 157-176
+    const int N = g.N, M = N + 1;
+    {
+        std::vector<Real> aa(M, 0), bb(M, 0), cc(M, 0);
+        for (int j = 0; j < N; ++j) {
+            const Real invV = 2 * PI / g.V[j];
+            aa[j] = (j > 0) ? g.rmid[j - 1] / g.h[j - 1] * invV : 0.0;
+            cc[j] = g.rmid[j] / g.h[j] * invV;
+            bb[j] = -(aa[j] + cc[j]);
+        }
+        bb[N] = 1.0;
+        a[L0] = aa; b[L0] = bb; c[L0] = cc;
+    }
+    auto build_fv = [&](int n, bool flux, int op) {
+        std::vector<Real> aa, bb, cc;
+        fv_rows(g, n, aa, bb, cc);
+        auto kp = [&](int j) { return 1.0 / (g.h[j] * c_pow(g.rmid[j], 2 * n - 1)); };
+        if (flux) {
+            const Real dV = 0.5 * g.h[N - 1];
+            const Real k = kp(N - 1) / dV;
+            aa[N] = g.r[N - 1] * k;
+            bb[N] = -g.r[N] * k;
+            flux_coef = 1.0 / dV;
+        } else {
+            bb[N] = 1.0;
+        }
+        a[op] = aa; b[op] = bb; c[op] = cc;
+    };
+    build_fv(1, false, L1D);
+    build_fv(1, true, L1F);
+    build_fv(2, false, L2D);
+    gm.assign(M, 0); g0.assign(M, 0); gp.assign(M, 0); ir.assign(M, 0);
+    for (int j = 1; j < N; ++j) {
+        const Real hm = g.h[j - 1], hp = g.h[j];
+        const Real den = hm * hp * (hm + hp);
+        gp[j] = hm * hm / den;
+        gm[j] = -hp * hp / den;
+        g0[j] = (hp * hp - hm * hm) / den;
+    }
+    {
+        const Real h1 = g.h[N - 1], h2 = (N >= 2) ? g.h[N - 2] : g.h[N - 1];
+        const Real s = h1 + h2;
+        g0[N] = (2 * h1 + h2) / (h1 * s);
+        gm[N] = -s / (h1 * h2);
+        gp[N] = h1 / (h2 * s);
+    }
+    for (int j = 1; j <= N; ++j) ir[j] = 1.0 / g.r[j];
+    ir[0] = 1.0 / g.r[1];
+    // the chi-free solves (psi, E_z, B_z) prepare b unchanged: Dirichlet rows keep b0, the others b0 - 0 = b0
+    thomas_factors(a[L0], b[L0], c[L0], l0_m, l0_cp);
+    thomas_factors(a[L1D], b[L1D], c[L1D], l1_m, l1_cp);
+    for (int op = 0; op < 4; ++op) {
+        d_a[op].upload(a[op], st); d_b[op].upload(b[op], st); d_c[op].upload(c[op], st);
+    }
+    d_gm.upload(gm, st); d_g0.upload(g0, st); d_gp.upload(gp, st); d_ir.upload(ir, st);
+    d_l0_m.upload(l0_m, st); d_l0_cp.upload(l0_cp, st); d_l1_m.upload(l1_m, st); d_l1_cp.upload(l1_cp, st);
+}
+
+// ---------------------------------------------------------------------------- PlasmaSpecies (src/PlasmaSpecies.cpp)
+static void make_ab(int order, Real* c) {   // :14-28
+    for (int i = 0; i < 5; ++i) c[i] = 0;
+    switch (order) {
+        case 1: c[0] = 1; break;
+        case 2: c[0] = 1.5; c[1] = -0.5; break;
+        case 3: c[0] = 23. / 12; c[1] = -16. / 12; c[2] = 5. / 12; break;
+        case 4: c[0] = 55. / 24; c[1] = -59. / 24; c[2] = 37. / 24; c[3] = -9. / 24; break;
+        case 5:
+            c[0] = 1901. / 720; c[1] = -2774. / 720; c[2] = 2616. / 720; c[3] = -1274. / 720; c[4] = 251. / 720;
+            break;
+        default: throw std::runtime_error("pusher.ab_order must be 1..5");
+    }
+}
+
+void PlasmaSpecies::init(const Config& cfg, const std::string& nm, const RadialGrid& g, bool mode1) {   // :92-185
+    name = nm;
+    m1 = mode1;
+    q = cfg.get_double(nm + ".charge", -1.0);
+    m = cfg.get_double(nm + ".mass", 1.0);
+    mobile = cfg.get_bool(nm + ".mobile", true);
+    ppc = cfg.get_int(nm + ".ppc", 4);
+    ntheta = cfg.get_int(nm + ".ntheta", m1 ? 8 : 1);
+    uth = cfg.get_double(nm + ".uth", 0.0);
+    density = cfg.get_double(nm + ".density", 1.0);
+    rmax = cfg.get_double("plasma.rmax", 1e300);
+    if (cfg.get_string("plasma.radial", "uniform") != "uniform")
+        throw std::runtime_error("plasma.radial: only the uniform profile is on this path");
+    delta_min = cfg.get_double("pusher.delta_min", 1e-3);
+    max_qsa = cfg.get_double("pusher.max_qsa_factor", 35.0);
+    ab_order = cfg.get_int("pusher.ab_order", 3);
+    make_ab(ab_order, ab_c);
+    if (ppc < 1) throw std::runtime_error(nm + ".ppc must be >= 1");
+    if (ntheta < 1) throw std::runtime_error(nm + ".ntheta must be >= 1");
+    if (m1 && ntheta < 4) throw std::runtime_error(nm + ".ntheta must be >= 4 with azimuthal mode 1 (8 recommended)");
+    cap = std::max(g.N * ppc * ntheta, 1);
+    Np = cap;
+    x.alloc(cap); y.alloc(cap); px.alloc(cap); py.alloc(cap); dl.alloc(cap); w.alloc(cap);
+    hist.alloc(static_cast<size_t>(ab_order) * 5 * cap);
+}
+
+// rings at the sub-ring volume centroids of every cell (src/PlasmaSpecies.cpp:202-295)
+void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st) {
+    (void)z;
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> gauss(0.0, 1.0);
+    const double fz = 1.0 * density;
+    hx.assign(cap, 0); hy.assign(cap, 0); hpx.assign(cap, 0); hpy.assign(cap, 0); hdl.assign(cap, 0); hw.assign(cap, 0);
+    int i = 0;
+    for (int j = 0; j < g.N; ++j) {
+        const double hh = g.h[j];
+        for (int mm = 0; mm < ppc; ++mm) {
+            const double ra = g.r[j] + mm * hh / ppc, rb = g.r[j] + (mm + 1) * hh / ppc;
+            const double r = (2.0 / 3.0) * (rb * rb * rb - ra * ra * ra) / (rb * rb - ra * ra);
+            const double vring = fz * PI * (rb * rb - ra * ra);
+            const double wring = (r > rmax ? 0.0 : 1.0) * vring;
+            const double off = ((j * ppc + mm) % 2) * 0.5;
+            for (int l = 0; l < ntheta; ++l, ++i) {
+                const double th = 2 * PI * (l + off) / ntheta;
+                double ux = 0, uy = 0, uz = 0;
+                if (uth > 0) { ux = uth * gauss(rng); uy = uth * gauss(rng); uz = uth * gauss(rng); }
+                ux += 0; uy += 0; uz += 0;
+                const double gam = std::sqrt(1 + ux * ux + uy * uy + uz * uz);
+                const double X = (ntheta == 1) ? r : r * std::cos(th), Y = (ntheta == 1) ? 0.0 : r * std::sin(th);
+                const double W = wring / ntheta;
+                if (W == 0) { --i; continue; }
+                hx[i] = X; hy[i] = Y; hpx[i] = ux; hpy[i] = uy; hdl[i] = gam - uz; hw[i] = W;
+            }
+        }
+    }
+    Np = i;
+    x.upload(hx, st); y.upload(hy, st); px.upload(hpx, st); py.upload(hpy, st); dl.upload(hdl, st); w.upload(hw, st);
+    hip_check(hipMemsetAsync(hist.p, 0, sizeof(Real) * hist.n, st), "hist reset");
+}
+
+// host deposit of the loaded rings (background / neutralization), serial index order
+void PlasmaSpecies::deposit_rho(const RadialGrid& g, std::vector<Real>& rhot, std::vector<Real>& rho) const {
+    const int M = g.N + 1;
+    for (int i = 0; i < Np; ++i) {
+        const Real wi = hw[i];
+        if (wi == Real(0)) continue;
+        const Real qq = q * Real(1);
+        if (qq == Real(0)) continue;
+        const Real X = hx[i], Y = hy[i];
+        const Real r = std::sqrt(X * X + Y * Y);
+        Real c, s;
+        if (r > Real(0)) { c = X / r; s = Y / r; } else { c = Real(1); s = Real(0); }
+        int j;
+        if (r <= Real(0)) j = 0;
+        else if (r >= g.R) j = g.N - 1;
+        else {
+            int lo = 0, hi = g.N;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (g.r[mid] <= r) lo = mid; else hi = mid;
+            }
+            j = lo;
+        }
+        Real t = (r - g.r[j]) * g.inv_h[j];
+        t = t < Real(0) ? Real(0) : (t > Real(1) ? Real(1) : t);
+        const Real W[2] = {Real(1) - t, t};
+        for (int cc = 0; cc < 2; ++cc) {
+            const int jj = j + cc;
+            const Real a = wi * qq * W[cc] * g.inv_V[jj];
+            rhot[jj] += a;
+            rho[jj] += a;
+            if (m1) {
+                rhot[M + jj] += a * c;
+                rhot[2 * M + jj] += -a * s;
+                rho[M + jj] += a * c;
+                rho[2 * M + jj] += -a * s;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------- Beam (src/Beam.cpp)
+void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st) {   // :53-207, 483-570
+    name = nm;
+    m1 = mode1;
+    q = cfg.get_double(nm + ".charge", -1.0);
+    m = cfg.get_double(nm + ".mass", 1.0);
+    rigid = cfg.get_bool(nm + ".rigid", false);
+    const std::string pm = cfg.get_string(nm + ".pusher", cfg.get_string("pusher.beam", "vay"));
+    if (pm == "vay") pusher = 0;
+    else if (pm == "boris") pusher = 1;
+    else if (pm == "hc") pusher = 2;
+    else if (pm == "imp") pusher = 3;
+    else throw std::runtime_error("pusher.beam must be vay|boris|hc|imp on this path");
+    const std::string& p = nm;
+    const std::string lprof = cfg.get_string(p + ".profile", "gaussian");
+    if (lprof != "gaussian" && lprof != "flattop") throw std::runtime_error(p + ".profile must be gaussian|flattop on this path");
+    const double n0 = cfg.get_double(p + ".n0");
+    const double sr = cfg.get_double(p + ".sigma_r");
+    const double xi0 = xi_from_user(cfg.get_double(p + ".xi0"));
+    const double gamma0 = cfg.get_double(p + ".gamma");
+    const double espread = cfg.get_double(p + ".espread", 0.0);
+    const double emit = cfg.get_double(p + ".emit_n", 0.0);
+    const double cut = cfg.get_double(p + ".cut", 3.0);
+    const double cx0 = cfg.get_double(p + ".x0", 0.0), cy0 = cfg.get_double(p + ".y0", 0.0);
+    const double cxs = -cfg.get_double(p + ".x_slope", 0.0), cys = -cfg.get_double(p + ".y_slope", 0.0);
+    const double xp0 = cfg.get_double(p + ".xp0", 0.0), yp0 = cfg.get_double(p + ".yp0", 0.0);
+    if (!m1 && (cx0 != 0 || cy0 != 0 || cxs != 0 || cys != 0 || xp0 != 0 || yp0 != 0))
+        throw std::runtime_error(p + ": transverse offsets/tilts need azimuthal mode 1 (modes = 1)");
+    const int Ntot = cfg.get_int(p + ".nparticles", 100000);
+    const int nrot = cfg.get_int(p + ".nsym", 1);
+    const bool mirror = cfg.get_bool(p + ".mirror_y", false);
+    const int nsym = nrot * (mirror ? 2 : 1);
+    if (nrot < 1 || Ntot % nsym != 0) throw std::runtime_error(p + ".nparticles must be a multiple of nsym (x2 with mirror_y)");
+    const unsigned long long sd = cfg.get_int(p + ".seed", 12345);
+    const double xcut = cfg.has(p + ".xi_cut_head") ? xi_from_user(cfg.get_double(p + ".xi_cut_head")) : -1e300;
+    double sxi = 0, len = 0;
+    if (lprof == "gaussian") sxi = cfg.get_double(p + ".sigma_xi");
+    else len = cfg.get_double(p + ".length");
+    const double Pr = 1.0 - std::exp(-0.5 * cut * cut);
+    auto Phi = [](double v) { return 0.5 * (1.0 + std::erf(v / std::sqrt(2.0))); };
+    double wtot;
+    if (lprof == "gaussian") {
+        const double lo = std::max(-cut, (xcut - xi0) / sxi);
+        if (lo >= cut) throw std::runtime_error(p + ": xi_cut_head removes the whole beam");
+        wtot = n0 * c_pow(2 * PI, 1.5) * sr * sr * sxi * Pr * (Phi(cut) - Phi(lo));
+    } else {
+        const double lo = std::max(xi0 - 0.5 * len, xcut), hi = xi0 + 0.5 * len;
+        if (lo >= hi) throw std::runtime_error(p + ": xi_cut_head removes the whole beam");
+        wtot = n0 * 2 * PI * sr * sr * Pr * (hi - lo);
+    }
+    const double wmac = wtot / Ntot;
+    std::mt19937_64 rng(sd);
+    std::normal_distribution<double> N01(0.0, 1.0);
+    std::uniform_real_distribution<double> U01(0.0, 1.0);
+    const double spx = (sr > 0) ? emit / sr : 0.0;
+    double aw = 0, awg = 0, awxi = 0;
+    std::vector<double> all;
+    all.reserve(static_cast<size_t>(Ntot) * 7);
+    for (int i0 = 0; i0 < Ntot; i0 += nsym) {
+        double xx, yy;
+        do { xx = sr * N01(rng); yy = sr * N01(rng); } while (xx * xx + yy * yy > cut * cut * sr * sr);
+        double xv;
+        do {
+            if (lprof == "gaussian") {
+                do { xv = N01(rng); } while (std::abs(xv) > cut);
+                xv = xi0 + sxi * xv;
+            } else {
+                xv = xi0 + len * (U01(rng) - 0.5);
+            }
+        } while (xv < xcut);
+        double ux = spx * N01(rng), uy = spx * N01(rng);
+        const double gam = gamma0 * (1.0 + espread * N01(rng));
+        ux += gam * xp0;
+        uy += gam * yp0;
+        const double pz2 = gam * gam - 1.0 - ux * ux - uy * uy;
+        if (pz2 <= 0) throw std::runtime_error(p + ": gamma too small for the transverse momentum spread");
+        const double tpx = ux - gam * xp0, tpy = uy - gam * yp0;
+        for (int l = 0; l < nsym; ++l) {
+            const int lr = l % nrot;
+            const double sgn = (l >= nrot) ? -1.0 : 1.0;
+            const double ph = 2 * PI * lr / nrot, cp = std::cos(ph), sp = std::sin(ph);
+            const double rec[7] = {cp * xx - sp * yy + cx0 + cxs * (xv - xi0), sgn * (sp * xx + cp * yy) + cy0 + cys * (xv - xi0),
+                                   cp * tpx - sp * tpy + gam * xp0, sgn * (sp * tpx + cp * tpy) + gam * yp0,
+                                   std::sqrt(pz2), xv, wmac};
+            if (rec[6] != 0) {   // reference values of the moments (Beam::accumulate, :580-588)
+                const double gg = std::sqrt(1 + rec[2] * rec[2] + rec[3] * rec[3] + rec[4] * rec[4]);
+                aw += rec[6];
+                awg += rec[6] * gg;
+                awxi += rec[6] * rec[5];
+            }
+            all.insert(all.end(), rec, rec + 7);
+        }
+    }
+    set_particles(all, st);
+    gref = 0; xiref = 0;
+    if (aw > 0) { gref = awg / aw; xiref = awxi / aw; }
+}
+
+void Beam::set_particles(const std::vector<double>& p7, hipStream_t st) {   // :1323-1335
+    Np = static_cast<int>(p7.size() / 7);
+    std::vector<double> col(static_cast<size_t>(Np));
+    for (int c = 0; c < 7; ++c) {
+        for (int i = 0; i < Np; ++i) col[i] = p7[7 * static_cast<size_t>(i) + c];
+        a[c].alloc(static_cast<size_t>(std::max(Np, 1)));
+        if (Np) hip_check(hipMemcpyAsync(a[c].p, col.data(), 8 * static_cast<size_t>(Np), hipMemcpyHostToDevice, st), "beam upload");
+        hip_check(hipStreamSynchronize(st), "sync");
+    }
+}
+
+std::array<double, 17> Beam::sums(hipStream_t st) const {
+    std::array<double, 17> S{};
+    const qz_beam_soa s = soa();
+    qz_check(qz_beam_sums(Np, &s, gref, xiref, S.data(), st));
+    return S;
+}
+
+BeamDiag Beam::from_sums(const std::array<double, 17>& S, Real t) const {   // :1157-1192
+    BeamDiag d;
+    d.t = t;
+    d.alive = static_cast<long>(S[16]);
+    const double sw = S[0];
+    if (sw > 0) {
+        auto mo = [&](int c) { return S[c + 1] / sw; };
+        d.npart = sw;
+        d.charge = q * sw;
+        d.gamma_mean = gref + mo(0);
+        d.gamma_rms = std::sqrt(std::max(0.0, mo(1) - mo(0) * mo(0)));
+        d.xi_mean = xi_to_user(xiref + mo(2));
+        d.xi_rms = std::sqrt(std::max(0.0, mo(3) - mo(2) * mo(2)));
+        d.r_rms = std::sqrt(mo(4));
+        d.x_mean = mo(5);
+        d.y_mean = mo(6);
+        double xx = mo(7) - mo(5) * mo(5), yy = mo(8) - mo(6) * mo(6);
+        double pxx = mo(11) - mo(9) * mo(9), pyy = mo(12) - mo(10) * mo(10);
+        double xpx = mo(13) - mo(5) * mo(9), ypy = mo(14) - mo(6) * mo(10);
+        if (!m1) {   // m = 0: ring averages over the azimuth
+            d.x_mean = d.y_mean = 0;
+            xx = yy = 0.5 * mo(4);
+            pxx = pyy = 0.5 * (mo(11) + mo(12));
+            xpx = ypy = 0.5 * (mo(13) + mo(14));
+        }
+        d.sigma_x = std::sqrt(std::max(0.0, xx));
+        d.sigma_y = std::sqrt(std::max(0.0, yy));
+        d.emit_nx = std::sqrt(std::max(0.0, xx * pxx - xpx * xpx));
+        d.emit_ny = std::sqrt(std::max(0.0, yy * pyy - ypy * ypy));
+    }
+    return d;
+}
+
+// keys of this path: full names and the per-species / per-beam suffixes (the known-key rule of src/main.cpp:26-37)
+static bool known_key(const std::string& key) {
+    static const char* const full[] = {
+        "xi.convention", "modes", "solver.picard", "solver.tridiag", "plasma.smooth_length", "pusher.max_cells_per_step",
+        "grid.type", "grid.rmax", "grid.dr", "grid.regions", "grid.max_ratio", "grid.file", "xi.max", "xi.min",
+        "xi.step", "time.dt", "time.steps", "time.start", "time.first_step", "plasma.seed", "beams.xi_shape",
+        "plasma.species", "plasma.neutralize", "plasma.rmax", "plasma.radial", "beams", "pusher.beam",
+        "pusher.ab_order", "pusher.delta_min", "pusher.max_qsa_factor", "output.dir", "output.every",
+        "output.beam_every", "input.allow_unknown", "units.n0", "units.n0_cm3"};
+    static const char* const suffix[] = {
+        ".charge", ".mass", ".mobile", ".ppc", ".ntheta", ".uth", ".density", ".rigid", ".pusher", ".profile", ".n0",
+        ".sigma_r", ".xi0", ".gamma", ".espread", ".emit_n", ".cut", ".x0", ".y0", ".x_slope", ".y_slope", ".xp0",
+        ".yp0", ".nparticles", ".nsym", ".mirror_y", ".seed", ".xi_cut_head", ".sigma_xi", ".length"};
+    for (const char* f : full)
+        if (key == f) return true;
+    for (const char* s : suffix) {
+        const std::string sx(s);
+        if (key.size() > sx.size() && key.compare(key.size() - sx.size(), sx.size(), sx) == 0) return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------- Simulation (src/Simulation.cpp)
+Simulation::Simulation(const std::string& deck, const std::string& overrides) {   // :34-318 (the keys of this path)
+    Config cfg;
+    cfg.parse(deck);
+    cfg.overrides(overrides);
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+    if (!cfg.has("xi.convention"))
+        throw std::runtime_error("xi.convention is missing (QUARZ v2 decks state xi.convention = z-t)");
+    if (cfg.get_string("xi.convention") != "z-t") throw std::runtime_error("xi.convention: only z-t is supported");
+    const int modes = cfg.get_int("modes", 0);
+    if (modes != 0 && modes != 1) throw std::runtime_error("modes must be 0 (axisymmetric) or 1 (m = 0 and 1)");
+    m1 = (modes == 1);
+    picard = cfg.get_int("solver.picard", m1 ? 3 : 1);
+    (void)cfg.get_string("solver.tridiag", "auto");   // every method solves the same system; Thomas order here
+    if (cfg.get_double("plasma.smooth_length", 0.0) != 0.0) throw std::runtime_error("plasma.smooth_length is not on this path");
+    if (cfg.get_double("pusher.max_cells_per_step", 0.0) != 0.0) throw std::runtime_error("sub-slicing is not on this path");
+    {
+        const std::string type = cfg.get_string("grid.type", "uniform");
+        std::vector<Real> nodes;
+        if (type == "uniform") nodes = RadialGrid::make_uniform(cfg.get_double("grid.rmax"), cfg.get_double("grid.dr"));
+        else if (type == "regions" || type == "stretched")
+            nodes = RadialGrid::make_regions(cfg.get_double("grid.rmax"), cfg.get_pairs("grid.regions"),
+                                             cfg.get_double("grid.max_ratio", 1.05));
+        else if (type == "file") nodes = RadialGrid::read_file(cfg.get_string("grid.file"));
+        else throw std::runtime_error("grid: unknown grid.type '" + type + "' (uniform|regions|file)");
+        grid.finalize(nodes, stream);
+    }
+    solver.init(grid, stream);
+    const Real xu_max = cfg.get_double("xi.max", 0.0);
+    const Real xu_min = cfg.get_double("xi.min");
+    box.xi_min = xi_from_user(xu_max);
+    box.dxi = cfg.get_double("xi.step");
+    if (!(xu_max > xu_min) || !(box.dxi > 0)) throw std::runtime_error("xi: need xi.min < xi.max and xi.step > 0");
+    box.nxi = static_cast<int>(std::lround((xu_max - xu_min) / box.dxi)) + 1;
+    box.k0 = 0;
+    box.nloc = box.nxi;
+    dt = cfg.get_double("time.dt", 0.0);
+    nsteps = cfg.get_int("time.steps", 0);
+    t = cfg.get_double("time.start", 0.0);
+    if (dt <= 0 && nsteps > 0) throw std::runtime_error("time.dt must be > 0");
+    seed = cfg.get_int("plasma.seed", 1);
+    const std::string shape = cfg.get_string("beams.xi_shape", "linear");
+    if (shape == "ngp") box.ngp = 1;
+    else if (shape == "linear") box.ngp = 0;
+    else throw std::runtime_error("beams.xi_shape must be linear or ngp");
+    auto names = cfg.get_list("plasma.species");
+    if (names.empty()) names.push_back("electrons");
+    bool has_positive = false;
+    int nmobile = 0;
+    for (const auto& nm : names) {
+        species.push_back(std::make_unique<PlasmaSpecies>());
+        species.back()->init(cfg, nm, grid, m1);
+        if (species.back()->q > 0) has_positive = true;
+        if (species.back()->mobile) ++nmobile;
+    }
+    if (nmobile > QZ_MAX_SPECIES) throw std::runtime_error("at most 2 mobile plasma species on this path");
+    neutralize = cfg.get_bool("plasma.neutralize", !has_positive);
+    for (const auto& nm : cfg.get_list("beams")) {
+        beams.push_back(std::make_unique<Beam>());
+        beams.back()->init(cfg, nm, m1, stream);
+    }
+    outdir = cfg.get_string("output.dir", "");
+    out_every = cfg.get_int("output.every", 0);
+    (void)cfg.get_int("output.beam_every", 0);
+    M = grid.N + 1;
+    K = box.nloc;
+    NCF = m1 ? 18 : 6;
+    NCB = m1 ? 15 : 5;
+    NCD = m1 ? 9 : 3;
+    fld.alloc(static_cast<size_t>(K) * M * NCF);
+    bsrc.alloc(static_cast<size_t>(K) * M * NCB);
+    diag.alloc(static_cast<size_t>(K) * M * NCD);
+    hip_check(hipMemsetAsync(fld.p, 0, sizeof(Real) * fld.n, stream), "memset");
+    hip_check(hipMemsetAsync(diag.p, 0, sizeof(Real) * diag.n, stream), "memset");
+    bg.alloc(6 * static_cast<size_t>(M));
+    ws.alloc(qz_sweep_workspace_doubles(M));
+    lost.alloc(QZ_MAX_SPECIES);
+    counters.alloc(4);
+    int maxnp = 0;
+    for (const auto& b : beams) maxnp = std::max(maxnp, b->Np);
+    dep_work.alloc(qz_beam_deposit_workspace_bytes(std::max(maxnp, 1), K, grid.N));
+    step_n = cfg.get_int("time.first_step", 0);
+    // keys that were never read: a key of this path that the setup does not need is reported, any other key
+    // is an error before the first step (src/main.cpp:100-116)
+    {
+        const bool allow = cfg.get_bool("input.allow_unknown", false);
+        std::string unknown;
+        for (const auto& k : cfg.unused_keys()) {
+            if (known_key(k)) std::fprintf(stderr, "WARNING: input key '%s' was not used\n", k.c_str());
+            else unknown += " '" + k + "'";
+        }
+        if (!unknown.empty() && !allow)
+            throw std::runtime_error("unknown input key(s) or keys outside this path:" + unknown + " (misspelt?)");
+    }
+    hip_check(hipStreamSynchronize(stream), "setup");
+}
+
+Simulation::~Simulation() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+// src/Simulation.cpp:636-795 (one rank, no sub-slicing)
+void Simulation::step_fields() {
+    // ---- beams -> (xi, r) arrays
+    hip_check(hipEventRecord(ev[4], stream), "event");
+    hip_check(hipMemsetAsync(bsrc.p, 0, sizeof(Real) * bsrc.n, stream), "bsrc reset");
+    const qz_radial_grid g = grid.view();
+    hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * counters.n, stream), "counter reset");
+    for (const auto& b : beams) {
+        const qz_beam_soa s = b->soa();
+        qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 2, stream));
+        qz_check(qz_beam_deposit(&g, &box, m1 ? 1 : 0, b->q, b->Np, &s, bsrc.p, NCB, dep_work.p, stream));
+    }
+    hip_check(hipEventRecord(ev[5], stream), "event");
+    // ---- plasma at the head of the box, background
+    std::vector<Real> bgt(3 * static_cast<size_t>(M), 0.0), bgr(3 * static_cast<size_t>(M), 0.0);
+    const Real z_front = t - box.xi_min;
+    unsigned long long sd = seed;
+    for (auto& s : species) s->load(grid, z_front, sd++, stream);
+    for (auto& s : species)
+        if (!s->mobile) s->deposit_rho(grid, bgt, bgr);
+    if (neutralize) {
+        std::vector<Real> a(3 * static_cast<size_t>(M), 0.0), b(3 * static_cast<size_t>(M), 0.0);
+        for (auto& s : species)
+            if (s->mobile) s->deposit_rho(grid, a, b);
+        for (size_t j = 0; j < a.size(); ++j) { bgt[j] -= a[j]; bgr[j] -= b[j]; }
+    }
+    std::vector<Real> bgall(bgt);
+    bgall.insert(bgall.end(), bgr.begin(), bgr.end());
+    bg.upload(bgall, stream);
+    hip_check(hipMemsetAsync(lost.p, 0, sizeof(int) * lost.n, stream), "lost reset");
+    // ---- the sweep
+    qz_sweep_args A{};
+    A.grid = g;
+    A.modes = m1 ? 1 : 0;
+    A.picard = picard;
+    A.nslices = K;
+    A.ngp = box.ngp;
+    A.dxi = box.dxi;
+    int nsp = 0;
+    for (auto& s : species) {
+        if (!s->mobile) continue;
+        A.sp[nsp++] = s->state();
+        A.ab_order = s->ab_order;
+        for (int i = 0; i < 5; ++i) A.ab_c[i] = s->ab_c[i];
+        A.delta_min = s->delta_min;
+        A.max_qsa = s->max_qsa;
+    }
+    A.nspecies = nsp;
+    if (nsp == 0) {   // no mobile species: nothing is pushed, the weights are not used
+        make_ab(3, A.ab_c);
+        A.ab_order = 3;
+    }
+    for (int op = 0; op < 4; ++op) {
+        A.op_a[op] = solver.d_a[op].p; A.op_b[op] = solver.d_b[op].p; A.op_c[op] = solver.d_c[op].p;
+    }
+    A.flux_coef = solver.flux_coef;
+    A.grad_m = solver.d_gm.p; A.grad_0 = solver.d_g0.p; A.grad_p = solver.d_gp.p; A.inv_r = solver.d_ir.p;
+    A.fac_l0_m = solver.d_l0_m.p; A.fac_l0_cp = solver.d_l0_cp.p;
+    A.fac_l1_m = solver.d_l1_m.p; A.fac_l1_cp = solver.d_l1_cp.p;
+    A.bsrc = bsrc.p; A.ncb = NCB;
+    A.bg = bg.p;
+    A.fld = fld.p; A.ncf = NCF;
+    A.diag = diag.p; A.ncd = NCD;
+    A.lost = lost.p;
+    A.ws = ws.p;
+    A.stats = counters.p;
+    hip_check(hipEventRecord(ev[0], stream), "event");
+    qz_check(qz_sweep(&A, stream));
+    hip_check(hipEventRecord(ev[1], stream), "event");
+    std::vector<int> hl;
+    lost.download(hl, stream);
+    lost_last = 0;
+    for (int i = 0; i < nsp; ++i) lost_last += hl[i];
+    hip_check(hipEventElapsedTime(&ms_sweep, ev[0], ev[1]), "elapsed");
+    hip_check(hipEventElapsedTime(&ms_beam_dep, ev[4], ev[5]), "elapsed");
+    std::vector<unsigned long long> cnt;
+    counters.download(cnt, stream);
+    particle_slices = static_cast<double>(cnt[0]);
+    beam_deposited = static_cast<double>(cnt[2]);
+    swept = true;
+}
+
+// src/Simulation.cpp:1435-1456 (fixed dt)
+void Simulation::push_beams() {
+    const bool last = (step_n >= nsteps);
+    ms_beam = 0;
+    if (last) return;
+    if (!have_prev) { dt_prev = dt; have_prev = true; }
+    const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
+    const qz_radial_grid g = grid.view();
+    hip_check(hipMemsetAsync(counters.p + 1, 0, sizeof(unsigned long long), stream), "counter reset");
+    for (auto& b : beams)
+        if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 1, stream));
+    hip_check(hipEventRecord(ev[2], stream), "event");
+    for (auto& b : beams) {
+        if (b->rigid) continue;
+        const qz_beam_soa s = b->soa();
+        const Real qm = b->q / b->m;
+        if (!b->started) {   // leapfrog start-up (src/Beam.cpp:900-905)
+            qz_check(qz_beam_advance(&g, &box, m1 ? 1 : 0, qm, b->pusher, 0, b->Np, &s, fld.p, NCF, -0.5 * dkick,
+                                     -0.5 * dkick, 0, stream));
+            b->started = true;
+        }
+        qz_check(qz_beam_advance(&g, &box, m1 ? 1 : 0, qm, b->pusher, 0, b->Np, &s, fld.p, NCF, dkick, ddrift, 1, stream));
+    }
+    hip_check(hipEventRecord(ev[3], stream), "event");
+    hip_check(hipEventSynchronize(ev[3]), "sync");
+    hip_check(hipEventElapsedTime(&ms_beam, ev[2], ev[3]), "elapsed");
+    {
+        unsigned long long v = 0;
+        hip_check(hipMemcpy(&v, counters.p + 1, sizeof(v), hipMemcpyDeviceToHost), "counter");
+        beam_pushed = static_cast<double>(v);
+    }
+    t += dt;
+    dt_prev = dt;
+}
+
+void Simulation::step() {
+    if (done) throw std::runtime_error("the run has ended (time.steps reached)");
+    step_fields();
+    if (!outdir.empty() && out_every > 0 && step_n % out_every == 0) {
+        char nm[32];
+        std::snprintf(nm, sizeof(nm), "/fields_%06d.bin", step_n);
+        write_fields(outdir + nm);
+    }
+    const bool last = (step_n >= nsteps);
+    push_beams();
+    if (last) done = true;
+    ++step_n;
+}
+
+// fields_NNNNNN.bin version 3 (src/Simulation.cpp:873-993), all components, one rank
+void Simulation::write_fields(const std::string& fn) {
+    std::vector<Real> F, D, B;
+    fld.download(F, stream);
+    diag.download(D, stream);
+    bsrc.download(B, stream);
+    auto f = [&](int k, int j, int c) { return F[(static_cast<size_t>(k) * M + j) * NCF + c]; };
+    auto d = [&](int k, int j, int c) { return D[(static_cast<size_t>(k) * M + j) * NCD + c]; };
+    auto b = [&](int k, int j, int c) { return B[(static_cast<size_t>(k) * M + j) * NCB + c]; };
+    std::vector<std::pair<std::string, std::function<double(int, int)>>> comps;
+    auto add = [&](const std::string& n, std::function<double(int, int)> fn2) { comps.emplace_back(n, fn2); };
+    add("psi", [&](int k, int j) { return d(k, j, 0); });
+    add("ez", [&](int k, int j) { return f(k, j, 4); });
+    add("er", [&](int k, int j) { return f(k, j, 0); });
+    add("eth", [&](int k, int j) { return f(k, j, 1); });
+    add("br", [&](int k, int j) { return f(k, j, 2); });
+    add("bth", [&](int k, int j) { return f(k, j, 3); });
+    add("bz", [&](int k, int j) { return f(k, j, 5); });
+    add("ne", [&](int k, int j) { return d(k, j, 1); });
+    add("ni", [&](int k, int j) { return d(k, j, 2); });
+    add("rhob", [&](int k, int j) { return b(k, j, 4); });
+    if (m1) {
+        auto scalD = [&](const std::string& n, int cr, int ci) {
+            add(n + "_c", [&, cr](int k, int j) { return 2.0 * d(k, j, cr); });
+            add(n + "_s", [&, ci](int k, int j) { return -2.0 * d(k, j, ci); });
+        };
+        auto scalF = [&](const std::string& n, int cr, int ci) {
+            add(n + "_c", [&, cr](int k, int j) { return 2.0 * f(k, j, cr); });
+            add(n + "_s", [&, ci](int k, int j) { return -2.0 * f(k, j, ci); });
+        };
+        scalD("psi", 3, 4);
+        scalF("ez", 14, 15);
+        scalF("bz", 16, 17);
+        scalD("ne", 5, 6);
+        scalD("ni", 7, 8);
+        add("rhob_c", [&](int k, int j) { return 2.0 * b(k, j, 13); });
+        add("rhob_s", [&](int k, int j) { return -2.0 * b(k, j, 14); });
+        auto vec = [&](const std::string& nr, const std::string& nt, int c0r, int c0i, int c2r, int c2i) {
+            add(nr + "_c", [&, c0r, c2r](int k, int j) { return f(k, j, c0r) + f(k, j, c2r); });
+            add(nr + "_s", [&, c0i, c2i](int k, int j) { return f(k, j, c0i) - f(k, j, c2i); });
+            add(nt + "_c", [&, c0i, c2i](int k, int j) { return f(k, j, c0i) + f(k, j, c2i); });
+            add(nt + "_s", [&, c0r, c2r](int k, int j) { return f(k, j, c2r) - f(k, j, c0r); });
+        };
+        vec("er", "eth", 6, 7, 8, 9);
+        vec("br", "bth", 10, 11, 12, 13);
+    }
+    std::FILE* fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot write " + fn);
+    const int32_t hdr[4] = {3, M, K, m1 ? 1 : 0};
+    std::fwrite(hdr, 4, 4, fp);
+    std::fwrite(&t, 8, 1, fp);
+    std::fwrite(grid.r.data(), 8, M, fp);
+    for (int k = 0; k < K; ++k) {
+        const double xx = xi_to_user(box.xi_min + (K - 1 - k) * box.dxi);
+        std::fwrite(&xx, 8, 1, fp);
+    }
+    const int32_t nc = static_cast<int32_t>(comps.size());
+    std::fwrite(&nc, 4, 1, fp);
+    std::vector<double> row(static_cast<size_t>(M));
+    for (auto& c : comps) {
+        char name[16];
+        std::memset(name, 0, sizeof(name));
+        std::strncpy(name, c.first.c_str(), 15);
+        std::fwrite(name, 1, 16, fp);
+        for (int r = 0; r < K; ++r) {
+            const int k = K - 1 - r;
+            for (int j = 0; j < M; ++j) row[j] = c.second(k, j);
+            std::fwrite(row.data(), 8, M, fp);
+        }
+    }
+    std::fclose(fp);
+}
+
+}  // namespace qz
+
+// ============================================================================ qz_sim_* C-ABI
+struct qz_sim {
+    std::unique_ptr<qz::Simulation> s;
+};
+
+#define QZ_SIM_TRY(...)                                     \
+    try {                                                   \
+        __VA_ARGS__;                                        \
+        return QZ_OK;                                       \
+    } catch (const std::exception& e) {                     \
+        return qz_fail(QZ_EINVAL, e.what());                \
+    }
+
+int qz_fail(int code, const std::string& msg);
+
+extern "C" int qz_sim_create(const char* deck, const char* overrides, qz_sim** out) {
+    if (!deck || !out) return qz_fail(QZ_EINVAL, "qz_sim_create: null argument");
+    QZ_SIM_TRY({
+        auto* h = new qz_sim;
+        try {
+            h->s = std::make_unique<qz::Simulation>(deck, overrides ? overrides : "");
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    })
+}
+extern "C" void qz_sim_destroy(qz_sim* s) { delete s; }
+extern "C" int qz_sim_info(const qz_sim* h, int* info) {
+    if (!h || !info) return qz_fail(QZ_EINVAL, "qz_sim_info: null argument");
+    const auto& s = *h->s;
+    info[0] = s.M; info[1] = s.K; info[2] = s.m1 ? 1 : 0; info[3] = s.NCF; info[4] = s.NCB; info[5] = s.NCD;
+    info[6] = static_cast<int>(s.species.size()); info[7] = static_cast<int>(s.beams.size());
+    info[8] = s.nsteps; info[9] = s.step_n;
+    return QZ_OK;
+}
+extern "C" int qz_sim_step(qz_sim* h) { QZ_SIM_TRY(h->s->step()) }
+extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
+extern "C" int qz_sim_push_beams(qz_sim* h) {
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        const bool last = (s.step_n >= s.nsteps);
+        s.push_beams();
+        if (last) s.done = true;
+        ++s.step_n;
+    })
+}
+extern "C" long qz_sim_get(qz_sim* h, const char* what, double* out, long cap) {
+    try {
+        auto& s = *h->s;
+        const std::string w(what);
+        std::vector<double> v;
+        if (w == "fld") s.fld.download(v, s.stream);
+        else if (w == "diag") s.diag.download(v, s.stream);
+        else if (w == "bsrc") s.bsrc.download(v, s.stream);
+        else if (w == "nodes") v = s.grid.r;
+        else if (w.rfind("beam", 0) == 0) {
+            const int b = std::atoi(w.c_str() + 4);
+            const std::string c = w.substr(w.find('.') + 1);
+            if (b < 0 || b >= static_cast<int>(s.beams.size())) return -1;
+            static const char* names[7] = {"x", "y", "px", "py", "pz", "xi", "w"};
+            int ci = -1;
+            for (int i = 0; i < 7; ++i) if (c == names[i]) ci = i;
+            if (ci < 0) return -1;
+            s.beams[b]->a[ci].download(v, s.stream);
+            v.resize(static_cast<size_t>(s.beams[b]->Np));
+        } else if (w.rfind("sp", 0) == 0) {
+            const int b = std::atoi(w.c_str() + 2);
+            const std::string c = w.substr(w.find('.') + 1);
+            if (b < 0 || b >= static_cast<int>(s.species.size())) return -1;
+            auto& S = *s.species[b];
+            const qz::DevBuf<double>* src = c == "x" ? &S.x : c == "y" ? &S.y : c == "px" ? &S.px : c == "py" ? &S.py
+                                          : c == "dl" ? &S.dl : c == "w" ? &S.w : nullptr;
+            if (!src) return -1;
+            src->download(v, s.stream);
+            v.resize(static_cast<size_t>(S.Np));
+        } else {
+            return -1;
+        }
+        const long n = static_cast<long>(v.size());
+        if (out && cap >= n) std::copy(v.begin(), v.end(), out);
+        return n;
+    } catch (const std::exception& e) {
+        qz_fail(QZ_EINVAL, e.what());
+        return -1;
+    }
+}
+extern "C" int qz_sim_beam_sums(qz_sim* h, int b, double* out17, double* refs) {
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
+        const auto S = s.beams[b]->sums(s.stream);
+        std::copy(S.begin(), S.end(), out17);
+        if (refs) { refs[0] = s.beams[b]->gref; refs[1] = s.beams[b]->xiref; }
+    })
+}
+extern "C" int qz_sim_beam_diag(qz_sim* h, int b, double* out15) {
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
+        const qz::BeamDiag d = s.beams[b]->from_sums(s.beams[b]->sums(s.stream), s.t);
+        const double v[15] = {d.t, d.npart, d.charge, d.gamma_mean, d.gamma_rms, d.xi_mean, d.xi_rms, d.r_rms,
+                              d.emit_nx, d.x_mean, d.y_mean, d.emit_ny, d.sigma_x, d.sigma_y, double(d.alive)};
+        std::copy(v, v + 15, out15);
+    })
+}
+extern "C" int qz_sim_timing(const qz_sim* h, double* out2) {
+    if (!h || !out2) return qz_fail(QZ_EINVAL, "qz_sim_timing: null argument");
+    out2[0] = h->s->ms_sweep;
+    out2[1] = h->s->ms_beam;
+    return QZ_OK;
+}
+extern "C" int qz_sim_stats(const qz_sim* h, double* out6) {
+    if (!h || !out6) return qz_fail(QZ_EINVAL, "qz_sim_stats: null argument");
+    const auto& s = *h->s;
+    out6[0] = s.particle_slices;
+    out6[1] = s.beam_pushed;
+    out6[2] = s.beam_deposited;
+    out6[3] = s.ms_sweep;
+    out6[4] = s.ms_beam_dep;
+    out6[5] = s.ms_beam;
+    return QZ_OK;
+}
+extern "C" long qz_sim_lost(const qz_sim* h) { return h->s->lost_last; }
+extern "C" double qz_sim_time(const qz_sim* h) { return h->s->t; }
+extern "C" int qz_sim_write_fields(qz_sim* h, const char* path) { QZ_SIM_TRY(h->s->write_fields(path)) }

@@ -1,0 +1,153 @@
+// Host side of the quasi-static step: the classes of the reference's interface (RadialGrid, FieldSolver,
+// PlasmaSpecies, Beam, Simulation; src/*.hpp), with device arrays in HBM and the hot path in the gfx950
+// kernels behind the C-ABI of include/quarz_amd.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/quarz_amd.h"
+#include "config.hpp"
+#include "../qz_internal.hpp"
+
+namespace qz {
+
+using Real = double;
+
+// ---------------------------------------------------------------------------- device buffer
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count);
+    void release();
+    void upload(const std::vector<T>& h, hipStream_t st);
+    void download(std::vector<T>& h, hipStream_t st) const;
+};
+
+// ---------------------------------------------------------------------------- RadialGrid (src/RadialGrid.*)
+struct RadialGrid {
+    int N = 0;
+    Real R = 0;
+    std::vector<Real> r, h, rmid, V, inv_h, inv_V;
+    DevBuf<Real> d_r, d_inv_h, d_inv_V;
+
+    static std::vector<Real> make_uniform(Real R, Real dr);
+    static std::vector<Real> make_regions(Real R, const std::vector<std::pair<double, double>>& regions, Real max_ratio);
+    static std::vector<Real> read_file(const std::string& filename);
+    void finalize(const std::vector<Real>& nodes, hipStream_t st);
+    qz_radial_grid view() const { return {d_r.p, d_inv_h.p, d_inv_V.p, N, R}; }
+};
+
+// ---------------------------------------------------------------------------- FieldSolver (src/FieldSolver.*)
+struct FieldSolver {
+    enum { L0 = 0, L1D = 1, L1F = 2, L2D = 3 };
+    std::vector<Real> a[4], b[4], c[4];
+    Real flux_coef = 0;
+    std::vector<Real> gm, g0, gp, ir;
+    std::vector<Real> l0_m, l0_cp, l1_m, l1_cp;   // Thomas factors of the chi-free L0 and L1D
+    DevBuf<Real> d_a[4], d_b[4], d_c[4], d_gm, d_g0, d_gp, d_ir, d_l0_m, d_l0_cp, d_l1_m, d_l1_cp;
+    void init(const RadialGrid& g, hipStream_t st);
+};
+
+// ---------------------------------------------------------------------------- PlasmaSpecies (src/PlasmaSpecies.*)
+struct PlasmaSpecies {
+    std::string name;
+    bool m1 = false;
+    Real q = -1, m = 1;
+    bool mobile = true;
+    int ppc = 4, ntheta = 1;
+    Real uth = 0, density = 1, rmax = 1e300;
+    Real delta_min = 1e-3, max_qsa = 35;
+    int ab_order = 3;
+    Real ab_c[5] = {0, 0, 0, 0, 0};
+    int Np = 0, cap = 0;
+    std::vector<Real> hx, hy, hpx, hpy, hdl, hw;   // host copy of the last load
+    DevBuf<Real> x, y, px, py, dl, w, hist;
+
+    void init(const Config& cfg, const std::string& nm, const RadialGrid& g, bool mode1);
+    void load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st);
+    // rho - J_z and rho of the loaded particles, mode arrays (3 M each), host (src/PlasmaSpecies.cpp:480-506)
+    void deposit_rho(const RadialGrid& g, std::vector<Real>& rhot, std::vector<Real>& rho) const;
+    qz_species_state state() const {
+        return {x.p, y.p, px.p, py.p, dl.p, w.p, hist.p, Np, cap, q, m};
+    }
+};
+
+// ---------------------------------------------------------------------------- Beam (src/Beam.*)
+struct BeamDiag {
+    double t = 0, npart = 0, charge = 0, gamma_mean = 0, gamma_rms = 0, xi_mean = 0, xi_rms = 0;
+    double r_rms = 0, emit_nx = 0, emit_ny = 0, x_mean = 0, y_mean = 0, sigma_x = 0, sigma_y = 0;
+    long alive = 0;
+};
+
+struct Beam {
+    std::string name;
+    bool m1 = false;
+    Real q = -1, m = 1;
+    bool rigid = false;
+    int pusher = 0;
+    bool started = false;
+    double gref = 0, xiref = 0;
+    int Np = 0;
+    DevBuf<Real> a[7];   // x y px py pz xi w
+    void init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st);
+    void set_particles(const std::vector<double>& p7, hipStream_t st);
+    qz_beam_soa soa() const { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a[6].p}; }
+    std::array<double, 17> sums(hipStream_t st) const;
+    BeamDiag from_sums(const std::array<double, 17>& S, Real t) const;
+};
+
+// ---------------------------------------------------------------------------- Simulation (src/Simulation.*)
+class Simulation {
+public:
+    Simulation(const std::string& deck, const std::string& overrides);
+    ~Simulation();
+
+    void step_fields();      // the quasi-static sweep of the current step n
+    void push_beams();       // the beam push of step n (not on the last step)
+    void step();             // one iteration of Simulation::run
+    void write_fields(const std::string& path);
+
+    bool m1 = false;
+    int picard = 1;
+    RadialGrid grid;
+    FieldSolver solver;
+    qz_beam_grid box{};
+    Real dt = 0, t = 0;
+    int nsteps = 0, step_n = 0;
+    bool done = false, have_prev = false, swept = false;
+    double dt_prev = 0;
+    unsigned long long seed = 1;
+    bool neutralize = true;
+    std::vector<std::unique_ptr<PlasmaSpecies>> species;
+    std::vector<std::unique_ptr<Beam>> beams;
+    int M = 0, K = 0, NCF = 0, NCB = 0, NCD = 0;
+    DevBuf<Real> fld, bsrc, diag, bg, ws;
+    DevBuf<int> lost;
+    DevBuf<unsigned long long> counters;   // [0] particle-slices of the sweep, [1] beam particles pushed, [2] deposited
+    DevBuf<char> dep_work;
+    long lost_last = 0;
+    float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
+    double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    std::string outdir;
+    int out_every = 0;
+};
+
+}  // namespace qz

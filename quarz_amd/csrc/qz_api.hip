@@ -1,0 +1,122 @@
+// C-ABI glue: error reporting, device information, the batched Thomas solver and the sweep entry points.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "qz_device.hpp"
+#include "qz_internal.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int qz_fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+int qz_fail_hip(hipError_t e, const std::string& where) {
+    g_last_error = where + ": " + hipGetErrorString(e);
+    return QZ_EHIP;
+}
+
+extern "C" const char* qz_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" int qz_device_info(char* arch, int arch_len, int* n_cu) {
+    int dev = 0;
+    QZ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    QZ_HIP(hipGetDeviceProperties(&prop, dev));
+    if (arch && arch_len > 0) {
+        std::strncpy(arch, prop.gcnArchName, static_cast<size_t>(arch_len) - 1);
+        arch[arch_len - 1] = 0;
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    return QZ_OK;
+}
+
+namespace qz {
+// Thomas, one thread per system (src/Tridiag.cpp:111-126); c' and d' kept in x and a scratch-free way:
+// the forward sweep stores d' in x and c' is recomputed in the backward sweep from the same operands,
+// so no workspace is needed and every value is formed exactly as in the reference.
+__global__ void thomas_batched_kernel(const Real* __restrict__ a, const Real* __restrict__ b, const Real* __restrict__ c,
+                                      const Real* __restrict__ d, Real* __restrict__ x, Real* __restrict__ cpw, int M,
+                                      int batch) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= batch) return;
+    const size_t o = static_cast<size_t>(s) * M;
+    const Real *A = a + o, *B = b + o, *C = c + o, *D = d + o;
+    Real* X = x + o;
+    Real* CP = cpw + o;
+    Real cp = C[0] / B[0];
+    Real dp = D[0] / B[0];
+    CP[0] = cp;
+    X[0] = dp;
+    for (int i = 1; i < M; ++i) {
+        const Real m = Real(1) / (B[i] - A[i] * cp);
+        cp = C[i] * m;
+        dp = (D[i] - A[i] * dp) * m;
+        CP[i] = cp;
+        X[i] = dp;
+    }
+    Real xn = X[M - 1];
+    for (int i = M - 2; i >= 0; --i) {
+        xn = X[i] - CP[i] * xn;
+        X[i] = xn;
+    }
+}
+}  // namespace qz
+
+extern "C" int qz_tridiag_thomas_batched(const double* a, const double* b, const double* c, const double* d, double* x,
+                                         int M, int batch, void* stream) {
+    if (!a || !b || !c || !d || !x) return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: null argument");
+    if (M < 1 || batch < 0) return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: bad extent");
+    if (batch == 0) return QZ_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    double* cp = nullptr;
+    QZ_HIP(hipMallocAsync(reinterpret_cast<void**>(&cp), sizeof(double) * static_cast<size_t>(M) * batch, st));
+    const int tb = 64;
+    hipLaunchKernelGGL(qz::thomas_batched_kernel, dim3((batch + tb - 1) / tb), dim3(tb), 0, st, a, b, c, d, x, cp, M, batch);
+    hipError_t e = hipGetLastError();
+    QZ_HIP(hipFreeAsync(cp, st));
+    if (e != hipSuccess) return qz_fail_hip(e, "thomas launch");
+    return QZ_OK;
+}
+
+extern "C" size_t qz_sweep_workspace_doubles(int M) { return qz_sweep_workspace_doubles_impl(M); }
+
+static int validate_sweep(const qz_sweep_args& a) {
+    if (a.modes != 0 && a.modes != 1) return qz_fail(QZ_EINVAL, "sweep: modes must be 0 or 1");
+    if (a.grid.N < 2 || a.grid.N > 2048) return qz_fail(QZ_ERANGE, "sweep: need 2 <= N <= 2048 radial cells");
+    if (a.nslices < 1) return qz_fail(QZ_EINVAL, "sweep: nslices < 1");
+    if (a.ab_order < 1 || a.ab_order > 5) return qz_fail(QZ_EINVAL, "sweep: ab_order must be 1..5");
+    if (a.nspecies < 0 || a.nspecies > QZ_MAX_SPECIES) return qz_fail(QZ_ERANGE, "sweep: at most 2 mobile species");
+    if (a.ncf != (a.modes ? 18 : 6) || a.ncb != (a.modes ? 15 : 5) || a.ncd < (a.modes ? 9 : 3))
+        return qz_fail(QZ_EINVAL, "sweep: component counts do not match modes");
+    if (!a.ws || !a.fld || !a.diag || !a.bsrc || !a.bg || !a.lost) return qz_fail(QZ_EINVAL, "sweep: null array");
+    for (int s = 0; s < a.nspecies; ++s)
+        if (a.sp[s].np < 0 || a.sp[s].np > a.sp[s].cap || !a.sp[s].hist) return qz_fail(QZ_EINVAL, "sweep: bad species");
+    return QZ_OK;
+}
+
+static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
+    if (!runs || n < 1) return qz_fail(QZ_EINVAL, "sweep: no runs");
+    for (int i = 0; i < n; ++i) {
+        const int rc = validate_sweep(runs[i]);
+        if (rc) return rc;
+    }
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    qz_sweep_args* d = nullptr;
+    QZ_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(qz_sweep_args) * n, st));
+    QZ_HIP(hipMemcpyAsync(d, runs, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st));
+    const int rc = qz_sweep_launch(d, runs, n, st);
+    QZ_HIP(hipFreeAsync(d, st));
+    return rc;
+}
+
+extern "C" int qz_sweep(const qz_sweep_args* args, void* stream) { return sweep_many(args, 1, stream); }
+extern "C" int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream) {
+    return sweep_many(runs, nruns, stream);
+}

@@ -1,0 +1,392 @@
+// Beam kernels of the quasi-static step on gfx950: the push against the stored (xi, r, mode) fields
+// (Beam::advance, src/Beam.cpp:948-1131) and the deposit of rho - J_z, J_z, J+ and rho
+// (Beam::deposit, src/Beam.cpp:725-785).
+//
+// The deposit is deterministic and equals the reference's serial atomic_add order: particles are
+// bucketed by their (xi, r) cell (counting sort, each bucket then ordered by particle index), and one
+// thread per (slice, node) merges the index-ordered buckets of the four cells that touch its node and
+// adds their contributions sequentially, in particle-index order.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "qz_device.hpp"
+#include "qz_internal.hpp"
+
+namespace qz {
+
+// per-particle record of the deposit (the node-independent factors of src/Beam.cpp:738-757)
+enum BRec : int { R_WQ = 0, R_TK, R_T, R_OMVZ, R_VZ, R_VR, R_VT, R_C, R_S, R_VX, R_VY, R_N };
+
+__global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, Real qm, int mode, int i0, int np,
+                                    qz_beam_soa P, const Real* __restrict__ fld, int nc, Real dt_kick, Real dt,
+                                    int move) {
+    const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (P.w[i] == Real(0)) return;
+    const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
+    const Real xmax = bg.xi_min + (bg.nxi - 1) * bg.dxi;
+    const int k0 = bg.k0, nloc = bg.nloc;
+    const bool ngp = bg.ngp != 0;
+    const Real R = g.R;
+    const int M = g.N + 1;
+    const Real X = P.x[i], Y = P.y[i];
+    const Real r = dsqrt(X * X + Y * Y);
+    const Real c = r > Real(0) ? X / r : Real(1), sn = r > Real(0) ? Y / r : Real(0);
+    Real s = (P.xi[i] - xmin) * idxi - k0;
+    int k;
+    Real tk;
+    if (ngp) {
+        k = static_cast<int>(floor(s + Real(0.5)));
+        k = k < 0 ? 0 : (k > nloc - 1 ? nloc - 1 : k);
+        tk = Real(0);
+    } else {
+        k = static_cast<int>(floor(s));
+        if (k < 0) { k = 0; s = 0; }
+        if (k > nloc - 2) { k = nloc - 2; s = nloc - 1; }
+        tk = s - k;
+    }
+    const int k1 = ngp ? k : k + 1;
+    int j; Real t;
+    grid_weights(g.r, g.inv_h, g.N, R, r, j, t);
+    const Real* F0 = fld + (static_cast<size_t>(k) * M + j) * nc;
+    const Real* F1 = fld + (static_cast<size_t>(k1) * M + j) * nc;
+    auto bil = [&](int comp) {
+        return (Real(1) - tk) * ((Real(1) - t) * F0[comp] + t * F0[nc + comp]) +
+               tk * ((Real(1) - t) * F1[comp] + t * F1[nc + comp]);
+    };
+    Real Ex, Ey, Bx, By, Ez, Bz;
+    if (m1) {
+        eval_vector(bil(F_EP0R), bil(F_EP0I), bil(F_EP1R), bil(F_EP1I), bil(F_EP2R), bil(F_EP2I), c, sn, true, Ex, Ey);
+        eval_vector(bil(F_BP0R), bil(F_BP0I), bil(F_BP1R), bil(F_BP1I), bil(F_BP2R), bil(F_BP2I), c, sn, true, Bx, By);
+        Ez = eval_scalar(bil(F_EZ0), bil(F_EZ1R), bil(F_EZ1I), c, sn, true);
+        Bz = eval_scalar(bil(F_BZ0), bil(F_BZ1R), bil(F_BZ1I), c, sn, true);
+    } else {
+        eval_vector(Real(0), Real(0), bil(F_EP1R), bil(F_EP1I), Real(0), Real(0), c, sn, false, Ex, Ey);
+        eval_vector(Real(0), Real(0), bil(F_BP1R), bil(F_BP1I), Real(0), Real(0), c, sn, false, Bx, By);
+        Ez = bil(F_EZ0);
+        Bz = bil(F_BZ0);
+    }
+    const V3 E{Ex, Ey, Ez}, B{Bx, By, Bz};
+    V3 u{P.px[i], P.py[i], P.pz[i]};
+    const Real h = Real(0.5) * qm * dt_kick;
+    const V3 Fp{Real(0), Real(0), Real(0)};   // no laser on this path
+    if (!move) {
+        const Real g0 = dsqrt(Real(1) + dot(u, u));
+        u = u + (Real(2) * h) * (E + cross((Real(1) / g0) * u, B)) + dt_kick * Fp;
+        P.px[i] = u.x; P.py[i] = u.y; P.pz[i] = u.z;
+        return;
+    }
+    u = u + (Real(0.5) * dt_kick) * Fp;
+    u = push_u(mode, u, E, B, h);
+    u = u + (Real(0.5) * dt_kick) * Fp;
+    const Real p2 = u.x * u.x + u.y * u.y;
+    const Real gam = dsqrt(Real(1) + p2 + u.z * u.z);
+    const Real omvz = (Real(1) + p2) / (gam * (gam + u.z));
+    const Real xn = X + u.x / gam * dt, yn = Y + u.y / gam * dt;
+    P.x[i] = xn; P.y[i] = yn;
+    P.px[i] = u.x; P.py[i] = u.y; P.pz[i] = u.z;
+    const Real xin = P.xi[i] + omvz * dt;
+    P.xi[i] = xin;
+    if (xn * xn + yn * yn >= R * R || xin < xmin || xin > xmax) P.w[i] = Real(0);
+}
+
+// live particles (w != 0) of a beam, added to *count
+__global__ void beam_count_live_kernel(int np, const Real* __restrict__ w, unsigned long long* __restrict__ count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long live = (i < np && w[i] != Real(0)) ? 1ull : 0ull;
+    unsigned long long s = live;
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(count, s);
+}
+
+// deposit step 1: cell key, per-particle record and bucket counts
+__global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int* __restrict__ key,
+                                Real* __restrict__ rec, int* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    key[i] = -1;
+    const Real wi = P.w[i];
+    if (wi == Real(0)) return;
+    const Real X = P.x[i], Y = P.y[i];
+    const Real r = dsqrt(X * X + Y * Y);
+    if (r > g.R) return;
+    const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
+    const int k0 = bg.k0, nloc = bg.nloc;
+    const bool ngp = bg.ngp != 0;
+    const Real c = r > Real(0) ? X / r : Real(1), s = r > Real(0) ? Y / r : Real(0);
+    const Real sx = (P.xi[i] - xmin) * idxi;
+    const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
+    if (k < k0 - 1 || k >= k0 + nloc) return;
+    const Real tk = ngp ? Real(0) : sx - k;
+    const Real px = P.px[i], py = P.py[i], pz = P.pz[i];
+    const Real p2 = px * px + py * py;
+    const Real gam = dsqrt(Real(1) + p2 + pz * pz);
+    const Real omvz = (Real(1) + p2) / (gam * (gam + pz));
+    const Real vz = pz / gam, vx = px / gam, vy = py / gam;
+    const Real vr = c * vx + s * vy, vt = c * vy - s * vx;
+    int j; Real t;
+    grid_weights(g.r, g.inv_h, g.N, g.R, r, j, t);
+    const int kl = k - k0;   // local slice, -1 .. nloc-1
+    const int kk = (kl + 1) * g.N + j;
+    key[i] = kk;
+    Real* rr = rec + static_cast<size_t>(i) * R_N;
+    rr[R_WQ] = wi * q; rr[R_TK] = tk; rr[R_T] = t; rr[R_OMVZ] = omvz; rr[R_VZ] = vz; rr[R_VR] = vr; rr[R_VT] = vt;
+    rr[R_C] = c; rr[R_S] = s; rr[R_VX] = vx; rr[R_VY] = vy;
+    atomicAdd(&counts[kk], 1);
+}
+
+// deposit step 2: scatter the particle indices into their buckets (order fixed in step 3)
+__global__ void beam_scatter_kernel(int np, const int* __restrict__ key, const int* __restrict__ start,
+                                    int* __restrict__ fill, int* __restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const int kk = key[i];
+    if (kk < 0) return;
+    const int pos = start[kk] + atomicAdd(&fill[kk], 1);
+    list[pos] = i;
+}
+
+// deposit step 3: order every bucket by particle index
+__global__ void beam_bucket_sort_kernel(int ncells, const int* __restrict__ start, int* __restrict__ list) {
+    const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cidx >= ncells) return;
+    const int lo = start[cidx], hi = start[cidx + 1];
+    for (int a = lo + 1; a < hi; ++a) {
+        const int v = list[a];
+        int b = a - 1;
+        while (b >= lo && list[b] > v) { list[b + 1] = list[b]; --b; }
+        list[b + 1] = v;
+    }
+}
+
+// deposit step 4: one thread per (local slice, node): the four buckets touching the node, merged by index
+__global__ void beam_node_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, const int* __restrict__ start,
+                                 const int* __restrict__ list, const Real* __restrict__ rec, Real* __restrict__ src, int nc) {
+    const int M = g.N + 1, N = g.N;
+    const long node = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (node >= static_cast<long>(bg.nloc) * M) return;
+    const int kk = static_cast<int>(node / M), jj = static_cast<int>(node % M);
+    const bool ngp = bg.ngp != 0;
+    const Real idxi = Real(1) / bg.dxi;
+    // buckets (a = slice offset, b = node offset of the particle's corner): cell (kk - a, jj - b)
+    int lo[4], hi[4], ca[4], cb[4];
+    int nb = 0;
+    for (int a = 0; a < (ngp ? 1 : 2); ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int kc = kk - a, jc = jj - b;
+            if (jc < 0 || jc >= N) continue;
+            const int key = (kc + 1) * N + jc;   // kc >= -1
+            lo[nb] = start[key]; hi[nb] = start[key + 1]; ca[nb] = a; cb[nb] = b;
+            if (lo[nb] < hi[nb]) ++nb;
+        }
+    if (nb == 0) return;
+    Real* S = src + static_cast<size_t>(node) * nc;
+    Real acc[15];
+    for (int c = 0; c < nc; ++c) acc[c] = S[c];
+    const Real iv = g.inv_V[jj];
+    while (true) {
+        // next particle in index order among the buckets
+        int best = -1, bi = 0x7fffffff;
+        for (int q = 0; q < nb; ++q)
+            if (lo[q] < hi[q]) {
+                const int v = list[lo[q]];
+                if (v < bi) { bi = v; best = q; }
+            }
+        if (best < 0) break;
+        ++lo[best];
+        const Real* rr = rec + static_cast<size_t>(bi) * R_N;
+        const Real tk = rr[R_TK], t = rr[R_T];
+        const Real Wk = ca[best] == 0 ? Real(1) - tk : tk;
+        const Real Wr = cb[best] == 0 ? Real(1) - t : t;
+        const Real d = rr[R_WQ] * Wk * Wr * iv * idxi;
+        const Real omvz = rr[R_OMVZ], vz = rr[R_VZ];
+        acc[B_RT0] += d * omvz;
+        acc[B_JZ0] += d * vz;
+        acc[B_JP1R] += d * rr[R_VR];
+        acc[B_JP1I] += d * rr[R_VT];
+        acc[B_RHO0] += d;
+        if (m1) {
+            const Real c = rr[R_C], s = rr[R_S], vx = rr[R_VX], vy = rr[R_VY];
+            acc[B_RT1R] += d * omvz * c;
+            acc[B_RT1I] += -d * omvz * s;
+            acc[B_JZ1R] += d * vz * c;
+            acc[B_JZ1I] += -d * vz * s;
+            acc[B_RHO1R] += d * c;
+            acc[B_RHO1I] += -d * s;
+            acc[B_JP0R] += d * vx;
+            acc[B_JP0I] += d * vy;
+            const Real c2 = c * c - s * s, s2 = Real(2) * c * s;
+            acc[B_JP2R] += d * (vx * c2 + vy * s2);
+            acc[B_JP2I] += d * (vy * c2 - vx * s2);
+        }
+    }
+    for (int c = 0; c < nc; ++c) S[c] = acc[c];
+}
+
+// ---- exclusive scan of n ints (counts -> starts, start[n] = total): per-block sums, one-block scan, add
+__global__ void scan_block_sums(const int* __restrict__ in, int n, int* __restrict__ bsum) {
+    __shared__ int sh[1024];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    sh[threadIdx.x] = i < n ? in[i] : 0;
+    __syncthreads();
+    for (int d = 512; d > 0; d >>= 1) {
+        if (threadIdx.x < d) sh[threadIdx.x] += sh[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = sh[0];
+}
+__global__ void scan_sums_serial(int* __restrict__ bsum, int nb, int* __restrict__ total) {
+    // one block: exclusive scan of the block sums by 1024 threads in chunks
+    __shared__ int sh[1024];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int d = 1; d < 1024; d <<= 1) {
+            const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + sh[threadIdx.x] - v;
+        const int blocktotal = sh[1023];
+        __syncthreads();
+        carry += blocktotal;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+__global__ void scan_finish(const int* __restrict__ in, int n, const int* __restrict__ bsum, int* __restrict__ out) {
+    __shared__ int sh[1024];
+    const int i = blockIdx.x * 1024 + threadIdx.x;
+    const int v = i < n ? in[i] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    if (i < n) out[i] = bsum[blockIdx.x] + sh[threadIdx.x] - v;
+}
+
+}  // namespace qz
+
+using namespace qz;
+
+namespace {
+struct DepWork {   // layout of the deposit workspace
+    size_t key, rec, counts, start, fill, list, bsum, total, bytes;
+};
+DepWork dep_layout(int np, int nloc, int N) {
+    auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t nc = static_cast<size_t>(nloc + 1) * N;
+    const size_t nb = (nc + 1 + 1023) / 1024;
+    DepWork d;
+    size_t o = 0;
+    d.key = o; o = al(o + 4 * static_cast<size_t>(np));
+    d.rec = o; o = al(o + 8 * static_cast<size_t>(R_N) * np);
+    d.counts = o; o = al(o + 4 * (nc + 1));
+    d.start = o; o = al(o + 4 * (nc + 1));
+    d.fill = o; o = al(o + 4 * (nc + 1));
+    d.list = o; o = al(o + 4 * static_cast<size_t>(np));
+    d.bsum = o; o = al(o + 4 * nb);
+    d.total = o; o = al(o + 4);
+    d.bytes = o;
+    return d;
+}
+}  // namespace
+
+int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st) {
+    if (np <= 0) return QZ_OK;
+    const int tb = 256;
+    hipLaunchKernelGGL(beam_count_live_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, w, count);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
+extern "C" size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N) {
+    return dep_layout(np, nloc, N).bytes;
+}
+
+extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
+                               const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream) {
+    if (!grid || !box || !p || !bsrc || (np > 0 && !work)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
+    if (np < 0 || grid->N < 2 || box->nloc < 1) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
+    if (nc != (modes == 1 ? B_NC1 : B_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: nc must be 5 (m = 0) or 15 (m = 1)");
+    if (np == 0) return QZ_OK;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const DepWork L = dep_layout(np, box->nloc, grid->N);
+    char* w = static_cast<char*>(work);
+    int* key = reinterpret_cast<int*>(w + L.key);
+    Real* rec = reinterpret_cast<Real*>(w + L.rec);
+    int* counts = reinterpret_cast<int*>(w + L.counts);
+    int* start = reinterpret_cast<int*>(w + L.start);
+    int* fill = reinterpret_cast<int*>(w + L.fill);
+    int* list = reinterpret_cast<int*>(w + L.list);
+    int* bsum = reinterpret_cast<int*>(w + L.bsum);
+    int* total = reinterpret_cast<int*>(w + L.total);
+    const int ncells = (box->nloc + 1) * grid->N;
+    QZ_HIP(hipMemsetAsync(counts, 0, 4 * (static_cast<size_t>(ncells) + 1), st));
+    QZ_HIP(hipMemsetAsync(fill, 0, 4 * (static_cast<size_t>(ncells) + 1), st));
+    const int tb = 256;
+    hipLaunchKernelGGL(beam_key_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q, np, *p, key, rec, counts);
+    const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
+    const int nb = (n1 + 1023) / 1024;
+    hipLaunchKernelGGL(scan_block_sums, dim3(nb), dim3(1024), 0, st, counts, n1, bsum);
+    hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+    hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
+    hipLaunchKernelGGL(beam_scatter_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, key, start, fill, list);
+    hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tb - 1) / tb), dim3(tb), 0, st, ncells, start, list);
+    const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
+    hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tb - 1) / tb)), dim3(tb), 0, st, *grid, *box,
+                       modes == 1 ? 1 : 0, start, list, rec, bsrc, nc);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
+extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,
+                               int i0, int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick,
+                               double dt, int move, void* stream) {
+    if (!grid || !box || !p || !fld) return qz_fail(QZ_EINVAL, "qz_beam_advance: null argument");
+    if (np < 0 || i0 < 0 || grid->N < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance: bad extent");
+    if (!box->ngp && box->nloc < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance: linear shape needs >= 2 slices");
+    if (nc != (modes == 1 ? F_NC1 : F_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_advance: nc must be 6 (m = 0) or 18 (m = 1)");
+    if (pusher < 0 || pusher > 3) return qz_fail(QZ_EINVAL, "qz_beam_advance: pusher must be 0 vay, 1 boris, 2 hc, 3 imp");
+    if (np - i0 <= 0) return QZ_OK;
+    const int tb = 256;
+    hipLaunchKernelGGL(beam_advance_kernel, dim3((np - i0 + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
+                       *grid, *box, modes == 1 ? 1 : 0, qm, pusher, i0, np, *p, fld, nc, dt_kick, dt, move);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
+// Beam::sums (src/Beam.cpp:1133-1155): copied to the host and summed in index order like the reference
+extern "C" int qz_beam_sums(int np, const qz_beam_soa* p, double gref, double xiref, double* out, void* stream) {
+    if (!p || !out || np < 0) return qz_fail(QZ_EINVAL, "qz_beam_sums: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<double> h[7];
+    double* src[7] = {p->x, p->y, p->px, p->py, p->pz, p->xi, p->w};
+    for (int c = 0; c < 7; ++c) {
+        h[c].resize(static_cast<size_t>(np));
+        if (np) QZ_HIP(hipMemcpyAsync(h[c].data(), src[c], 8 * static_cast<size_t>(np), hipMemcpyDeviceToHost, st));
+    }
+    QZ_HIP(hipStreamSynchronize(st));
+    for (int c = 0; c < 17; ++c) out[c] = 0;
+    for (int i = 0; i < np; ++i) {
+        const double w = h[6][i];
+        if (w == 0) continue;
+        out[16] += 1;
+        const double PX = h[2][i], PY = h[3][i], PZ = h[4][i];
+        const double g = std::sqrt(1 + PX * PX + PY * PY + PZ * PZ);
+        const double X = h[0][i], Y = h[1][i];
+        const double G = g - gref, XI = h[5][i] - xiref;
+        const double v[15] = {G, G * G, XI, XI * XI, X * X + Y * Y, X, Y, X * X, Y * Y, PX, PY, PX * PX, PY * PY, X * PX, Y * PY};
+        out[0] += w;
+        for (int c = 0; c < 15; ++c) out[c + 1] += w * v[c];
+    }
+    return QZ_OK;
+}

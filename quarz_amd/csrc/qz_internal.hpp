@@ -1,0 +1,22 @@
+// Internal helpers of the quarz_amd library: error reporting and the launch functions of the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/quarz_amd.h"
+
+int qz_fail(int code, const std::string& msg);
+int qz_fail_hip(hipError_t e, const std::string& where);
+
+#define QZ_HIP(call)                                              \
+    do {                                                          \
+        hipError_t _e = (call);                                   \
+        if (_e != hipSuccess) return qz_fail_hip(_e, #call);      \
+    } while (0)
+
+size_t qz_sweep_workspace_doubles_impl(int M);
+int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
+int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);

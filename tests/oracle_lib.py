@@ -256,7 +256,8 @@ class Sim:
         check(lib().or_sim_sweep_partial(self.h, nk))
 
     def timing(self):
-        out = np.zeros(2)
+        """(sweep s, beam push s, particle-slices of the last sweep)"""
+        out = np.zeros(3)
         lib().or_sim_timing(self.h, _p(out))
         return out
 

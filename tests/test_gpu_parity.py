@@ -1,0 +1,198 @@
+"""GPU parity: the gfx950 path (libquarz_amd.so, through its C-ABI) against the oracle on the same inputs.
+
+The bar is bit-identity: every array the reference's serial build would produce (fields, diagnostics, beam
+sources, particles) must be equal element for element (np.array_equal; -0.0 == +0.0).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def qa():
+    import quarz_amd
+    quarz_amd.lib()
+    return quarz_amd
+
+
+def _same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+def _report(name, a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    bad = np.nonzero(a != b)
+    n = len(bad[0])
+    first = tuple(int(x[0]) for x in bad) if n else None
+    return f"{name}: {n} of {a.size} differ; first at {first}: {a[first] if n else ''} vs {b[first] if n else ''}"
+
+
+# ------------------------------------------------------------------------------------------ Thomas
+@pytest.mark.parametrize("M", [16, 100, 513, 1537])
+def test_thomas_batched_bit_exact(torch_dev, qa, oracle, M):
+    torch = torch_dev
+    rng = np.random.default_rng(M)
+    batch = 7
+    a = rng.uniform(-1, 1, (batch, M)); a[:, 0] = 0
+    c = rng.uniform(-1, 1, (batch, M)); c[:, -1] = 0
+    b = 2.5 + rng.uniform(-1, 1, (batch, M))
+    d = rng.uniform(-1, 1, (batch, M))
+    dev = [torch.tensor(v, dtype=torch.float64, device="cuda") for v in (a, b, c, d)]
+    x = torch.zeros((batch, M), dtype=torch.float64, device="cuda")
+    qa.thomas_batched(*dev, x)
+    torch.cuda.synchronize()
+    got = x.cpu().numpy()
+    for s in range(batch):
+        ref = oracle.thomas(a[s].copy(), b[s].copy(), c[s].copy(), d[s].copy())
+        assert _same(got[s], ref), _report(f"system {s}", got[s], ref)
+
+
+# ------------------------------------------------------------------------------------------ beam kernels
+def _grid_dev(torch, s):
+    V, iV, ih = s.grid_arrays()
+    r = torch.tensor(s.nodes, dtype=torch.float64, device="cuda")
+    return (r, torch.tensor(ih, dtype=torch.float64, device="cuda"), torch.tensor(iV, dtype=torch.float64, device="cuda"))
+
+
+def _beam_parts(rng, n, R, xi_lo, xi_hi, m1, gamma=2000.0):
+    r = R * np.sqrt(rng.uniform(0, 1.02, n))
+    th = rng.uniform(0, 2 * np.pi, n) if m1 else np.zeros(n)
+    parts = {
+        "x": r * np.cos(th), "y": r * np.sin(th),
+        "px": rng.normal(0, 2.0, n), "py": rng.normal(0, 2.0, n), "pz": gamma * (1 + 0.01 * rng.normal(0, 1, n)),
+        "xi": rng.uniform(xi_lo, xi_hi, n), "w": rng.uniform(0.5, 1.5, n) * 1e-4,
+    }
+    parts["w"][rng.uniform(0, 1, n) < 0.02] = 0.0
+    return parts
+
+
+@pytest.mark.parametrize("m1", [False, True])
+@pytest.mark.parametrize("ngp", [False, True])
+def test_beam_deposit_bit_exact(torch_dev, qa, oracle, m1, ngp):
+    torch = torch_dev
+    rng = np.random.default_rng(11 + m1 + 2 * ngp)
+    nodes = oracle.make_regions(4.0, [(0.3, 0.005), (4.0, 0.05)], 1.05)
+    s = oracle.Solver(nodes)
+    nxi, dxi, xi_min = 120, 0.02, 0.0
+    n = 60000
+    parts = _beam_parts(rng, n, 3.0, -0.1, nxi * dxi + 0.1, m1)
+    # a dense core: many particles per cell
+    parts["x"][:5000] *= 0.01
+    parts["y"][:5000] *= 0.01
+    nc = 15 if m1 else 5
+    ref = oracle.beam_deposit(s, xi_min, dxi, nxi, ngp, m1, -1.0, parts, nc)
+    r, ih, iV = _grid_dev(torch, s)
+    g = qa.radial_grid(r, ih, iV)
+    box = qa.BeamGridC(xi_min, dxi, nxi, 0, nxi, int(ngp))
+    p = {k: torch.tensor(parts[k], dtype=torch.float64, device="cuda") for k in qa.BEAM_ARRAYS}
+    bsrc = torch.zeros((nxi, s.M, nc), dtype=torch.float64, device="cuda")
+    work = torch.empty(qa.beam_deposit_workspace_bytes(n, nxi, s.M - 1), dtype=torch.uint8, device="cuda")
+    qa.beam_deposit(g, box, 1 if m1 else 0, -1.0, p, bsrc, work)
+    torch.cuda.synchronize()
+    got = bsrc.cpu().numpy()
+    assert _same(got, ref), _report("bsrc", got, ref)
+
+
+@pytest.mark.parametrize("m1", [False, True])
+@pytest.mark.parametrize("pusher", [0, 1, 2, 3])
+@pytest.mark.parametrize("move", [False, True])
+def test_beam_advance_bit_exact(torch_dev, qa, oracle, m1, pusher, move):
+    torch = torch_dev
+    rng = np.random.default_rng(100 + 8 * m1 + 2 * pusher + move)
+    nodes = oracle.make_uniform(3.0, 0.02)
+    s = oracle.Solver(nodes)
+    nxi, dxi, xi_min = 200, 0.01, 0.0
+    n = 20000
+    parts = _beam_parts(rng, n, 2.9, 0.1, nxi * dxi - 0.1, m1)
+    nc = 18 if m1 else 6
+    fld = rng.normal(0, 1.0, (nxi, s.M, nc))
+    ref = oracle.beam_advance(s, xi_min, dxi, nxi, False, m1, -1.0, pusher, parts, fld, 10.0, 10.0, move)
+    r, ih, iV = _grid_dev(torch, s)
+    g = qa.radial_grid(r, ih, iV)
+    box = qa.BeamGridC(xi_min, dxi, nxi, 0, nxi, 0)
+    p = {k: torch.tensor(parts[k], dtype=torch.float64, device="cuda") for k in qa.BEAM_ARRAYS}
+    f = torch.tensor(fld, dtype=torch.float64, device="cuda")
+    qa.beam_advance(g, box, 1 if m1 else 0, -1.0, pusher, p, f, 10.0, 10.0, move)
+    torch.cuda.synchronize()
+    for k in qa.BEAM_ARRAYS:
+        got = p[k].cpu().numpy()
+        assert _same(got, ref[k]), _report(k, got, ref[k])
+
+
+# ------------------------------------------------------------------------------------------ whole step
+def _compare_sims(a, b, what=("fld", "diag", "bsrc")):
+    for w in what:
+        ga, gb = getattr(a, w), getattr(b, w)
+        assert _same(ga, gb), _report(w, ga, gb)
+
+
+def test_sweep_blowout_m0_bit_exact(qa, oracle, decks):
+    """The reference's blowout validation deck (m = 0, 3200 particles per slice, 1001 slices, near-singular E_z)."""
+    ov = {"xi.min": -10}
+    g = qa.Simulation(f"{decks}/blowout.in", ov)
+    o = oracle.Sim(f"{decks}/blowout.in", ov)
+    g.step()
+    o.step()
+    _compare_sims(g, o)
+
+
+def test_sweep_stretched_m0_bit_exact(qa, oracle, decks):
+    ov = {"xi.min": -10, "xi.step": 0.005, "grid.type": "regions", "grid.regions": "0.5:0.0025 2.5:0.01 8:0.05",
+          "grid.max_ratio": 1.05}
+    g = qa.Simulation(f"{decks}/blowout.in", ov)
+    o = oracle.Sim(f"{decks}/blowout.in", ov)
+    g.step()
+    o.step()
+    _compare_sims(g, o)
+
+
+def test_sweep_m1_mobile_bit_exact(qa, oracle, decks):
+    """m = 1 with mobile electrons (8 azimuths per ring), an offset driver and Picard coupling."""
+    ov = {"modes": 1, "xi.min": -6, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05, "driver.nparticles": 40000,
+          "driver.sigma_r": 0.5, "electrons.ppc": 1}
+    g = qa.Simulation(f"{decks}/linear.in", ov)
+    o = oracle.Sim(f"{decks}/linear.in", ov)
+    g.step()
+    o.step()
+    _compare_sims(g, o)
+
+
+def test_steps_gain_bit_exact(qa, oracle, decks):
+    """Driver + witness, beam pushes between sweeps: fields and witness phase space after 3 steps."""
+    ov = {"time.steps": 3, "driver.nparticles": 50000, "xi.min": -8}
+    g = qa.Simulation(f"{decks}/gain.in", ov)
+    o = oracle.Sim(f"{decks}/gain.in", ov)
+    for _ in range(4):
+        g.step()
+        o.step()
+    _compare_sims(g, o)
+    for k in qa.BEAM_ARRAYS:
+        a, b = g.get(f"beam1.{k}"), o.get(f"beam1.{k}")
+        assert _same(a, b), _report(f"witness {k}", a, b)
+    da, db = g.beam_diag(1), o.beam_diag(1)
+    for k in da:
+        assert da[k] == db[k], (k, da[k], db[k])
+
+
+def test_betatron_m1_steps_bit_exact(qa, oracle, decks):
+    """Immobile ion channel, offset witness, m = 1, 10 Vay steps."""
+    ov = {"modes": 1, "witness.x0": 0.05, "witness.nsym": 4, "time.steps": 10}
+    g = qa.Simulation(f"{decks}/betatron.in", ov)
+    o = oracle.Sim(f"{decks}/betatron.in", ov)
+    for _ in range(11):
+        g.step()
+        o.step()
+    _compare_sims(g, o)
+    for k in qa.BEAM_ARRAYS:
+        a, b = g.get(f"beam0.{k}"), o.get(f"beam0.{k}")
+        assert _same(a, b), _report(f"witness {k}", a, b)
