@@ -94,7 +94,8 @@ typedef struct qz_sweep_args {
     qz_species_state sp[QZ_MAX_SPECIES];
     int* lost;                 /* one counter per species (particles removed as trapped) */
     double* ws;                /* per-run workspace of qz_sweep_workspace_doubles(M) doubles */
-    unsigned long long* stats; /* optional: stats[0] += live particles of every slice (particle-slices) */
+    unsigned long long* stats; /* optional: stats[0] += live particles of every slice (particle-slices),
+                                  stats[1..10] += wall-clock ticks of the sweep phases */
 } qz_sweep_args;
 
 const char* qz_last_error(void);
@@ -152,6 +153,9 @@ int qz_sim_timing(const qz_sim* s, double* out2);
 /* work of the last step: particle-slices of the sweep, beam particles pushed, beam particles deposited,
  * sweep kernel ms, beam deposit ms, beam push ms */
 int qz_sim_stats(const qz_sim* s, double* out6);
+/* wall-clock ticks (100 MHz) of the phases of the last sweep, measured by one thread of the run:
+ * geometry, sort, deposit, combine, B_z rhs, psi/E_z/B_z solves, W+, S+ deposit, Picard B+, store+push */
+int qz_sim_phase_ticks(const qz_sim* s, double* out10);
 long qz_sim_lost(const qz_sim* s);
 double qz_sim_time(const qz_sim* s);
 int qz_sim_write_fields(qz_sim* s, const char* path);

@@ -68,6 +68,7 @@ def lib():
         L.qz_sim_beam_sums.argtypes = [C.c_void_p, C.c_int, DP, DP]
         L.qz_sim_timing.argtypes = [C.c_void_p, DP]
         L.qz_sim_stats.argtypes = [C.c_void_p, DP]
+        L.qz_sim_phase_ticks.argtypes = [C.c_void_p, DP]
         L.qz_sim_lost.restype = C.c_long
         L.qz_sim_lost.argtypes = [C.c_void_p]
         L.qz_sim_time.restype = C.c_double
@@ -141,6 +142,14 @@ class Simulation:
         out = np.zeros(6)
         check(lib().qz_sim_stats(self.h, out.ctypes.data_as(DP)))
         return dict(zip(["particle_slices", "beam_pushed", "beam_deposited", "sweep_ms", "beam_dep_ms", "beam_push_ms"], out))
+
+    PHASES = ["geometry", "sort", "deposit", "combine", "bz_rhs", "solves", "wplus", "s_deposit", "picard", "store_push"]
+
+    def phase_ms(self):
+        """Milliseconds of each sweep phase of the last sweep (one thread's wall clock, 100 MHz ticks)."""
+        out = np.zeros(10)
+        check(lib().qz_sim_phase_ticks(self.h, out.ctypes.data_as(DP)))
+        return dict(zip(self.PHASES, out * 1e-5))
 
     def lost(self):
         return lib().qz_sim_lost(self.h)

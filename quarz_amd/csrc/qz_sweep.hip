@@ -3,19 +3,21 @@
 //
 // Per slice:
 //   1. geometry of every live plasma particle (kept in registers for the slice);
-//   2. a stable counting sort of the particles by radial cell in LDS (ballot ranks per wave), from which
-//      every particle gets its position in the contribution lists of its two nodes, in particle-index
-//      order;
+//   2. a stable counting sort of the particles by radial cell in LDS (ballot ranks per wave); every
+//      particle gets its position in the contribution lists of its two nodes, in particle-index order;
 //   3. the mode deposit (PlasmaSpecies::deposit, src/PlasmaSpecies.cpp:409-478): particles write their two
-//      contributions per array into LDS, one thread per node then sums its list sequentially -- the
+//      contributions per array into LDS, one thread per (node, array) sums the list sequentially -- the
 //      reference's atomic_add performed in index order, i.e. the serial result bit for bit;
 //   4. combine with the background and the beam sources (Simulation::combine_sources);
-//   5. field solves: psi, E_z, B_z (chi-free Thomas with factors formed once per run) while another wave
-//      forms the per-slice Thomas factors of the chi-dependent B+ operators; W+, the S+ deposit, and the
-//      Picard solves for B+ (Simulation::solve_slice_fields / compute_wplus / compute_bz / solve_bplus);
+//   5. psi, E_z, B_z solves (chi-free Thomas, factors formed once per run) on one wave while another wave
+//      forms this slice's Thomas factors of the chi-dependent B+ operators; W+; the S+ deposit; the
+//      Picard solves for B+ (Simulation::solve_slice_fields, compute_wplus, compute_bz, solve_bplus);
 //   6. the field store (Simulation::store_slice) and the Adams-Bashforth push (PlasmaSpecies::push).
-// Per-slice arrays live in the run's workspace (L2-resident); the deposit lists and the sort in LDS.
+// The Thomas chains and the particle gathers read LDS; the run's workspace (L2-resident) carries arrays
+// between phases.  LDS holds the grid, the sort and one region that every phase reuses for its own arrays.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "qz_device.hpp"
 #include "qz_internal.hpp"
@@ -24,141 +26,51 @@ namespace qz {
 
 // workspace arrays (M doubles each)
 enum WsArr : int {
-    // deposit components (the first 30 are also the deposit component ids)
+    // node sums of the plasma deposit (the first 24 ids are also the deposit component ids)
     W_RHOT0 = 0, W_DRHO0, W_JZ0, W_CHI0, W_NE0, W_NI0, W_JPR1, W_JPI1,
     W_RHOT1R, W_RHOT1I, W_DRHO1R, W_DRHO1I, W_JZ1R, W_JZ1I, W_CHI1R, W_CHI1I, W_NE1R, W_NE1I, W_NI1R, W_NI1I,
     W_JPR0, W_JPI0, W_JPR2, W_JPI2,
+    // S+ node sums, order r1 i1 r0 i0 r2 i2
     W_SR1, W_SI1, W_SR0, W_SI0, W_SR2, W_SI2,
     // fields of the slice
-    W_PSI0, W_PSI1R, W_PSI1I, W_EZ0, W_EZ1R, W_EZ1I, W_BZ0, W_BZ1R, W_BZ1I,
-    W_WPR0, W_WPI0, W_WPR1, W_WPI1, W_WPR2, W_WPI2,
-    W_BPR0, W_BPI0, W_BPR1, W_BPI1, W_BPR2, W_BPI2,
-    // right-hand sides
-    W_T3, W_T4, W_RHS,
-    W_GR0, W_GI0, W_GR1, W_GI1, W_GR2, W_GI2,
-    W_RBR0, W_RBI0, W_RBR1, W_RBI1, W_RBR2, W_RBI2,
-    // Thomas factors (m, c') of the chi-dependent operators of this slice
-    W_FM_L0, W_FCP_L0, W_FM_L1D, W_FCP_L1D, W_FM_L1F, W_FCP_L1F, W_FM_L2D, W_FCP_L2D,
-    // forward-sweep scratch of concurrent solves
-    W_DP0, W_DP_END = W_DP0 + 9,
-    W_NARR = W_DP_END
+    W_EZ0, W_EZ1R, W_EZ1I, W_BZ0, W_BZ1R, W_BZ1I,
+    W_WPR1, W_WPI1, W_WPR0, W_WPI0, W_WPR2, W_WPI2,
+    W_BPR1, W_BPI1, W_BPR0, W_BPI0, W_BPR2, W_BPI2,
+    // Thomas factors (m, c') of this slice's chi-dependent operators, in the order L1D, L1F, L0, L2D
+    W_FAC0,
+    W_NARR = W_FAC0 + 8
 };
 
 enum OpId : int { OP_L0 = 0, OP_L1D = 1, OP_L1F = 2, OP_L2D = 3 };
 
+template <int V> using IC = std::integral_constant<int, V>;
+
 constexpr int kBS = 512;           // threads per run
 constexpr int kNW = kBS / 64;      // waves per run
 constexpr int kLdsBytes = 160 * 1024;
+constexpr int kMiscBytes = 512;    // counters, bw0 of the chi factors, deposit component list, phase clocks
 
-struct SpecInfo {
-    const qz_species_state* st;
-    int off, np;
-    Real q, qm;
+// byte offsets of the LDS carve-up: r | 1/h | 1/V | wave counters | cell starts | sorted ids | misc | region
+struct LdsPlan {
+    int Mp, wcnt, cs, sidx, misc, region;
+    __host__ __device__ LdsPlan(int N, int pt) {
+        const int M = N + 1;
+        Mp = (M + 1) & ~1;
+        wcnt = 3 * Mp * 8;
+        cs = wcnt + (((kNW * N * 2) + 15) & ~15);
+        sidx = cs + ((((N + 3) * 4) + 15) & ~15);
+        misc = sidx + (((kBS * pt * 2) + 15) & ~15);
+        region = misc + kMiscBytes;
+    }
 };
 
-// One Thomas solve with factors formed beforehand (Tridiag.cpp:114-125 with cp, m precomputed):
-//   dp_0 = d_0 / b_0,  dp_i = (d_i - a_i dp_{i-1}) m_i,  x_{M-1} = dp_{M-1},  x_i = dp_i - cp_i x_{i+1}
-// d_i is the prepared right-hand side of FieldSolver::solve (src/FieldSolver.cpp:253-259).
-__device__ void thomas_apply(const Real* __restrict__ a, const Real* __restrict__ m, const Real* __restrict__ cp,
-                             Real b0, const Real* __restrict__ rhs, Real scale, bool dir0, bool dirN, bool fluxN,
-                             Real fcw, Real* __restrict__ dp, Real* __restrict__ x, int M) {
-    const int N = M - 1;
-    Real d0 = dir0 ? Real(0) : scale * rhs[0];
-    Real prev = d0 / b0;
-    dp[0] = prev;
-    for (int i = 1; i < N; ++i) {
-        const Real di = scale * rhs[i];
-        prev = (di - a[i] * prev) * m[i];
-        dp[i] = prev;
-    }
-    {
-        Real dN;
-        if (dirN) dN = Real(0);
-        else {
-            dN = scale * rhs[N];
-            if (fluxN) dN -= fcw;
-        }
-        prev = (dN - a[N] * prev) * m[N];
-        dp[N] = prev;
-    }
-    Real xn = prev;
-    x[N] = xn;
-    for (int i = N - 1; i >= 0; --i) {
-        xn = dp[i] - cp[i] * xn;
-        x[i] = xn;
-    }
-}
-
-// Thomas factors of (L - chi) with the Dirichlet rows of FieldSolver::solve (src/FieldSolver.cpp:253-256):
-//   bw_i = b_i (Dirichlet row) or b_i - chi_i;  cp_0 = c_0 / bw_0;  m_i = 1 / (bw_i - a_i cp_{i-1});  cp_i = c_i m_i
-// Returns bw_0 for the first row of the solve.
-__device__ Real thomas_factor(const Real* __restrict__ a, const Real* __restrict__ b, const Real* __restrict__ c,
-                              const Real* __restrict__ chi, bool dir0, bool dirN, Real* __restrict__ m,
-                              Real* __restrict__ cp, int M) {
-    const int N = M - 1;
-    const Real bw0 = dir0 ? b[0] : b[0] - chi[0];
-    Real cprev = c[0] / bw0;
-    cp[0] = cprev;
-    m[0] = Real(0);
-    for (int i = 1; i < M; ++i) {
-        const Real bwi = (i == N && dirN) ? b[i] : b[i] - chi[i];
-        const Real mi = Real(1) / (bwi - a[i] * cprev);
-        cprev = c[i] * mi;
-        m[i] = mi;
-        cp[i] = cprev;
-    }
-    return bw0;
-}
-
-// block-wide exclusive scan of n ints in place (one wave does it; the others wait at the caller's barrier)
-__device__ void wave_exclusive_scan(int* __restrict__ v, int n, int lane) {
-    const int per = (n + 63) / 64;
-    const int lo = lane * per, hi = min(lo + per, n);
-    int s = 0;
-    for (int i = lo; i < hi; ++i) s += v[i];
-    int incl = s;
-    for (int d = 1; d < 64; d <<= 1) {
-        const int o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += o;
-    }
-    int run = incl - s;
-    for (int i = lo; i < hi; ++i) {
-        const int t = v[i];
-        v[i] = run;
-        run += t;
-    }
-}
-
-// number of entries < key in the sorted range idx[lo, hi)
-__device__ __forceinline__ int count_less(const unsigned short* __restrict__ idx, int lo, int hi, int key) {
-    int a = lo, b = hi;
-    while (a < b) {
-        const int mid = (a + b) >> 1;
-        if (static_cast<int>(idx[mid]) < key) a = mid + 1; else b = mid;
-    }
-    return a - lo;
-}
-
-template <bool M1, int PT>
-struct Sweep {
-    // ---------------------------------------------------------------- per-thread particle cache (one slice)
-    int cj[PT], cpl[PT], cpr[PT], cspec[PT], cli[PT];
-    Real ct[PT], cc[PT], cs[PT], cir[PT], cinvd[PT], cgam[PT], cPX[PT], cPY[PT], cD[PT], cw[PT];
-
-    __device__ __forceinline__ int slot(int p, int w, int lane) const { return w * (PT * 64) + p * 64 + lane; }
-};
-
-// value of deposit component `comp` for the node on side cc (0: j, 1: j+1) of a particle
-// (PlasmaSpecies::deposit, src/PlasmaSpecies.cpp:420-477; rho is not used on this path and not formed)
+// value of deposit component `comp` for the node on side cc (0: j, 1: j+1) of one particle
+// (PlasmaSpecies::deposit, src/PlasmaSpecies.cpp:420-477; rho is not read on this path and not formed)
 __device__ __forceinline__ Real dep_value(int comp, int cc, int j, Real t, Real c, Real sn, Real ir, Real invd,
-                                          Real gam, Real PX, Real PY, Real D, Real wi, Real q, Real qm,
-                                          const Real* __restrict__ inv_V, const Real* __restrict__ inv_h) {
-    const int jj = j + cc;
-    const Real iv = inv_V[jj];
-    const Real ih = inv_h[j];
+                                          Real gam, Real PX, Real PY, Real pz, Real wi, Real q, Real qm, Real iv,
+                                          Real ih) {
     const Real W = cc == 0 ? Real(1) - t : t;
     const Real dW = cc == 0 ? -ih : ih;
-    const Real pz = gam - D;
     const Real ux = PX * invd, uy = PY * invd;
     const Real ur = c * ux + sn * uy, uth = c * uy - sn * ux;
     const Real wq = wi * q;
@@ -204,143 +116,354 @@ __device__ __forceinline__ Real dep_value(int comp, int cc, int j, Real t, Real 
     }
 }
 
-__device__ __forceinline__ Real lin(const Real* __restrict__ f, int j, Real t) {
-    return (Real(1) - t) * f[j] + t * f[j + 1];
+// Thomas solve with factors formed beforehand, every array in LDS (Tridiag.cpp:114-125 with c', m precomputed):
+//   dp_0 = d_0 / b_0,  dp_i = (d_i - a_i dp_{i-1}) m_i,  x_{M-1} = dp_{M-1},  x_i = dp_i - c'_i x_{i+1}
+// d_i is the prepared right-hand side of FieldSolver::solve (src/FieldSolver.cpp:253-259): 0 on Dirichlet
+// rows, scale * rhs otherwise, minus fcw on the flux row.  w holds rhs on entry and dp afterwards; the
+// solution goes to x (which may be w).  Rows are loaded four at a time, one group ahead of the arithmetic.
+__device__ __forceinline__ void thomas_lds(const Real* a, const Real* m, const Real* cp, Real b0, Real* w, Real* x, Real scale,
+                           bool dir0, bool dirN, bool fluxN, Real fcw, int M) {
+    const int N = M - 1;
+    Real prev = (dir0 ? Real(0) : scale * w[0]) / b0;
+    w[0] = prev;
+    const int nq = (N - 1) >> 2;   // full groups of four rows in 1 .. N-1
+    int i = 1;
+    if (nq > 0) {
+        Real d0 = w[1], d1 = w[2], d2 = w[3], d3 = w[4];
+        Real a0 = a[1], a1 = a[2], a2 = a[3], a3 = a[4];
+        Real m0 = m[1], m1 = m[2], m2 = m[3], m3 = m[4];
+        for (int q = 0; q < nq; ++q, i += 4) {
+            const int nx = (q + 1 < nq) ? i + 4 : i;   // next group (the last one reloads its own rows)
+            const Real e0 = w[nx], e1 = w[nx + 1], e2 = w[nx + 2], e3 = w[nx + 3];
+            const Real b0_ = a[nx], b1 = a[nx + 1], b2 = a[nx + 2], b3 = a[nx + 3];
+            const Real n0 = m[nx], n1 = m[nx + 1], n2 = m[nx + 2], n3 = m[nx + 3];
+            prev = (scale * d0 - a0 * prev) * m0; w[i] = prev;
+            prev = (scale * d1 - a1 * prev) * m1; w[i + 1] = prev;
+            prev = (scale * d2 - a2 * prev) * m2; w[i + 2] = prev;
+            prev = (scale * d3 - a3 * prev) * m3; w[i + 3] = prev;
+            d0 = e0; d1 = e1; d2 = e2; d3 = e3;
+            a0 = b0_; a1 = b1; a2 = b2; a3 = b3;
+            m0 = n0; m1 = n1; m2 = n2; m3 = n3;
+        }
+    }
+    for (; i < N; ++i) {
+        prev = (scale * w[i] - a[i] * prev) * m[i];
+        w[i] = prev;
+    }
+    {
+        Real dN;
+        if (dirN) dN = Real(0);
+        else {
+            dN = scale * w[N];
+            if (fluxN) dN -= fcw;
+        }
+        prev = (dN - a[N] * prev) * m[N];
+    }
+    Real xn = prev;
+    x[N] = xn;
+    // backward: rows N-1 .. 0, four at a time from the top, loads one group ahead
+    i = N - 1;
+    const int nb = N >> 2;
+    if (nb > 0) {
+        Real p0 = w[i], p1 = w[i - 1], p2 = w[i - 2], p3 = w[i - 3];
+        Real c0 = cp[i], c1 = cp[i - 1], c2 = cp[i - 2], c3 = cp[i - 3];
+        for (int q = 0; q < nb; ++q, i -= 4) {
+            const int nx = (q + 1 < nb) ? i - 4 : i;
+            const Real e0 = w[nx], e1 = w[nx - 1], e2 = w[nx - 2], e3 = w[nx - 3];
+            const Real f0 = cp[nx], f1 = cp[nx - 1], f2 = cp[nx - 2], f3 = cp[nx - 3];
+            xn = p0 - c0 * xn; x[i] = xn;
+            xn = p1 - c1 * xn; x[i - 1] = xn;
+            xn = p2 - c2 * xn; x[i - 2] = xn;
+            xn = p3 - c3 * xn; x[i - 3] = xn;
+            p0 = e0; p1 = e1; p2 = e2; p3 = e3;
+            c0 = f0; c1 = f1; c2 = f2; c3 = f3;
+        }
+    }
+    for (; i >= 0; --i) {
+        xn = w[i] - cp[i] * xn;
+        x[i] = xn;
+    }
 }
+
+// Thomas factors of (L - chi) with the Dirichlet rows of FieldSolver::solve (src/FieldSolver.cpp:253-256):
+//   bw_i = b_i (Dirichlet row) or b_i - chi_i;  c'_0 = c_0 / bw_0;  m_i = 1 / (bw_i - a_i c'_{i-1});  c'_i = c_i m_i
+// Rows 0 .. N-1 from a, b, c (LDS); the last row's a_N, b_N given (the flux and Dirichlet variants of L_1
+// share every other row).  m, c' go to global memory.  Returns bw_0.
+__device__ __forceinline__ Real thomas_factor(const Real* a, const Real* b, const Real* c, const Real* chi, Real aN, Real bN,
+                              bool dir0, bool dirN, Real* __restrict__ m, Real* __restrict__ cp, int M) {
+    const int N = M - 1;
+    const Real bw0 = dir0 ? b[0] : b[0] - chi[0];
+    Real cprev = c[0] / bw0;
+    cp[0] = cprev;
+    m[0] = Real(0);
+    for (int i = 1; i < N; ++i) {
+        const Real mi = Real(1) / ((b[i] - chi[i]) - a[i] * cprev);
+        cprev = c[i] * mi;
+        m[i] = mi;
+        cp[i] = cprev;
+    }
+    const Real bwN = dirN ? bN : bN - chi[N];
+    const Real mN = Real(1) / (bwN - aN * cprev);
+    m[N] = mN;
+    cp[N] = c[N] * mN;
+    return bw0;
+}
+
+// exclusive scan of n ints in place by one wave
+__device__ __forceinline__ void wave_exclusive_scan(int* __restrict__ v, int n, int lane) {
+    const int per = (n + 63) / 64;
+    const int lo = lane * per, hi = min(lo + per, n);
+    int s = 0;
+    for (int i = lo; i < hi; ++i) s += v[i];
+    int incl = s;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += o;
+    }
+    int run = incl - s;
+    for (int i = lo; i < hi; ++i) {
+        const int t = v[i];
+        v[i] = run;
+        run += t;
+    }
+}
+
+// entries < key in the index-sorted range idx[lo, hi)
+__device__ __forceinline__ int count_less(const unsigned short* __restrict__ idx, int lo, int hi, int key) {
+    int a = lo, b = hi;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (static_cast<int>(idx[mid]) < key) a = mid + 1; else b = mid;
+    }
+    return a - lo;
+}
+
+// Opaque register fence: the compiler must assume the values changed here, so nothing derived from them is
+// hoisted out of the enclosing loop (per-particle live ranges stay at the cache itself).
+template <class T, int PT> __device__ __forceinline__ void fence_regs(T (&a)[PT]) {
+#pragma unroll
+    for (int i = 0; i < PT; ++i) asm volatile("" : "+v"(a[i]));
+}
+
+__device__ __forceinline__ Real lin(const Real* f, int j, Real t) { return (Real(1) - t) * f[j] + t * f[j + 1]; }
+
+// 3-point gradients and f/r (FieldSolver::gradient / gradient_odd / over_r, src/FieldSolver.cpp:263-289)
+struct Stencil {
+    const Real *gm, *g0, *gp, *ir;
+    int N;
+    __device__ __forceinline__ Real grad(const Real* f, int j) const {
+        if (j == 0) return Real(0);
+        if (j == N) return g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
+        return gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
+    }
+    __device__ __forceinline__ Real grad_odd(const Real* f, int j) const {
+        if (j == 0) return f[1] * ir[0];
+        if (j == N) return g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
+        return gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
+    }
+    __device__ __forceinline__ Real over_r(const Real* f, int j) const { return (j == 0 ? f[1] : f[j]) * ir[j]; }
+};
+
+// array slots (Mp doubles each) of the shared LDS region in each phase
+template <bool M1> struct Lay {
+    // solves: chi-free operators and factors, right-hand sides (solved in place), J+, chi factor inputs
+    static constexpr int L0A = 0, L0M = 1, L0CP = 2, L1A = 3, L1M = 4, L1CP = 5;
+    static constexpr int WPSI0 = M1 ? 6 : 4, WEZ0 = WPSI0 + 1, WBZ0 = WPSI0 + 2;
+    static constexpr int WPSI1R = 9, WPSI1I = 10, WEZ1R = 11, WEZ1I = 12, WBZ1R = 13, WBZ1I = 14;
+    static constexpr int JP = M1 ? 15 : 7;          // J+ r1 i1 r0 i0 r2 i2
+    static constexpr int L1B = M1 ? 21 : 9, L1C = L1B + 1, CHI0 = L1B + 2;
+    static constexpr int L0B = 24, L0C = 25, L2A = 26, L2B = 27, L2C = 28;
+    static constexpr int NSOLVE = M1 ? 29 : 12;
+    // S+ deposit: W+ (r1 i1 r0 i0 r2 i2), E_z, B_z modes; the contribution lists follow
+    static constexpr int SW = 0, SEZ = M1 ? 6 : 2, SBZ = SEZ + (M1 ? 3 : 1), NSDEP = SBZ + (M1 ? 3 : 1);
+    // Picard: a of each operator, factors (m, c') per operator, gz, right-hand sides, B+, chi_1
+    static constexpr int PA_L1D = 0, PA_L1F = 1, PA_L0 = 2, PA_L2D = 3;
+    static constexpr int PF = M1 ? 4 : 2;             // (m, c') of L1D, L1F, [L0, L2D]
+    static constexpr int PGZ = PF + (M1 ? 8 : 4);     // r1 i1 r0 i0 r2 i2
+    static constexpr int PRB = PGZ + (M1 ? 6 : 2);
+    static constexpr int PBP = PRB + (M1 ? 6 : 2);
+    static constexpr int PCHI = PBP + (M1 ? 6 : 2);   // chi_1 r, i
+    static constexpr int NPICARD = PCHI + (M1 ? 2 : 0);
+    // push: W+ (r1 i1 r0 i0 r2 i2), E_z, B_z, B+ (r1 i1 r0 i0 r2 i2)
+    static constexpr int GW = 0, GEZ = M1 ? 6 : 2, GBZ = GEZ + (M1 ? 3 : 1), GBP = GBZ + (M1 ? 3 : 1);
+    static constexpr int NPUSH = GBP + (M1 ? 6 : 2);
+    static constexpr int NMAX = NSOLVE > NPICARD ? (NSOLVE > NPUSH ? NSOLVE : NPUSH) : (NPICARD > NPUSH ? NPICARD : NPUSH);
+};
 
 template <bool M1, int PT>
 __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restrict__ runs) {
+    using LY = Lay<M1>;
     const qz_sweep_args& A = runs[blockIdx.x];
     extern __shared__ double smem_d[];
     char* smem = reinterpret_cast<char*>(smem_d);
 
-    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int N = A.grid.N, M = N + 1;
     const Real R = A.grid.R;
-    const Real* __restrict__ gr = A.grid.r;
-    const Real* __restrict__ gih = A.grid.inv_h;
-    const Real* __restrict__ giv = A.grid.inv_V;
     Real* __restrict__ ws = A.ws;
     auto WS = [&](int arr) -> Real* { return ws + static_cast<size_t>(arr) * M; };
     const int K = A.nslices;
     const Real dxi = A.dxi;
     const Real idxi = Real(1) / dxi;
+    const Stencil stn{A.grad_m, A.grad_0, A.grad_p, A.inv_r, N};
+    unsigned long long* __restrict__ stats = A.stats;
 
-    // ---- species
+    // ---- species (at most two; scalars, no indexed register arrays)
     const int nsp = A.nspecies;
-    SpecInfo sp[QZ_MAX_SPECIES];
-    int NP = 0;
-    bool has_neg = false, has_pos = false;
-    for (int s = 0; s < QZ_MAX_SPECIES; ++s) {
-        sp[s].st = &A.sp[s];
-        sp[s].off = NP;
-        sp[s].np = s < nsp ? A.sp[s].np : 0;
-        sp[s].q = s < nsp ? A.sp[s].q : Real(0);
-        sp[s].qm = s < nsp ? A.sp[s].q / A.sp[s].m : Real(0);
-        if (s < nsp) {
-            NP += sp[s].np;
-            if (sp[s].q < 0) has_neg = true; else has_pos = true;
+    const int np0 = nsp > 0 ? A.sp[0].np : 0, np1 = nsp > 1 ? A.sp[1].np : 0;
+    const int NP = np0 + np1;
+    const Real q0 = nsp > 0 ? A.sp[0].q : Real(0), q1 = nsp > 1 ? A.sp[1].q : Real(0);
+    const Real qm0 = nsp > 0 ? A.sp[0].q / A.sp[0].m : Real(0), qm1 = nsp > 1 ? A.sp[1].q / A.sp[1].m : Real(0);
+    const bool has_neg = (nsp > 0 && q0 < 0) || (nsp > 1 && q1 < 0);
+    const bool has_pos = (nsp > 0 && !(q0 < 0)) || (nsp > 1 && !(q1 < 0));
+
+    // ---- LDS carve-up
+    const LdsPlan plan(N, PT);
+    const int Mp = plan.Mp;
+    Real* s_r = smem_d;
+    Real* s_ih = s_r + Mp;
+    Real* s_iv = s_ih + Mp;
+    unsigned short* s_wcnt = reinterpret_cast<unsigned short*>(smem + plan.wcnt);
+    int* s_cs = reinterpret_cast<int*>(smem + plan.cs);          // N + 3 (cell starts shifted by one)
+    unsigned short* s_sidx = reinterpret_cast<unsigned short*>(smem + plan.sidx);
+    int* s_misc = reinterpret_cast<int*>(smem + plan.misc);      // [0] live count
+    Real* s_bw0 = reinterpret_cast<Real*>(smem + plan.misc + 16);                        // 4 doubles
+    int* s_comp = reinterpret_cast<int*>(smem + plan.misc + 48);                         // 24 ints
+    unsigned long long* s_ph = reinterpret_cast<unsigned long long*>(smem + plan.misc + 160);   // 11 clocks
+    Real* B = reinterpret_cast<Real*>(smem + plan.region);
+    const int Bcap = (kLdsBytes - plan.region) / 8;
+    auto BA = [&](int slot) -> Real* { return B + static_cast<size_t>(slot) * Mp; };
+
+    // deposit components in pass order (chi_0 first: it feeds the B+ factors); the sequence of the
+    // deposit phase below follows this list
+    {
+        int n = 0;
+        auto put = [&](int c) { if (tid == 0) s_comp[n] = c; ++n; };
+        put(W_CHI0); put(W_RHOT0); put(W_DRHO0); put(W_JZ0);
+        if (has_neg) put(W_NE0);
+        if (has_pos) put(W_NI0);
+        put(W_JPR1); put(W_JPI1);
+        if (M1) {
+            put(W_CHI1R); put(W_CHI1I); put(W_RHOT1R); put(W_RHOT1I); put(W_DRHO1R); put(W_DRHO1I);
+            put(W_JZ1R); put(W_JZ1I);
+            if (has_neg) { put(W_NE1R); put(W_NE1I); }
+            if (has_pos) { put(W_NI1R); put(W_NI1I); }
+            put(W_JPR0); put(W_JPI0); put(W_JPR2); put(W_JPI2);
         }
     }
-
-    // ---- LDS carve-up: wave counters, cell starts, sorted indices, contribution lists
-    unsigned short* s_wcnt = reinterpret_cast<unsigned short*>(smem);          // kNW * N
-    int off = ((kNW * N * 2) + 15) & ~15;
-    int* s_cs = reinterpret_cast<int*>(smem + off);                           // N + 3 (cell starts shifted by one)
-    off += (((N + 3) * 4) + 15) & ~15;
-    unsigned short* s_sidx = reinterpret_cast<unsigned short*>(smem + off);   // kBS * PT
-    off += ((kBS * PT * 2) + 15) & ~15;
-    int* s_misc = reinterpret_cast<int*>(smem + off);                         // 16 ints
-    off += 64;
-    Real* s_vbuf = reinterpret_cast<Real*>(smem + off);
-    const int vbuf_len = (kLdsBytes - off) / 8;
-
-    // ---- deposit component lists
-    int comps[24];
-    int ncomp = 0;
-    comps[ncomp++] = W_RHOT0; comps[ncomp++] = W_DRHO0; comps[ncomp++] = W_JZ0; comps[ncomp++] = W_CHI0;
-    if (has_neg) comps[ncomp++] = W_NE0;
-    if (has_pos) comps[ncomp++] = W_NI0;
-    comps[ncomp++] = W_JPR1; comps[ncomp++] = W_JPI1;
-    if (M1) {
-        comps[ncomp++] = W_RHOT1R; comps[ncomp++] = W_RHOT1I; comps[ncomp++] = W_DRHO1R; comps[ncomp++] = W_DRHO1I;
-        comps[ncomp++] = W_JZ1R; comps[ncomp++] = W_JZ1I; comps[ncomp++] = W_CHI1R; comps[ncomp++] = W_CHI1I;
-        if (has_neg) { comps[ncomp++] = W_NE1R; comps[ncomp++] = W_NE1I; }
-        if (has_pos) { comps[ncomp++] = W_NI1R; comps[ncomp++] = W_NI1I; }
-        comps[ncomp++] = W_JPR0; comps[ncomp++] = W_JPI0; comps[ncomp++] = W_JPR2; comps[ncomp++] = W_JPI2;
+    for (int j = tid; j < M; j += kBS) {
+        s_r[j] = A.grid.r[j];
+        s_iv[j] = A.grid.inv_V[j];
+        if (j < N) s_ih[j] = A.grid.inv_h[j];
+        for (int a = W_BPR1; a <= W_BPI2; ++a) WS(a)[j] = Real(0);   // B+ := 0 at the head (Simulation.cpp:705-708)
     }
-    const int nS = M1 ? 6 : 2;   // S components W_SR1.. in order r1 i1 r0 i0 r2 i2
-
-    // ---- B+ := 0 at the head of the box (src/Simulation.cpp:705-708); absent density components are 0
-    for (int n = tid; n < M; n += kBS) {
-        for (int a = W_BPR0; a <= W_BPI2; ++a) WS(a)[n] = Real(0);
+    if (tid == 0) {
+        for (int i = 0; i < 11; ++i) s_ph[i] = 0;
+        s_ph[10] = wall_clock64();
     }
-    __syncthreads();
-
-    Sweep<M1, PT> P;
+    const int nS = M1 ? 6 : 2;
     const Real dmin = A.delta_min, qsamax = A.max_qsa;
     const int ord = A.ab_order;
+#define QZ_PHASE(i)                                          \
+    do {                                                     \
+        if (tid == 0 && stats) {                             \
+            const unsigned long long tnow = wall_clock64();  \
+            s_ph[i] += tnow - s_ph[10];                      \
+            s_ph[10] = tnow;                                 \
+        }                                                    \
+    } while (0)
+    __syncthreads();
+
+    // per-thread particle cache of the slice (pp is a compile-time index in every loop): cell, node-list
+    // positions, and the geometry that costs divisions; momenta and weights are re-read where needed
+    int cj[PT], cpos[PT];   // cell; node-list positions left | right << 16
+    Real ct[PT], cc[PT], cs[PT], cir[PT], cinvd[PT], cgam[PT];
+    auto fence_cache = [&]() {
+        fence_regs(cj); fence_regs(cpos); fence_regs(ct); fence_regs(cc); fence_regs(cs); fence_regs(cir);
+        fence_regs(cinvd); fence_regs(cgam);
+    };
 
     for (int k = 0; k < K; ++k) {
+        // every per-thread value (addresses, lane roles) is formed inside the slice, not kept live across it
+        int tid_k = threadIdx.x;
+        asm volatile("" : "+v"(tid_k));
+        const int tid = tid_k;
+        const int lane = tid & 63;
+        auto slot_of = [&](int pp) { return w * (PT * 64) + pp * 64 + lane; };
+        // momenta, Delta and weight of cached particle pp (species arrays, L2-resident); the slot is made
+        // opaque so the addresses are formed at each use instead of being held in registers
+        auto load_mom = [&](int pp, Real& PX, Real& PY, Real& D, Real& wi) {
+            int gi = slot_of(pp);
+            asm volatile("" : "+v"(gi));
+            const bool s1 = gi >= np0;
+            const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
+            const int li = s1 ? gi - np0 : gi;
+            PX = st.px[li]; PY = st.py[li]; D = st.dl[li]; wi = st.w[li];
+        };
         // ================================================================ 1. geometry (PlasmaSpecies.cpp:52-70)
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            const int gi = P.slot(p, w, lane);
-            P.cj[p] = -1;
+        for (int pp = 0; pp < PT; ++pp) {
+            const int gi = slot_of(pp);
+            cj[pp] = -1;
             if (gi < NP) {
-                const int s = (nsp > 1 && gi >= sp[1].off) ? 1 : 0;
-                const int li = gi - sp[s].off;
-                P.cspec[p] = s;
-                P.cli[p] = li;
-                const qz_species_state* st = sp[s].st;
-                const Real wi = st->w[li];
-                P.cw[p] = wi;
+                const bool s1 = gi >= np0;
+                const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
+                const int li = s1 ? gi - np0 : gi;
+                const Real wi = st.w[li];
                 if (wi != Real(0)) {
-                    const Real X = st->x[li], Y = st->y[li];
-                    const Real PX = st->px[li], PY = st->py[li], D = st->dl[li];
+                    const Real X = st.x[li], Y = st.y[li];
+                    const Real PX = st.px[li], PY = st.py[li], D = st.dl[li];
                     const Real r = dsqrt(X * X + Y * Y);
                     Real c, s_;
                     if (r > Real(0)) { c = X / r; s_ = Y / r; } else { c = Real(1); s_ = Real(0); }
                     int j; Real t;
-                    grid_weights(gr, gih, N, R, r, j, t);
-                    P.cj[p] = j;
-                    P.ct[p] = t;
-                    P.cc[p] = c;
-                    P.cs[p] = s_;
-                    P.cir[p] = r > Real(0) ? Real(1) / r : Real(0);
-                    P.cgam[p] = (Real(1) + PX * PX + PY * PY + D * D) / (Real(2) * D);
-                    P.cinvd[p] = Real(1) / D;
-                    P.cPX[p] = PX; P.cPY[p] = PY; P.cD[p] = D;
+                    grid_weights(s_r, s_ih, N, R, r, j, t);
+                    cj[pp] = j;
+                    ct[pp] = t;
+                    cc[pp] = c;
+                    cs[pp] = s_;
+                    cir[pp] = r > Real(0) ? Real(1) / r : Real(0);
+                    const Real gam = (Real(1) + PX * PX + PY * PY + D * D) / (Real(2) * D);
+                    cgam[pp] = gam;
+                    cinvd[pp] = Real(1) / D;
                 }
             }
         }
+        QZ_PHASE(0);
         // ================================================================ 2. stable sort by cell
         for (int i = tid; i < kNW * N; i += kBS) s_wcnt[i] = 0;
+        if (tid == 0) s_misc[0] = 0;
         __syncthreads();
         int lrank[PT];
-        const unsigned long long lt_mask = (1ull << lane) - 1ull;
-        int nbits = 1;
-        while ((1 << nbits) < N) ++nbits;
+        {
+            const unsigned long long lt_mask = (1ull << lane) - 1ull;
+            int nbits = 1;
+            while ((1 << nbits) < N) ++nbits;
+            int live = 0;
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            const int key = P.cj[p];
-            unsigned long long mask = __ballot(key >= 0);
-            for (int b = 0; b < nbits; ++b) {
-                const bool bit = key >= 0 && ((key >> b) & 1);
-                const unsigned long long bb = __ballot(bit);
-                mask &= bit ? bb : ~bb;
+            for (int pp = 0; pp < PT; ++pp) {
+                const int key = cj[pp];
+                unsigned long long mask = __ballot(key >= 0);
+                for (int b = 0; b < nbits; ++b) {
+                    const bool bit = key >= 0 && ((key >> b) & 1);
+                    const unsigned long long bb = __ballot(bit);
+                    mask &= bit ? bb : ~bb;
+                }
+                int old = 0;
+                if (key >= 0) old = s_wcnt[w * N + key];
+                const int rk = __popcll(mask & lt_mask);
+                lrank[pp] = old + rk;
+                live += key >= 0 ? 1 : 0;
+                __builtin_amdgcn_wave_barrier();
+                if (key >= 0 && rk == 0) s_wcnt[w * N + key] = static_cast<unsigned short>(old + __popcll(mask));
+                __builtin_amdgcn_wave_barrier();
             }
-            int old = 0;
-            if (key >= 0) old = s_wcnt[w * N + key];
-            const int rk = __popcll(mask & lt_mask);
-            lrank[p] = old + rk;
-            __builtin_amdgcn_wave_barrier();
-            if (key >= 0 && rk == 0) s_wcnt[w * N + key] = static_cast<unsigned short>(old + __popcll(mask));
-            __builtin_amdgcn_wave_barrier();
+            for (int d = 32; d >= 1; d >>= 1) live += __shfl_xor(live, d, 64);
+            if (lane == 0) atomicAdd(&s_misc[0], live);
         }
         __syncthreads();
-        // per cell: exclusive prefix over waves (in place) and the cell total
-        for (int c = tid; c < N; c += kBS) {
+        for (int c = tid; c < N; c += kBS) {   // exclusive prefix over the waves per cell; cell totals
             int run = 0;
             for (int ww = 0; ww < kNW; ++ww) {
                 const int v = s_wcnt[ww * N + c];
@@ -350,88 +473,138 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             s_cs[c + 1] = run;
         }
         __syncthreads();
-        if (w == 0) wave_exclusive_scan(s_cs + 1, N, lane);   // s_cs[c + 1] = start of cell c
-        if (tid == 0) s_cs[0] = 0;                             // empty cell -1
-        __syncthreads();
-        // place the sorted particle indices: (cell, index) order
-#pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            const int j = P.cj[p];
-            if (j >= 0) {
-                const int pos = s_cs[j + 1] + s_wcnt[w * N + j] + lrank[p];
-                s_sidx[pos] = static_cast<unsigned short>(P.slot(p, w, lane));
-                lrank[p] = pos - s_cs[j + 1];   // rank within the cell (index order)
-            }
-        }
-        // live total L = number of depositing particles
-        if (tid == 0) s_misc[0] = 0;
-        __syncthreads();
-        {
-            int cnt = 0;
-#pragma unroll
-            for (int p = 0; p < PT; ++p) cnt += P.cj[p] >= 0 ? 1 : 0;
-            for (int d = 32; d >= 1; d >>= 1) cnt += __shfl_xor(cnt, d, 64);
-            if (lane == 0) atomicAdd(&s_misc[0], cnt);
-        }
-        __syncthreads();
         const int L = s_misc[0];
+        if (w == 0) wave_exclusive_scan(s_cs + 1, N, lane);   // s_cs[c + 1] = start of cell c
         if (tid == 0) {
+            s_cs[0] = 0;
             s_cs[N + 1] = L;
             s_cs[N + 2] = L;
-            if (A.stats) atomicAdd(A.stats, static_cast<unsigned long long>(L));
+            if (stats) atomicAdd(stats, static_cast<unsigned long long>(L));
         }
         __syncthreads();
-        // positions in the node lists: node n holds cells n-1 (right entries) and n (left entries), index order
 #pragma unroll
-        for (int p = 0; p < PT; ++p) {
-            const int j = P.cj[p];
+        for (int pp = 0; pp < PT; ++pp) {
+            const int j = cj[pp];
             if (j >= 0) {
-                const int gi = P.slot(p, w, lane);
-                const int less_m = count_less(s_sidx, s_cs[j], s_cs[j + 1], gi);        // cell j-1
-                const int less_p = count_less(s_sidx, s_cs[j + 2], s_cs[j + 3], gi);   // cell j+1
-                P.cpl[p] = (s_cs[j] + s_cs[j + 1]) + lrank[p] + less_m;
-                P.cpr[p] = (s_cs[j + 1] + s_cs[j + 2]) + lrank[p] + less_p;
+                const int pos = s_cs[j + 1] + s_wcnt[w * N + j] + lrank[pp];
+                s_sidx[pos] = static_cast<unsigned short>(slot_of(pp));
+                lrank[pp] = pos - s_cs[j + 1];   // rank within the cell (index order)
             }
         }
-        const int G = max(1, min(vbuf_len / max(2 * L, 1), 8));
+        __syncthreads();
+        // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index order
+#pragma unroll
+        for (int pp = 0; pp < PT; ++pp) {
+            const int j = cj[pp];
+            cpos[pp] = 0;
+            if (j >= 0) {
+                const int gi = slot_of(pp);
+                const int less_m = count_less(s_sidx, s_cs[j], s_cs[j + 1], gi);
+                const int less_p = count_less(s_sidx, s_cs[j + 2], s_cs[j + 3], gi);
+                const int pl = (s_cs[j] + s_cs[j + 1]) + lrank[pp] + less_m;
+                const int pr = (s_cs[j + 1] + s_cs[j + 2]) + lrank[pp] + less_p;
+                cpos[pp] = pl | (pr << 16);
+            }
+        }
+        const int G = max(1, min(Bcap / max(2 * L, 1), 8));
+        QZ_PHASE(1);
 
         // ================================================================ 3. deposit (node sums in index order)
-        for (int c0 = 0; c0 < ncomp; c0 += G) {
-            const int g1 = min(G, ncomp - c0);
-            __syncthreads();
-#pragma unroll
-            for (int p = 0; p < PT; ++p) {
-                const int j = P.cj[p];
-                if (j >= 0) {
-                    const SpecInfo& S = sp[P.cspec[p]];
-                    for (int g = 0; g < g1; ++g) {
-                        const int comp = comps[c0 + g];
-                        s_vbuf[g * 2 * L + P.cpl[p]] = dep_value(comp, 0, j, P.ct[p], P.cc[p], P.cs[p], P.cir[p], P.cinvd[p], P.cgam[p],
-                                                                 P.cPX[p], P.cPY[p], P.cD[p], P.cw[p], S.q, S.qm, giv, gih);
-                        s_vbuf[g * 2 * L + P.cpr[p]] = dep_value(comp, 1, j, P.ct[p], P.cc[p], P.cs[p], P.cir[p], P.cinvd[p], P.cgam[p],
-                                                                 P.cPX[p], P.cPY[p], P.cD[p], P.cw[p], S.q, S.qm, giv, gih);
-                    }
-                }
-            }
-            __syncthreads();
-            for (int n = tid; n < M; n += kBS) {
-                const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
-                for (int g = 0; g < g1; ++g) {
+        // Components run in a fixed compile-time sequence (the order of s_comp); every G of them share one
+        // pass: contributions into LDS lists, a barrier, the per-node sums, a barrier.
+        Real* __restrict__ Dg = A.diag + static_cast<size_t>(k) * M * A.ncd;
+        const int ncd = A.ncd;
+        {
+            int gcur = 0, c0 = 0;
+            auto reduce = [&]() {
+                __syncthreads();
+                for (int item = tid; item < M * gcur; item += kBS) {
+                    const int g = item / M, n = item - g * M;
+                    const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
+                    const Real* vb = B + g * 2 * L;
                     Real acc = Real(0);
-                    const Real* vb = s_vbuf + g * 2 * L;
                     for (int e = e0; e < e1; ++e) acc += vb[e];
-                    WS(comps[c0 + g])[n] = acc;
+                    const int comp = s_comp[c0 + g];
+                    Real* Dn = Dg + static_cast<size_t>(n) * ncd;
+                    if (comp == W_NE0) Dn[D_NE0] = acc;
+                    else if (comp == W_NE1R) Dn[D_NE1R] = acc;
+                    else if (comp == W_NE1I) Dn[D_NE1I] = acc;
+                    else if (comp == W_NI1R) Dn[D_NI1R] = acc;
+                    else if (comp == W_NI1I) Dn[D_NI1I] = acc;
+                    else if (comp == W_NI0) {   // combine_sources adds the positive background (Simulation.cpp:441)
+                        const Real bgr0 = A.bg[3 * M + n];
+                        Dn[D_NI0] = bgr0 > Real(0) ? acc + bgr0 : acc;
+                    } else WS(comp)[n] = acc;
                 }
+                __syncthreads();
+                c0 += gcur;
+                gcur = 0;
+            };
+            // contributions of every cached particle to component C (straight-line over the particles)
+            auto comp_pass = [&](auto cst) {
+                constexpr int C = decltype(cst)::value;
+                Real* vb = B + gcur * 2 * L;
+                fence_cache();
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp) {
+                    const int j = cj[pp];
+                    if (j < 0) continue;
+                    Real PX, PY, D, wi;
+                    load_mom(pp, PX, PY, D, wi);
+                    const bool s1 = slot_of(pp) >= np0;
+                    const Real q = s1 ? q1 : q0, qm = s1 ? qm1 : qm0;
+                    const int pl = cpos[pp] & 0xffff, pr = cpos[pp] >> 16;
+                    const Real ih = s_ih[j];
+                    const Real pz = cgam[pp] - D;
+                    vb[pl] = dep_value(C, 0, j, ct[pp], cc[pp], cs[pp], cir[pp], cinvd[pp], cgam[pp], PX, PY, pz, wi, q,
+                                       qm, s_iv[j], ih);
+                    vb[pr] = dep_value(C, 1, j, ct[pp], cc[pp], cs[pp], cir[pp], cinvd[pp], cgam[pp], PX, PY, pz, wi, q,
+                                       qm, s_iv[j + 1], ih);
+                }
+                if (++gcur == G) reduce();
+            };
+            comp_pass(IC<W_CHI0>{});
+            comp_pass(IC<W_RHOT0>{});
+            comp_pass(IC<W_DRHO0>{});
+            comp_pass(IC<W_JZ0>{});
+            if (has_neg) comp_pass(IC<W_NE0>{});
+            if (has_pos) comp_pass(IC<W_NI0>{});
+            comp_pass(IC<W_JPR1>{});
+            comp_pass(IC<W_JPI1>{});
+            if (M1) {
+                comp_pass(IC<W_CHI1R>{});
+                comp_pass(IC<W_CHI1I>{});
+                comp_pass(IC<W_RHOT1R>{});
+                comp_pass(IC<W_RHOT1I>{});
+                comp_pass(IC<W_DRHO1R>{});
+                comp_pass(IC<W_DRHO1I>{});
+                comp_pass(IC<W_JZ1R>{});
+                comp_pass(IC<W_JZ1I>{});
+                if (has_neg) { comp_pass(IC<W_NE1R>{}); comp_pass(IC<W_NE1I>{}); }
+                if (has_pos) { comp_pass(IC<W_NI1R>{}); comp_pass(IC<W_NI1I>{}); }
+                comp_pass(IC<W_JPR0>{});
+                comp_pass(IC<W_JPI0>{});
+                comp_pass(IC<W_JPR2>{});
+                comp_pass(IC<W_JPI2>{});
+            }
+            if (gcur > 0) reduce();
+        }
+        // densities of absent species are zero, plus the positive background for n_i
+        for (int n = tid; n < M; n += kBS) {
+            Real* Dn = Dg + static_cast<size_t>(n) * ncd;
+            if (!has_neg) {
+                Dn[D_NE0] = Real(0);
+                if (M1) { Dn[D_NE1R] = Real(0); Dn[D_NE1I] = Real(0); }
+            }
+            if (!has_pos) {
+                const Real bgr0 = A.bg[3 * M + n];
+                Dn[D_NI0] = bgr0 > Real(0) ? Real(0) + bgr0 : Real(0);
+                if (M1) { Dn[D_NI1R] = Real(0); Dn[D_NI1I] = Real(0); }
             }
         }
         __syncthreads();
+        QZ_PHASE(2);
 
-        // absent density components stay zero (zero_sources, Simulation.cpp:394-406)
-        for (int n = tid; n < M; n += kBS) {
-            if (!has_neg) { WS(W_NE0)[n] = Real(0); WS(W_NE1R)[n] = Real(0); WS(W_NE1I)[n] = Real(0); }
-            if (!has_pos) { WS(W_NI0)[n] = Real(0); WS(W_NI1R)[n] = Real(0); WS(W_NI1I)[n] = Real(0); }
-        }
-        __syncthreads();
         // ================================================================ 4. combine_sources (Simulation.cpp:408-457)
         {
             const int KL = K;
@@ -439,138 +612,126 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             const Real* __restrict__ bs = A.bsrc;
             const Real* __restrict__ bg = A.bg;
             const bool ngp = A.ngp != 0;
-            auto b = [&](int kk, int j, int comp) { return bs[(static_cast<size_t>(kk) * M + j) * ncb + comp]; };
+            auto bb = [&](int kk, int j, int comp) { return bs[(static_cast<size_t>(kk) * M + j) * ncb + comp]; };
             for (int j = tid; j < M; j += kBS) {
                 auto dxi_b = [&](int comp) {
                     if (ngp) {
-                        const Real prev = (k > 0) ? b(k - 1, j, comp) : Real(0);
-                        return (b(k, j, comp) - prev) * idxi;
+                        const Real prev = (k > 0) ? bb(k - 1, j, comp) : Real(0);
+                        return (bb(k, j, comp) - prev) * idxi;
                     }
                     if (KL == 1) return Real(0);
-                    if (k == 0) return (b(1, j, comp) - b(0, j, comp)) * idxi;
-                    if (k == KL - 1) return (b(KL - 1, j, comp) - b(KL - 2, j, comp)) * idxi;
-                    return Real(0.5) * (b(k + 1, j, comp) - b(k - 1, j, comp)) * idxi;
+                    if (k == 0) return (bb(1, j, comp) - bb(0, j, comp)) * idxi;
+                    if (k == KL - 1) return (bb(KL - 1, j, comp) - bb(KL - 2, j, comp)) * idxi;
+                    return Real(0.5) * (bb(k + 1, j, comp) - bb(k - 1, j, comp)) * idxi;
                 };
-                const Real bgt0 = bg[j], bgr0 = bg[3 * M + j];
-                WS(W_RHOT0)[j] += bgt0 + b(k, j, B_RT0);
-                WS(W_DRHO0)[j] += dxi_b(B_RT0);
-                WS(W_JZ0)[j] += b(k, j, B_JZ0);
-                WS(W_JPR1)[j] += b(k, j, B_JP1R);
-                WS(W_JPI1)[j] += b(k, j, B_JP1I);
-                if (bgr0 > Real(0)) WS(W_NI0)[j] += bgr0;
+                BA(LY::WPSI0)[j] = WS(W_RHOT0)[j] + (bg[j] + bb(k, j, B_RT0));
+                BA(LY::WEZ0)[j] = WS(W_DRHO0)[j] + dxi_b(B_RT0);
+                WS(W_JZ0)[j] += bb(k, j, B_JZ0);
+                BA(LY::JP + 0)[j] = WS(W_JPR1)[j] + bb(k, j, B_JP1R);
+                BA(LY::JP + 1)[j] = WS(W_JPI1)[j] + bb(k, j, B_JP1I);
                 if (M1) {
-                    WS(W_RHOT1R)[j] += bg[M + j] + b(k, j, B_RT1R);
-                    WS(W_RHOT1I)[j] += bg[2 * M + j] + b(k, j, B_RT1I);
-                    WS(W_DRHO1R)[j] += dxi_b(B_RT1R);
-                    WS(W_DRHO1I)[j] += dxi_b(B_RT1I);
-                    WS(W_JZ1R)[j] += b(k, j, B_JZ1R);
-                    WS(W_JZ1I)[j] += b(k, j, B_JZ1I);
-                    WS(W_JPR0)[j] += b(k, j, B_JP0R);
-                    WS(W_JPI0)[j] += b(k, j, B_JP0I);
-                    WS(W_JPR2)[j] += b(k, j, B_JP2R);
-                    WS(W_JPI2)[j] += b(k, j, B_JP2I);
+                    BA(LY::WPSI1R)[j] = WS(W_RHOT1R)[j] + (bg[M + j] + bb(k, j, B_RT1R));
+                    BA(LY::WPSI1I)[j] = WS(W_RHOT1I)[j] + (bg[2 * M + j] + bb(k, j, B_RT1I));
+                    BA(LY::WEZ1R)[j] = WS(W_DRHO1R)[j] + dxi_b(B_RT1R);
+                    BA(LY::WEZ1I)[j] = WS(W_DRHO1I)[j] + dxi_b(B_RT1I);
+                    WS(W_JZ1R)[j] += bb(k, j, B_JZ1R);
+                    WS(W_JZ1I)[j] += bb(k, j, B_JZ1I);
+                    BA(LY::JP + 2)[j] = WS(W_JPR0)[j] + bb(k, j, B_JP0R);
+                    BA(LY::JP + 3)[j] = WS(W_JPI0)[j] + bb(k, j, B_JP0I);
+                    BA(LY::JP + 4)[j] = WS(W_JPR2)[j] + bb(k, j, B_JP2R);
+                    BA(LY::JP + 5)[j] = WS(W_JPI2)[j] + bb(k, j, B_JP2I);
+                }
+                // operators of the solves and of the chi factors
+                BA(LY::L0A)[j] = A.op_a[OP_L0][j];
+                BA(LY::L0M)[j] = A.fac_l0_m[j];
+                BA(LY::L0CP)[j] = A.fac_l0_cp[j];
+                BA(LY::L1A)[j] = A.op_a[OP_L1D][j];
+                BA(LY::L1B)[j] = A.op_b[OP_L1D][j];
+                BA(LY::L1C)[j] = A.op_c[OP_L1D][j];
+                BA(LY::CHI0)[j] = WS(W_CHI0)[j];
+                if (M1) {
+                    BA(LY::L1M)[j] = A.fac_l1_m[j];
+                    BA(LY::L1CP)[j] = A.fac_l1_cp[j];
+                    BA(LY::L0B)[j] = A.op_b[OP_L0][j];
+                    BA(LY::L0C)[j] = A.op_c[OP_L0][j];
+                    BA(LY::L2A)[j] = A.op_a[OP_L2D][j];
+                    BA(LY::L2B)[j] = A.op_b[OP_L2D][j];
+                    BA(LY::L2C)[j] = A.op_c[OP_L2D][j];
                 }
             }
         }
         __syncthreads();
-
-        // ================================================================ 5a. B_z right-hand side (compute_bz, Simulation.cpp:506-526)
-        {
-            const Real* __restrict__ gm = A.grad_m;
-            const Real* __restrict__ g0 = A.grad_0;
-            const Real* __restrict__ gp = A.grad_p;
-            const Real* __restrict__ ir = A.inv_r;
-            auto grad = [&](const Real* f, int j) -> Real {
-                if (j == 0) return Real(0);
-                if (j == N) return g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
-                return gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
-            };
-            auto grad_odd = [&](const Real* f, int j) -> Real {
-                if (j == 0) return f[1] * ir[0];
-                if (j == N) return g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
-                return gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
-            };
-            auto over_r = [&](const Real* f, int j) -> Real { return (j == 0 ? f[1] : f[j]) * ir[j]; };
-            for (int j = tid; j < M; j += kBS) {
-                const Real t1 = grad_odd(WS(W_JPI1), j), t2 = over_r(WS(W_JPI1), j);
-                WS(W_T3)[j] = -(t1 + t2);
-                if (M1) {
-                    const Real gr0 = grad_odd(WS(W_JPR2), j), gi0 = over_r(WS(W_JPR2), j);
-                    const Real gr1 = grad_odd(WS(W_JPI2), j), gi1 = over_r(WS(W_JPI2), j);
-                    const Real gr2 = grad(WS(W_JPR0), j), gi2 = grad(WS(W_JPI0), j);
-                    const Real D1r = gr0 + Real(2) * gi0, D1i = gr1 + Real(2) * gi1;
-                    const Real Dmr = gr2, Dmi = gi2;
-                    WS(W_T4)[j] = -Real(0.5) * (D1i + Dmi);
-                    WS(W_RHS)[j] = Real(0.5) * (D1r - Dmr);
-                }
+        QZ_PHASE(3);
+        // B_z right-hand side (compute_bz, Simulation.cpp:506-526)
+        for (int j = tid; j < M; j += kBS) {
+            const Real t1 = stn.grad_odd(BA(LY::JP + 1), j), t2 = stn.over_r(BA(LY::JP + 1), j);
+            BA(LY::WBZ0)[j] = -(t1 + t2);
+            if (M1) {
+                const Real gr0 = stn.grad_odd(BA(LY::JP + 4), j), gi0 = stn.over_r(BA(LY::JP + 4), j);
+                const Real gr1 = stn.grad_odd(BA(LY::JP + 5), j), gi1 = stn.over_r(BA(LY::JP + 5), j);
+                const Real gr2 = stn.grad(BA(LY::JP + 2), j), gi2 = stn.grad(BA(LY::JP + 3), j);
+                const Real D1r = gr0 + Real(2) * gi0, D1i = gr1 + Real(2) * gi1;
+                const Real Dmr = gr2, Dmi = gi2;
+                BA(LY::WBZ1R)[j] = -Real(0.5) * (D1i + Dmi);
+                BA(LY::WBZ1I)[j] = Real(0.5) * (D1r - Dmr);
             }
         }
         __syncthreads();
+        QZ_PHASE(4);
 
-        // ================================================================ 5b. psi, E_z, B_z solves || chi factors
+        // ================================================================ 5a. psi, E_z, B_z solves || chi factors
         if (w == 0) {
             // lanes: 0 psi0, 1 ez0, 2 bz0 (L0);  m1: 3 psi1r, 4 psi1i, 5 ez1r, 6 ez1i, 7 bz1r, 8 bz1i (L1D)
             const int njobs = M1 ? 9 : 3;
             if (lane < njobs) {
-                int op, rhs, out; Real scale;
-                switch (lane) {
-                    case 0: op = OP_L0; rhs = W_RHOT0; out = W_PSI0; scale = Real(-1); break;
-                    case 1: op = OP_L0; rhs = W_DRHO0; out = W_EZ0; scale = Real(-1); break;
-                    case 2: op = OP_L0; rhs = W_T3; out = W_BZ0; scale = Real(1); break;
-                    case 3: op = OP_L1D; rhs = W_RHOT1R; out = W_PSI1R; scale = Real(-1); break;
-                    case 4: op = OP_L1D; rhs = W_RHOT1I; out = W_PSI1I; scale = Real(-1); break;
-                    case 5: op = OP_L1D; rhs = W_DRHO1R; out = W_EZ1R; scale = Real(-1); break;
-                    case 6: op = OP_L1D; rhs = W_DRHO1I; out = W_EZ1I; scale = Real(-1); break;
-                    case 7: op = OP_L1D; rhs = W_T4; out = W_BZ1R; scale = Real(1); break;
-                    default: op = OP_L1D; rhs = W_RHS; out = W_BZ1I; scale = Real(1); break;
-                }
-                const bool l0 = op == OP_L0;
-                thomas_apply(A.op_a[op], l0 ? A.fac_l0_m : A.fac_l1_m, l0 ? A.fac_l0_cp : A.fac_l1_cp, A.op_b[op][0],
-                             WS(rhs), scale, !l0, true, false, Real(0), WS(W_DP0 + lane), WS(out), M);
+                const bool l0 = lane < 3;
+                const int slot = l0 ? LY::WPSI0 + lane : LY::WPSI1R + (lane - 3);
+                const Real scale = (lane == 2 || lane >= 7) ? Real(1) : Real(-1);
+                const Real b0 = l0 ? A.op_b[OP_L0][0] : A.op_b[OP_L1D][0];
+                thomas_lds(BA(l0 ? LY::L0A : LY::L1A), BA(l0 ? LY::L0M : LY::L1M), BA(l0 ? LY::L0CP : LY::L1CP), b0,
+                           BA(slot), BA(slot), scale, !l0, true, false, Real(0), M);
             }
         } else if (w == 1) {
-            // per-slice factors of (L - chi0): lanes 0 L1D, 1 L1F, m1: 2 L0, 3 L2D
+            // this slice's factors of (L - chi0): lanes 0 L1D, 1 L1F, m1: 2 L0, 3 L2D
             const int nf = M1 ? 4 : 2;
             if (lane < nf) {
+                const Real* a = lane < 2 ? BA(LY::L1A) : lane == 2 ? BA(LY::L0A) : BA(LY::L2A);
+                const Real* b = lane < 2 ? BA(LY::L1B) : lane == 2 ? BA(LY::L0B) : BA(LY::L2B);
+                const Real* c = lane < 2 ? BA(LY::L1C) : lane == 2 ? BA(LY::L0C) : BA(LY::L2C);
                 const int op = lane == 0 ? OP_L1D : lane == 1 ? OP_L1F : lane == 2 ? OP_L0 : OP_L2D;
-                const int fm = lane == 0 ? W_FM_L1D : lane == 1 ? W_FM_L1F : lane == 2 ? W_FM_L0 : W_FM_L2D;
-                const bool dir0 = op != OP_L0;
-                const bool dirN = op != OP_L1F;
-                const Real bw0 = thomas_factor(A.op_a[op], A.op_b[op], A.op_c[op], WS(W_CHI0), dir0, dirN, WS(fm), WS(fm + 1), M);
-                // bw0 of each operator is kept in LDS for the Picard solves
-                reinterpret_cast<Real*>(s_misc + 8)[lane] = bw0;
+                const Real aN = A.op_a[op][N], bN = A.op_b[op][N];
+                const bool dir0 = op != OP_L0, dirN = op != OP_L1F;
+                s_bw0[lane] = thomas_factor(a, b, c, BA(LY::CHI0), aN, bN, dir0, dirN, WS(W_FAC0 + 2 * lane),
+                                            WS(W_FAC0 + 2 * lane + 1), M);
             }
         }
         __syncthreads();
+        QZ_PHASE(5);
 
-        // ================================================================ 5c. W+ (compute_wplus, Simulation.cpp:477-498)
+        // ================================================================ 5b. field outputs, W+ (compute_wplus, Simulation.cpp:477-498)
         {
-            const Real* __restrict__ gm = A.grad_m;
-            const Real* __restrict__ g0 = A.grad_0;
-            const Real* __restrict__ gp = A.grad_p;
-            const Real* __restrict__ ir = A.inv_r;
+            Real* __restrict__ F = A.fld + static_cast<size_t>(k) * M * A.ncf;
+            const int ncf = A.ncf;
             for (int j = tid; j < M; j += kBS) {
-                const Real* f = WS(W_PSI0);
-                Real t1;
-                if (j == 0) t1 = Real(0);
-                else if (j == N) t1 = g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
-                else t1 = gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
+                Real* Fj = F + static_cast<size_t>(j) * ncf;
+                Real* Dj = Dg + static_cast<size_t>(j) * ncd;
+                const Real ez0 = BA(LY::WEZ0)[j], bz0 = BA(LY::WBZ0)[j];
+                Dj[D_PSI0] = BA(LY::WPSI0)[j];
+                Fj[F_EZ0] = ez0; Fj[F_BZ0] = bz0;
+                WS(W_EZ0)[j] = ez0; WS(W_BZ0)[j] = bz0;
+                const Real t1 = stn.grad(BA(LY::WPSI0), j);
                 WS(W_WPR1)[j] = -t1;
                 WS(W_WPI1)[j] = Real(0);
                 if (M1) {
-                    const Real* fr = WS(W_PSI1R);
-                    const Real* fi = WS(W_PSI1I);
-                    Real t2, t4;
-                    if (j == 0) { t2 = fr[1] * ir[0]; t4 = fi[1] * ir[0]; }
-                    else if (j == N) {
-                        t2 = g0[N] * fr[N] + gm[N] * fr[N - 1] + gp[N] * fr[N - 2];
-                        t4 = g0[N] * fi[N] + gm[N] * fi[N - 1] + gp[N] * fi[N - 2];
-                    } else {
-                        t2 = gm[j] * fr[j - 1] + g0[j] * fr[j] + gp[j] * fr[j + 1];
-                        t4 = gm[j] * fi[j - 1] + g0[j] * fi[j] + gp[j] * fi[j + 1];
-                    }
-                    const Real t3 = (j == 0 ? fr[1] : fr[j]) * ir[j];
-                    const Real rh = (j == 0 ? fi[1] : fi[j]) * ir[j];
+                    const Real* fr = BA(LY::WPSI1R);
+                    const Real* fi = BA(LY::WPSI1I);
+                    Dj[D_PSI1R] = fr[j]; Dj[D_PSI1I] = fi[j];
+                    const Real e1r = BA(LY::WEZ1R)[j], e1i = BA(LY::WEZ1I)[j];
+                    const Real z1r = BA(LY::WBZ1R)[j], z1i = BA(LY::WBZ1I)[j];
+                    Fj[F_EZ1R] = e1r; Fj[F_EZ1I] = e1i; Fj[F_BZ1R] = z1r; Fj[F_BZ1I] = z1i;
+                    WS(W_EZ1R)[j] = e1r; WS(W_EZ1I)[j] = e1i; WS(W_BZ1R)[j] = z1r; WS(W_BZ1I)[j] = z1i;
+                    const Real t2 = stn.grad_odd(fr, j), t3 = stn.over_r(fr, j);
+                    const Real t4 = stn.grad_odd(fi, j), rh = stn.over_r(fi, j);
                     WS(W_WPR2)[j] = -(t2 - t3);
                     WS(W_WPI2)[j] = -(t4 - rh);
                     WS(W_WPR0)[j] = -(t2 + t3);
@@ -579,194 +740,237 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
+        // W+, E_z, B_z into LDS for the S+ gather
+        for (int j = tid; j < M; j += kBS) {
+            BA(LY::SW + 0)[j] = WS(W_WPR1)[j];
+            BA(LY::SW + 1)[j] = WS(W_WPI1)[j];
+            BA(LY::SEZ)[j] = WS(W_EZ0)[j];
+            BA(LY::SBZ)[j] = WS(W_BZ0)[j];
+            if (M1) {
+                BA(LY::SW + 2)[j] = WS(W_WPR0)[j]; BA(LY::SW + 3)[j] = WS(W_WPI0)[j];
+                BA(LY::SW + 4)[j] = WS(W_WPR2)[j]; BA(LY::SW + 5)[j] = WS(W_WPI2)[j];
+                BA(LY::SEZ + 1)[j] = WS(W_EZ1R)[j]; BA(LY::SEZ + 2)[j] = WS(W_EZ1I)[j];
+                BA(LY::SBZ + 1)[j] = WS(W_BZ1R)[j]; BA(LY::SBZ + 2)[j] = WS(W_BZ1I)[j];
+            }
+        }
+        QZ_PHASE(6);
 
-        // ================================================================ 5d. S+ deposit (PlasmaSpecies::deposit_S, PlasmaSpecies.cpp:516-558)
+        // ================================================================ 5c. S+ deposit (PlasmaSpecies::deposit_S, PlasmaSpecies.cpp:516-558)
         {
-            const int GS = max(1, min(G, nS));
-            for (int c0 = 0; c0 < nS; c0 += GS) {
-                const int g1 = min(GS, nS - c0);
-                __syncthreads();
+            Real* VB = BA(LY::NSDEP);
+            const int vcap = Bcap - LY::NSDEP * Mp;
+            const int GS = max(1, min(vcap / max(2 * L, 1), nS));
+            __syncthreads();
+            // gather W+, E_z, B_z (PlasmaSpecies.cpp:75-89, without B+) and form the particle's a_x, a_y
+            Real cax[PT], cay[PT];
+            fence_cache();
 #pragma unroll
-                for (int p = 0; p < PT; ++p) {
-                    const int j = P.cj[p];
+            for (int pp = 0; pp < PT; ++pp) {
+                const int j = cj[pp];
+                if (j < 0) continue;
+                Real PX, PY, D, wi;
+                load_mom(pp, PX, PY, D, wi);
+                const bool s1 = slot_of(pp) >= np0;
+                const Real qm = s1 ? qm1 : qm0;
+                const Real t = ct[pp], c = cc[pp], sn = cs[pp];
+                const Real invd = cinvd[pp];
+                Real Wx, Wy;
+                if (M1)
+                    eval_vector(lin(BA(LY::SW + 2), j, t), lin(BA(LY::SW + 3), j, t), lin(BA(LY::SW + 0), j, t),
+                                lin(BA(LY::SW + 1), j, t), lin(BA(LY::SW + 4), j, t), lin(BA(LY::SW + 5), j, t), c, sn, true,
+                                Wx, Wy);
+                else
+                    eval_vector(Real(0), Real(0), lin(BA(LY::SW + 0), j, t), lin(BA(LY::SW + 1), j, t), Real(0), Real(0), c,
+                                sn, false, Wx, Wy);
+                const Real Ez = eval_scalar(lin(BA(LY::SEZ), j, t), M1 ? lin(BA(LY::SEZ + 1), j, t) : Real(0),
+                                            M1 ? lin(BA(LY::SEZ + 2), j, t) : Real(0), c, sn, M1);
+                const Real Bz = eval_scalar(lin(BA(LY::SBZ), j, t), M1 ? lin(BA(LY::SBZ + 1), j, t) : Real(0),
+                                            M1 ? lin(BA(LY::SBZ + 2), j, t) : Real(0), c, sn, M1);
+                const Real dA = Real(0);
+                const Real gam = cgam[pp];
+                const Real fp = -Real(0.5) * qm * qm * dA * invd;
+                const Real dD = qm * ((PX * Wx + PY * Wy) * invd - Ez);
+                cax[pp] = (qm * (gam * Wx + PY * Bz) * invd + fp * c) * invd - PX * dD * invd * invd;
+                cay[pp] = (qm * (gam * Wy - PX * Bz) * invd + fp * sn) * invd - PY * dD * invd * invd;
+            }
+            int gcur = 0, c0 = 0;
+            auto reduce = [&]() {
+                __syncthreads();
+                for (int item = tid; item < M * gcur; item += kBS) {
+                    const int g = item / M, n = item - g * M;
+                    const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
+                    const Real* vb = VB + g * 2 * L;
+                    Real acc = Real(0);
+                    for (int e = e0; e < e1; ++e) acc += vb[e];
+                    WS(W_SR1 + c0 + g)[n] = acc;
+                }
+                __syncthreads();
+                c0 += gcur;
+                gcur = 0;
+            };
+            // component SC of S+ (0 r1, 1 i1, 2 r0, 3 i0, 4 r2, 5 i2) for every cached particle
+            auto s_pass = [&](auto cst) {
+                constexpr int SC = decltype(cst)::value;
+                constexpr int mode = SC < 2 ? 1 : (SC < 4 ? 0 : 2);
+                Real* vb = VB + gcur * 2 * L;
+                fence_cache();
+                fence_regs(cax); fence_regs(cay);
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp) {
+                    const int j = cj[pp];
                     if (j < 0) continue;
-                    const SpecInfo& S = sp[P.cspec[p]];
-                    const Real t = P.ct[p], c = P.cc[p], sn = P.cs[p];
-                    const Real qq = S.q, qm = S.qm;
-                    const Real PX = P.cPX[p], PY = P.cPY[p];
-                    const Real wi = P.cw[p];
-                    // gather (PlasmaSpecies.cpp:75-89) without B+
-                    Real Wx, Wy;
-                    eval_vector(lin(WS(W_WPR0), j, t), lin(WS(W_WPI0), j, t), lin(WS(W_WPR1), j, t), lin(WS(W_WPI1), j, t),
-                                lin(WS(W_WPR2), j, t), lin(WS(W_WPI2), j, t), c, sn, M1, Wx, Wy);
-                    const Real Ez = eval_scalar(lin(WS(W_EZ0), j, t), M1 ? lin(WS(W_EZ1R), j, t) : Real(0),
-                                                M1 ? lin(WS(W_EZ1I), j, t) : Real(0), c, sn, M1);
-                    const Real Bz = eval_scalar(lin(WS(W_BZ0), j, t), M1 ? lin(WS(W_BZ1R), j, t) : Real(0),
-                                                M1 ? lin(WS(W_BZ1I), j, t) : Real(0), c, sn, M1);
-                    const Real dA = Real(0);
-                    const Real gam = P.cgam[p];
-                    const Real invd = P.cinvd[p];
-                    const Real fp = -Real(0.5) * qm * qm * dA * invd;
+                    Real PX, PY, D, wi;
+                    load_mom(pp, PX, PY, D, wi);
+                    const bool s1 = slot_of(pp) >= np0;
+                    const Real qq = s1 ? q1 : q0;
+                    const Real t = ct[pp], c = cc[pp], sn = cs[pp];
+                    const Real invd = cinvd[pp];
                     const Real ux = PX * invd, uy = PY * invd;
                     const Real ur = c * ux + sn * uy, uth = c * uy - sn * ux;
-                    const Real dD = qm * ((PX * Wx + PY * Wy) * invd - Ez);
-                    const Real ax = (qm * (gam * Wx + PY * Bz) * invd + fp * c) * invd - PX * dD * invd * invd;
-                    const Real ay = (qm * (gam * Wy - PX * Bz) * invd + fp * sn) * invd - PY * dD * invd * invd;
+                    const Real ck = mode == 1 ? c : (mode == 0 ? Real(1) : c * c - sn * sn);
+                    const Real sk = mode == 1 ? sn : (mode == 0 ? Real(0) : Real(2) * c * sn);
+                    const Real ax = cax[pp], ay = cay[pp];
                     const Real wq = wi * qq;
-                    const Real ih = gih[j];
-                    for (int g = 0; g < g1; ++g) {
-                        const int sc = c0 + g;              // 0 r1, 1 i1, 2 r0, 3 i0, 4 r2, 5 i2
-                        const int mode = sc < 2 ? 1 : (sc < 4 ? 0 : 2);
-                        const Real ck = mode == 1 ? c : (mode == 0 ? Real(1) : c * c - sn * sn);
-                        const Real sk = mode == 1 ? sn : (mode == 0 ? Real(0) : Real(2) * c * sn);
-                        for (int cc = 0; cc < 2; ++cc) {
-                            const int jj = j + cc;
-                            const Real iv = giv[jj] * wq;
-                            const Real W = cc == 0 ? Real(1) - t : t;
-                            const Real dW = cc == 0 ? -ih : ih;
-                            Real Ar = ax * W + ux * ur * dW;
-                            Real Ai = ay * W + uy * ur * dW;
-                            if (mode != 0) {
-                                const Real Wor = (j == 0) ? (cc == 0 ? Real(0) : ih) : W * P.cir[p];
-                                const Real f2 = mode * uth * Wor;
-                                Ar += f2 * uy;
-                                Ai -= f2 * ux;
-                            }
-                            const Real v = (sc & 1) == 0 ? iv * (Ar * ck + Ai * sk) : iv * (Ai * ck - Ar * sk);
-                            s_vbuf[g * 2 * L + (cc == 0 ? P.cpl[p] : P.cpr[p])] = v;
+                    const Real ih = s_ih[j];
+                    const int pl = cpos[pp] & 0xffff, pr = cpos[pp] >> 16;
+#pragma unroll
+                    for (int side = 0; side < 2; ++side) {
+                        const Real iv = s_iv[j + side] * wq;
+                        const Real W = side == 0 ? Real(1) - t : t;
+                        const Real dW = side == 0 ? -ih : ih;
+                        Real Ar = ax * W + ux * ur * dW;
+                        Real Ai = ay * W + uy * ur * dW;
+                        if (mode != 0) {
+                            const Real Wor = (j == 0) ? (side == 0 ? Real(0) : ih) : W * cir[pp];
+                            const Real f2 = mode * uth * Wor;
+                            Ar += f2 * uy;
+                            Ai -= f2 * ux;
                         }
+                        const Real v = (SC & 1) == 0 ? iv * (Ar * ck + Ai * sk) : iv * (Ai * ck - Ar * sk);
+                        vb[side == 0 ? pl : pr] = v;
                     }
+                }
+                if (++gcur == GS) reduce();
+            };
+            s_pass(IC<0>{});
+            s_pass(IC<1>{});
+            if (M1) {
+                s_pass(IC<2>{});
+                s_pass(IC<3>{});
+                s_pass(IC<4>{});
+                s_pass(IC<5>{});
+            }
+            if (gcur > 0) reduce();
+        }
+        __syncthreads();
+        QZ_PHASE(7);
+
+        // ================================================================ 5d. gz and the Picard iterations (Simulation.cpp:535-591)
+        for (int j = tid; j < M; j += kBS) {
+            // ((d_x + i d_y) J_z)_k + S_k
+            const Real t1 = stn.grad(WS(W_JZ0), j);
+            BA(LY::PGZ + 0)[j] = t1 + WS(W_SR1)[j];
+            BA(LY::PGZ + 1)[j] = WS(W_SI1)[j];
+            if (M1) {
+                const Real* fr = WS(W_JZ1R);
+                const Real* fi = WS(W_JZ1I);
+                const Real t2 = stn.grad_odd(fr, j), t3 = stn.over_r(fr, j);
+                const Real t4 = stn.grad_odd(fi, j), rh = stn.over_r(fi, j);
+                BA(LY::PGZ + 4)[j] = (t2 - t3) + WS(W_SR2)[j];
+                BA(LY::PGZ + 5)[j] = (t4 - rh) + WS(W_SI2)[j];
+                BA(LY::PGZ + 2)[j] = (t2 + t3) + WS(W_SR0)[j];
+                BA(LY::PGZ + 3)[j] = -(t4 + rh) + WS(W_SI0)[j];
+                BA(LY::PCHI + 0)[j] = WS(W_CHI1R)[j];
+                BA(LY::PCHI + 1)[j] = WS(W_CHI1I)[j];
+            }
+            BA(LY::PA_L1D)[j] = A.op_a[OP_L1D][j];
+            BA(LY::PA_L1F)[j] = A.op_a[OP_L1F][j];
+            for (int f = 0; f < (M1 ? 8 : 4); ++f) BA(LY::PF + f)[j] = WS(W_FAC0 + f)[j];
+            for (int f = 0; f < (M1 ? 6 : 2); ++f) BA(LY::PBP + f)[j] = WS(W_BPR1 + f)[j];
+            if (M1) {
+                BA(LY::PA_L0)[j] = A.op_a[OP_L0][j];
+                BA(LY::PA_L2D)[j] = A.op_a[OP_L2D][j];
+            }
+        }
+        {
+            const Real fcw = A.flux_coef * WS(W_JZ0)[N];
+            const int niter = M1 ? A.picard : 1;
+            for (int it = 0; it < niter; ++it) {
+                __syncthreads();
+                for (int j = tid; j < M; j += kBS) {
+                    Real r1 = -BA(LY::PGZ + 1)[j], i1 = BA(LY::PGZ + 0)[j];
+                    if (M1) {
+                        Real r0 = -BA(LY::PGZ + 3)[j], i0 = BA(LY::PGZ + 2)[j];
+                        Real r2 = -BA(LY::PGZ + 5)[j], i2 = BA(LY::PGZ + 4)[j];
+                        const Real cr = BA(LY::PCHI)[j], ci = BA(LY::PCHI + 1)[j];
+                        const Real b1r = BA(LY::PBP + 0)[j], b1i = BA(LY::PBP + 1)[j];
+                        const Real b0r = BA(LY::PBP + 2)[j], b0i = BA(LY::PBP + 3)[j];
+                        const Real b2r = BA(LY::PBP + 4)[j], b2i = BA(LY::PBP + 5)[j];
+                        r0 += cr * b1r + ci * b1i; i0 += cr * b1i - ci * b1r;
+                        r1 += cr * b0r - ci * b0i + cr * b2r + ci * b2i;
+                        i1 += cr * b0i + ci * b0r + cr * b2i - ci * b2r;
+                        r2 += cr * b1r - ci * b1i; i2 += cr * b1i + ci * b1r;
+                        BA(LY::PRB + 2)[j] = r0; BA(LY::PRB + 3)[j] = i0;
+                        BA(LY::PRB + 4)[j] = r2; BA(LY::PRB + 5)[j] = i2;
+                    }
+                    BA(LY::PRB + 0)[j] = r1; BA(LY::PRB + 1)[j] = i1;
                 }
                 __syncthreads();
-                for (int n = tid; n < M; n += kBS) {
-                    const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
-                    for (int g = 0; g < g1; ++g) {
-                        Real acc = Real(0);
-                        const Real* vb = s_vbuf + g * 2 * L;
-                        for (int e = e0; e < e1; ++e) acc += vb[e];
-                        WS(W_SR1 + c0 + g)[n] = acc;
+                // lanes: 0 B_r1 (L1D), 1 B_i1 (L1F, flux wall J_z(R)); m1: 2 B_r0, 3 B_i0 (L0), 4 B_r2, 5 B_i2 (L2D)
+                if (w == 0) {
+                    const int nj = M1 ? 6 : 2;
+                    if (lane < nj) {
+                        const int fidx = lane < 2 ? lane : (lane < 4 ? 2 : 3);   // factor set: L1D, L1F, L0, L2D
+                        const Real* a = BA(fidx == 0 ? LY::PA_L1D : fidx == 1 ? LY::PA_L1F : fidx == 2 ? LY::PA_L0 : LY::PA_L2D);
+                        const bool flux = fidx == 1;
+                        thomas_lds(a, BA(LY::PF + 2 * fidx), BA(LY::PF + 2 * fidx + 1), s_bw0[fidx], BA(LY::PRB + lane),
+                                   BA(LY::PBP + lane), Real(1), fidx != 2, !flux, flux, flux ? fcw : Real(0), M);
                     }
                 }
             }
         }
         __syncthreads();
-
-        // ================================================================ 5e. gz (solve_bplus, Simulation.cpp:542-558)
-        {
-            const Real* __restrict__ gm = A.grad_m;
-            const Real* __restrict__ g0 = A.grad_0;
-            const Real* __restrict__ gp = A.grad_p;
-            const Real* __restrict__ ir = A.inv_r;
-            for (int j = tid; j < M; j += kBS) {
-                const Real* f = WS(W_JZ0);
-                Real t1;
-                if (j == 0) t1 = Real(0);
-                else if (j == N) t1 = g0[N] * f[N] + gm[N] * f[N - 1] + gp[N] * f[N - 2];
-                else t1 = gm[j] * f[j - 1] + g0[j] * f[j] + gp[j] * f[j + 1];
-                WS(W_GR1)[j] = t1 + WS(W_SR1)[j];
-                WS(W_GI1)[j] = WS(W_SI1)[j];
-                if (M1) {
-                    const Real* fr = WS(W_JZ1R);
-                    const Real* fi = WS(W_JZ1I);
-                    Real t2, t4;
-                    if (j == 0) { t2 = fr[1] * ir[0]; t4 = fi[1] * ir[0]; }
-                    else if (j == N) {
-                        t2 = g0[N] * fr[N] + gm[N] * fr[N - 1] + gp[N] * fr[N - 2];
-                        t4 = g0[N] * fi[N] + gm[N] * fi[N - 1] + gp[N] * fi[N - 2];
-                    } else {
-                        t2 = gm[j] * fr[j - 1] + g0[j] * fr[j] + gp[j] * fr[j + 1];
-                        t4 = gm[j] * fi[j - 1] + g0[j] * fi[j] + gp[j] * fi[j + 1];
-                    }
-                    const Real t3 = (j == 0 ? fr[1] : fr[j]) * ir[j];
-                    const Real rh = (j == 0 ? fi[1] : fi[j]) * ir[j];
-                    WS(W_GR2)[j] = (t2 - t3) + WS(W_SR2)[j];
-                    WS(W_GI2)[j] = (t4 - rh) + WS(W_SI2)[j];
-                    WS(W_GR0)[j] = (t2 + t3) + WS(W_SR0)[j];
-                    WS(W_GI0)[j] = -(t4 + rh) + WS(W_SI0)[j];
-                }
-            }
-        }
-        const Real* bw0s = reinterpret_cast<const Real*>(s_misc + 8);
-        const int niter = M1 ? A.picard : 1;
-        for (int it = 0; it < niter; ++it) {
-            __syncthreads();
-            // ---- Picard right-hand sides (Simulation.cpp:563-581)
-            for (int j = tid; j < M; j += kBS) {
-                Real r1 = -WS(W_GI1)[j], i1 = WS(W_GR1)[j];
-                if (M1) {
-                    Real r0 = -WS(W_GI0)[j], i0 = WS(W_GR0)[j], r2 = -WS(W_GI2)[j], i2 = WS(W_GR2)[j];
-                    const Real cr = WS(W_CHI1R)[j], ci = WS(W_CHI1I)[j];
-                    const Real b0r = WS(W_BPR0)[j], b0i = WS(W_BPI0)[j], b1r = WS(W_BPR1)[j], b1i = WS(W_BPI1)[j];
-                    const Real b2r = WS(W_BPR2)[j], b2i = WS(W_BPI2)[j];
-                    r0 += cr * b1r + ci * b1i; i0 += cr * b1i - ci * b1r;
-                    r1 += cr * b0r - ci * b0i + cr * b2r + ci * b2i;
-                    i1 += cr * b0i + ci * b0r + cr * b2i - ci * b2r;
-                    r2 += cr * b1r - ci * b1i; i2 += cr * b1i + ci * b1r;
-                    WS(W_RBR0)[j] = r0; WS(W_RBI0)[j] = i0; WS(W_RBR2)[j] = r2; WS(W_RBI2)[j] = i2;
-                }
-                WS(W_RBR1)[j] = r1; WS(W_RBI1)[j] = i1;
-            }
-            __syncthreads();
-            // ---- solves (Simulation.cpp:582-589): 0 B_r0 L1D, 1 B_theta0 L1F (flux wall J_z(R)),
-            //      m1: 2 B0r L0, 3 B0i L0, 4 B2r L2D, 5 B2i L2D; all with chi0
-            if (w == 0) {
-                const int nj = M1 ? 6 : 2;
-                if (lane < nj) {
-                    int op, fm, rhs, out;
-                    switch (lane) {
-                        case 0: op = OP_L1D; fm = W_FM_L1D; rhs = W_RBR1; out = W_BPR1; break;
-                        case 1: op = OP_L1F; fm = W_FM_L1F; rhs = W_RBI1; out = W_BPI1; break;
-                        case 2: op = OP_L0; fm = W_FM_L0; rhs = W_RBR0; out = W_BPR0; break;
-                        case 3: op = OP_L0; fm = W_FM_L0; rhs = W_RBI0; out = W_BPI0; break;
-                        case 4: op = OP_L2D; fm = W_FM_L2D; rhs = W_RBR2; out = W_BPR2; break;
-                        default: op = OP_L2D; fm = W_FM_L2D; rhs = W_RBI2; out = W_BPI2; break;
-                    }
-                    const int fidx = op == OP_L1D ? 0 : op == OP_L1F ? 1 : op == OP_L0 ? 2 : 3;
-                    const bool flux = op == OP_L1F;
-                    const Real fcw = flux ? A.flux_coef * WS(W_JZ0)[N] : Real(0);
-                    thomas_apply(A.op_a[op], WS(fm), WS(fm + 1), bw0s[fidx], WS(rhs), Real(1), op != OP_L0, !flux, flux,
-                                 fcw, WS(W_DP0 + lane), WS(out), M);
-                }
-            }
-        }
-        __syncthreads();
+        QZ_PHASE(8);
 
         // ================================================================ 6. store (Simulation.cpp:594-633) + push
+        for (int j = tid; j < M; j += kBS)
+            for (int f = 0; f < (M1 ? 6 : 2); ++f) WS(W_BPR1 + f)[j] = BA(LY::PBP + f)[j];   // warm start of the next slice
+        __syncthreads();
+        for (int j = tid; j < M; j += kBS) {
+            BA(LY::GW + 0)[j] = WS(W_WPR1)[j];
+            BA(LY::GW + 1)[j] = WS(W_WPI1)[j];
+            BA(LY::GEZ)[j] = WS(W_EZ0)[j];
+            BA(LY::GBZ)[j] = WS(W_BZ0)[j];
+            BA(LY::GBP + 0)[j] = WS(W_BPR1)[j];
+            BA(LY::GBP + 1)[j] = WS(W_BPI1)[j];
+            if (M1) {
+                BA(LY::GW + 2)[j] = WS(W_WPR0)[j]; BA(LY::GW + 3)[j] = WS(W_WPI0)[j];
+                BA(LY::GW + 4)[j] = WS(W_WPR2)[j]; BA(LY::GW + 5)[j] = WS(W_WPI2)[j];
+                BA(LY::GEZ + 1)[j] = WS(W_EZ1R)[j]; BA(LY::GEZ + 2)[j] = WS(W_EZ1I)[j];
+                BA(LY::GBZ + 1)[j] = WS(W_BZ1R)[j]; BA(LY::GBZ + 2)[j] = WS(W_BZ1I)[j];
+                BA(LY::GBP + 2)[j] = WS(W_BPR0)[j]; BA(LY::GBP + 3)[j] = WS(W_BPI0)[j];
+                BA(LY::GBP + 4)[j] = WS(W_BPR2)[j]; BA(LY::GBP + 5)[j] = WS(W_BPI2)[j];
+            }
+        }
+        __syncthreads();
         {
             Real* __restrict__ F = A.fld + static_cast<size_t>(k) * M * A.ncf;
-            Real* __restrict__ Dg = A.diag + static_cast<size_t>(k) * M * A.ncd;
-            const int ncf = A.ncf, ncd = A.ncd;
+            const int ncf = A.ncf;
             for (int j = tid; j < M; j += kBS) {
                 Real* Fj = F + static_cast<size_t>(j) * ncf;
-                Real* Dj = Dg + static_cast<size_t>(j) * ncd;
-                Fj[F_EP1R] = WS(W_WPR1)[j] + WS(W_BPI1)[j];
-                Fj[F_EP1I] = WS(W_WPI1)[j] - WS(W_BPR1)[j];
-                Fj[F_BP1R] = WS(W_BPR1)[j];
-                Fj[F_BP1I] = WS(W_BPI1)[j];
-                Fj[F_EZ0] = WS(W_EZ0)[j];
-                Fj[F_BZ0] = WS(W_BZ0)[j];
-                Dj[D_PSI0] = WS(W_PSI0)[j];
-                Dj[D_NE0] = WS(W_NE0)[j];
-                Dj[D_NI0] = WS(W_NI0)[j];
+                const Real b1r = BA(LY::GBP + 0)[j], b1i = BA(LY::GBP + 1)[j];
+                Fj[F_EP1R] = BA(LY::GW + 0)[j] + b1i;
+                Fj[F_EP1I] = BA(LY::GW + 1)[j] - b1r;
+                Fj[F_BP1R] = b1r;
+                Fj[F_BP1I] = b1i;
                 if (M1) {
-                    Fj[F_EP0R] = WS(W_WPR0)[j] + WS(W_BPI0)[j];
-                    Fj[F_EP0I] = WS(W_WPI0)[j] - WS(W_BPR0)[j];
-                    Fj[F_EP2R] = WS(W_WPR2)[j] + WS(W_BPI2)[j];
-                    Fj[F_EP2I] = WS(W_WPI2)[j] - WS(W_BPR2)[j];
-                    Fj[F_BP0R] = WS(W_BPR0)[j];
-                    Fj[F_BP0I] = WS(W_BPI0)[j];
-                    Fj[F_BP2R] = WS(W_BPR2)[j];
-                    Fj[F_BP2I] = WS(W_BPI2)[j];
-                    Fj[F_EZ1R] = WS(W_EZ1R)[j];
-                    Fj[F_EZ1I] = WS(W_EZ1I)[j];
-                    Fj[F_BZ1R] = WS(W_BZ1R)[j];
-                    Fj[F_BZ1I] = WS(W_BZ1I)[j];
-                    Dj[D_PSI1R] = WS(W_PSI1R)[j];
-                    Dj[D_PSI1I] = WS(W_PSI1I)[j];
-                    Dj[D_NE1R] = WS(W_NE1R)[j];
-                    Dj[D_NE1I] = WS(W_NE1I)[j];
-                    Dj[D_NI1R] = WS(W_NI1R)[j];
-                    Dj[D_NI1I] = WS(W_NI1I)[j];
+                    const Real b0r = BA(LY::GBP + 2)[j], b0i = BA(LY::GBP + 3)[j];
+                    const Real b2r = BA(LY::GBP + 4)[j], b2i = BA(LY::GBP + 5)[j];
+                    Fj[F_EP0R] = BA(LY::GW + 2)[j] + b0i;
+                    Fj[F_EP0I] = BA(LY::GW + 3)[j] - b0r;
+                    Fj[F_EP2R] = BA(LY::GW + 4)[j] + b2i;
+                    Fj[F_EP2I] = BA(LY::GW + 5)[j] - b2r;
+                    Fj[F_BP0R] = b0r; Fj[F_BP0I] = b0i; Fj[F_BP2R] = b2r; Fj[F_BP2I] = b2i;
                 }
             }
         }
@@ -774,57 +978,78 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             // ---- Adams-Bashforth push (PlasmaSpecies::push, src/PlasmaSpecies.cpp:600-667)
             const int slot = k % ord;
             const bool first = (k == 0);
+            fence_cache();
 #pragma unroll
-            for (int p = 0; p < PT; ++p) {
-                const int j = P.cj[p];
+            for (int pp = 0; pp < PT; ++pp) {
+                const int j = cj[pp];
                 if (j < 0) continue;
-                const SpecInfo& S = sp[P.cspec[p]];
-                const qz_species_state* st = S.st;
-                const int li = P.cli[p];
-                const int cap = st->cap;
-                Real* __restrict__ hist = st->hist;
-                const Real X = st->x[li], Y = st->y[li];
-                const Real PX = P.cPX[p], PY = P.cPY[p], D = P.cD[p];
-                const Real qm = S.qm;
-                const Real t = P.ct[p], c = P.cc[p], sn = P.cs[p];
+                const int gi = slot_of(pp);
+                const bool s1 = gi >= np0;
+                const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
+                const int li = s1 ? gi - np0 : gi;
+                const int cap = st.cap;
+                Real* __restrict__ hist = st.hist;
+                const Real X = st.x[li], Y = st.y[li], D = st.dl[li];
+                const Real PX = st.px[li], PY = st.py[li];
+                const Real qm = s1 ? qm1 : qm0;
+                const Real t = ct[pp], c = cc[pp], sn = cs[pp];
                 Real Wx, Wy, Bx, By;
-                eval_vector(lin(WS(W_WPR0), j, t), lin(WS(W_WPI0), j, t), lin(WS(W_WPR1), j, t), lin(WS(W_WPI1), j, t),
-                            lin(WS(W_WPR2), j, t), lin(WS(W_WPI2), j, t), c, sn, M1, Wx, Wy);
-                const Real Ez = eval_scalar(lin(WS(W_EZ0), j, t), M1 ? lin(WS(W_EZ1R), j, t) : Real(0),
-                                            M1 ? lin(WS(W_EZ1I), j, t) : Real(0), c, sn, M1);
-                const Real Bz = eval_scalar(lin(WS(W_BZ0), j, t), M1 ? lin(WS(W_BZ1R), j, t) : Real(0),
-                                            M1 ? lin(WS(W_BZ1I), j, t) : Real(0), c, sn, M1);
-                eval_vector(lin(WS(W_BPR0), j, t), lin(WS(W_BPI0), j, t), lin(WS(W_BPR1), j, t), lin(WS(W_BPI1), j, t),
-                            lin(WS(W_BPR2), j, t), lin(WS(W_BPI2), j, t), c, sn, M1, Bx, By);
-                const Real dA = Real(0);
-                const Real gam = P.cgam[p];
-                const Real invd = P.cinvd[p];
-                const Real fp = -Real(0.5) * qm * qm * dA * invd;
-                Real fv[5];
-                fv[0] = PX * invd;
-                fv[1] = PY * invd;
-                fv[2] = qm * (gam * Wx * invd + By + PY * Bz * invd) + fp * c;
-                fv[3] = qm * (gam * Wy * invd - Bx - PX * Bz * invd) + fp * sn;
-                fv[4] = qm * ((PX * Wx + PY * Wy) * invd - Ez);
-                auto H = [&](int sl, int v) -> Real& { return hist[(static_cast<size_t>(sl) * 5 + v) * cap + li]; };
-                if (first) {
-                    for (int sl = 0; sl < ord; ++sl)
-                        for (int v = 0; v < 5; ++v) H(sl, v) = fv[v];
+                if (M1) {
+                    eval_vector(lin(BA(LY::GW + 2), j, t), lin(BA(LY::GW + 3), j, t), lin(BA(LY::GW + 0), j, t),
+                                lin(BA(LY::GW + 1), j, t), lin(BA(LY::GW + 4), j, t), lin(BA(LY::GW + 5), j, t), c, sn, true,
+                                Wx, Wy);
+                    eval_vector(lin(BA(LY::GBP + 2), j, t), lin(BA(LY::GBP + 3), j, t), lin(BA(LY::GBP + 0), j, t),
+                                lin(BA(LY::GBP + 1), j, t), lin(BA(LY::GBP + 4), j, t), lin(BA(LY::GBP + 5), j, t), c, sn,
+                                true, Bx, By);
                 } else {
-                    for (int v = 0; v < 5; ++v) H(slot, v) = fv[v];
+                    eval_vector(Real(0), Real(0), lin(BA(LY::GW + 0), j, t), lin(BA(LY::GW + 1), j, t), Real(0), Real(0),
+                                c, sn, false, Wx, Wy);
+                    eval_vector(Real(0), Real(0), lin(BA(LY::GBP + 0), j, t), lin(BA(LY::GBP + 1), j, t), Real(0),
+                                Real(0), c, sn, false, Bx, By);
                 }
-                Real sum[5] = {0, 0, 0, 0, 0};
-                for (int mm = 0; mm < ord; ++mm) {
-                    const int sl = (slot - mm + ord) % ord;
-                    for (int v = 0; v < 5; ++v) sum[v] += A.ab_c[mm] * (first ? fv[v] : (sl == slot ? fv[v] : H(sl, v)));
+                const Real Ez = eval_scalar(lin(BA(LY::GEZ), j, t), M1 ? lin(BA(LY::GEZ + 1), j, t) : Real(0),
+                                            M1 ? lin(BA(LY::GEZ + 2), j, t) : Real(0), c, sn, M1);
+                const Real Bz = eval_scalar(lin(BA(LY::GBZ), j, t), M1 ? lin(BA(LY::GBZ + 1), j, t) : Real(0),
+                                            M1 ? lin(BA(LY::GBZ + 2), j, t) : Real(0), c, sn, M1);
+                const Real dA = Real(0);
+                const Real gam = cgam[pp];
+                const Real invd = cinvd[pp];
+                const Real fp = -Real(0.5) * qm * qm * dA * invd;
+                const Real f0 = PX * invd;
+                const Real f1 = PY * invd;
+                const Real f2 = qm * (gam * Wx * invd + By + PY * Bz * invd) + fp * c;
+                const Real f3 = qm * (gam * Wy * invd - Bx - PX * Bz * invd) + fp * sn;
+                const Real f4 = qm * ((PX * Wx + PY * Wy) * invd - Ez);
+                auto H = [&](int sl, int v) -> Real& { return hist[(static_cast<size_t>(sl) * 5 + v) * cap + li]; };
+                Real s0 = 0, sm1 = 0, sm2 = 0, sm3 = 0, sm4 = 0;
+                if (first) {
+                    for (int sl = 0; sl < ord; ++sl) {
+                        H(sl, 0) = f0; H(sl, 1) = f1; H(sl, 2) = f2; H(sl, 3) = f3; H(sl, 4) = f4;
+                    }
+                    for (int mm = 0; mm < ord; ++mm) {
+                        const Real cm = A.ab_c[mm];
+                        s0 += cm * f0; sm1 += cm * f1; sm2 += cm * f2; sm3 += cm * f3; sm4 += cm * f4;
+                    }
+                } else {
+                    H(slot, 0) = f0; H(slot, 1) = f1; H(slot, 2) = f2; H(slot, 3) = f3; H(slot, 4) = f4;
+                    for (int mm = 0; mm < ord; ++mm) {
+                        const int sl = (slot - mm + ord) % ord;
+                        const Real cm = A.ab_c[mm];
+                        if (mm == 0) {
+                            s0 += cm * f0; sm1 += cm * f1; sm2 += cm * f2; sm3 += cm * f3; sm4 += cm * f4;
+                        } else {
+                            s0 += cm * H(sl, 0); sm1 += cm * H(sl, 1); sm2 += cm * H(sl, 2); sm3 += cm * H(sl, 3);
+                            sm4 += cm * H(sl, 4);
+                        }
+                    }
                 }
-                Real xn = X + dxi * sum[0], yn = Y + dxi * sum[1];
-                Real pxn = PX + dxi * sum[2], pyn = PY + dxi * sum[3];
-                const Real dn = D + dxi * sum[4];
+                Real xn = X + dxi * s0, yn = Y + dxi * sm1;
+                Real pxn = PX + dxi * sm2, pyn = PY + dxi * sm3;
+                const Real dn = D + dxi * sm4;
                 const Real gn = (Real(1) + pxn * pxn + pyn * pyn + dn * dn) / (Real(2) * dn);
                 if (!(dn > dmin) || gn > qsamax * dn) {
-                    st->w[li] = Real(0);
-                    atomicAdd(&A.lost[P.cspec[p]], 1);
+                    st.w[li] = Real(0);
+                    atomicAdd(&A.lost[s1 ? 1 : 0], 1);
                     continue;
                 }
                 const Real rn = dsqrt(xn * xn + yn * yn);
@@ -832,8 +1057,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     const Real nx = xn / rn, ny = yn / rn;
                     const Real rr = Real(2) * R - rn;
                     if (rr < Real(0)) {
-                        st->w[li] = Real(0);
-                        atomicAdd(&A.lost[P.cspec[p]], 1);
+                        st.w[li] = Real(0);
+                        atomicAdd(&A.lost[s1 ? 1 : 0], 1);
                         continue;
                     }
                     xn = nx * rr; yn = ny * rr;
@@ -846,11 +1071,27 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                         H(sl, 2) -= Real(2) * bb * nx; H(sl, 3) -= Real(2) * bb * ny;
                     }
                 }
-                st->x[li] = xn; st->y[li] = yn; st->px[li] = pxn; st->py[li] = pyn; st->dl[li] = dn;
+                st.x[li] = xn; st.y[li] = yn; st.px[li] = pxn; st.py[li] = pyn; st.dl[li] = dn;
             }
         }
         __syncthreads();
+        QZ_PHASE(9);
     }
+    if (tid == 0 && stats)
+        for (int i = 0; i < 10; ++i) atomicAdd(&stats[1 + i], s_ph[i]);
+#undef QZ_PHASE
+}
+
+// LDS bytes the kernel needs for N cells, PT particles per thread and L live particles
+long sweep_lds_needed(bool m1, int N, int pt, int L) {
+    const LdsPlan plan(N, pt);
+    const long Mp = plan.Mp;
+    const long nmax = m1 ? Lay<true>::NMAX : Lay<false>::NMAX;
+    const long nsdep = m1 ? Lay<true>::NSDEP : Lay<false>::NSDEP;
+    long need = nmax * Mp;
+    need = std::max(need, nsdep * Mp + 2L * L);   // staged fields + one S+ contribution list
+    need = std::max(need, 2L * L);                // one deposit contribution list
+    return plan.region + need * 8;
 }
 
 }  // namespace qz
@@ -862,7 +1103,7 @@ size_t qz_sweep_workspace_doubles_impl(int M) { return static_cast<size_t>(W_NAR
 
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream) {
     int maxnp = 0, maxN = 0;
-    bool m1 = host_runs[0].modes == 1;
+    const bool m1 = host_runs[0].modes == 1;
     for (int i = 0; i < nruns; ++i) {
         int np = 0;
         for (int s = 0; s < host_runs[i].nspecies; ++s) np += host_runs[i].sp[s].np;
@@ -870,27 +1111,21 @@ int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_run
         maxN = std::max(maxN, host_runs[i].grid.N);
         if ((host_runs[i].modes == 1) != m1) return qz_fail(QZ_EINVAL, "qz_sweep_batched: all runs need the same modes");
     }
-    if (maxnp > 65535) return qz_fail(QZ_ERANGE, "sweep: more than 65535 plasma particles per slice");
-    const int shmem = kLdsBytes;
-    {   // the LDS carve-up of the kernel must leave room for one contribution list of 2 L doubles
-        const int pt_cap = ((maxnp + kBS - 1) / kBS <= 2) ? 2 : ((maxnp + kBS - 1) / kBS <= 4 ? 4 : 8);
-        int off = ((kNW * maxN * 2) + 15) & ~15;
-        off += (((maxN + 3) * 4) + 15) & ~15;
-        off += ((kBS * pt_cap * 2) + 15) & ~15;
-        off += 64;
-        if (off + 16 * kBS * pt_cap > kLdsBytes) return qz_fail(QZ_ERANGE, "sweep: LDS too small for this grid");
-    }
+    if (maxnp > kBS * 8)
+        return qz_fail(QZ_ERANGE, "sweep: more than 4096 plasma particles per slice (one workgroup per run)");
+    const int pt = maxnp <= 2 * kBS ? 2 : (maxnp <= 4 * kBS ? 4 : 8);
+    if (sweep_lds_needed(m1, maxN, pt, maxnp) > kLdsBytes)
+        return qz_fail(QZ_ERANGE, "sweep: the grid and the particle count do not fit the LDS of one workgroup");
     auto launch = [&](auto kern) -> int {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize, shmem);
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return qz_fail_hip(e, "hipFuncSetAttribute");
-        hipLaunchKernelGGL(kern, dim3(nruns), dim3(kBS), shmem, stream, dev_runs);
+        hipLaunchKernelGGL(kern, dim3(nruns), dim3(kBS), kLdsBytes, stream, dev_runs);
         e = hipGetLastError();
         if (e != hipSuccess) return qz_fail_hip(e, "sweep launch");
         return QZ_OK;
     };
-    const int pt = (maxnp + kBS - 1) / kBS;
-    if (pt <= 2) return m1 ? launch(sweep_kernel<true, 2>) : launch(sweep_kernel<false, 2>);
-    if (pt <= 4) return m1 ? launch(sweep_kernel<true, 4>) : launch(sweep_kernel<false, 4>);
-    if (pt <= 8) return m1 ? launch(sweep_kernel<true, 8>) : launch(sweep_kernel<false, 8>);
-    return qz_fail(QZ_ERANGE, "sweep: more than 4096 plasma particles per slice (one workgroup per run)");
+    if (pt == 2) return m1 ? launch(sweep_kernel<true, 2>) : launch(sweep_kernel<false, 2>);
+    if (pt == 4) return m1 ? launch(sweep_kernel<true, 4>) : launch(sweep_kernel<false, 4>);
+    return m1 ? launch(sweep_kernel<true, 8>) : launch(sweep_kernel<false, 8>);
 }
