@@ -43,6 +43,79 @@ def cpu_baseline(deck, nthreads):
                       f"{wall:.2f} s, OpenMP atomics, g++ -O2 -ffp-contract=off)"}
 
 
+def bench_batched(args):
+    """c5: every step of all runs in one batched launch, against the same runs stepped one launch at a time."""
+    import torch
+    import quarz_amd
+    from quarz_amd import configs
+    runs = configs.c5_runs(args.runs, steps=args.warmup + args.steps + args.plain_steps + 1)
+    sims = []
+    t0 = time.perf_counter()
+    for i, (deck, _) in enumerate(runs):
+        sims.append(quarz_amd.Simulation(deck))
+        if (i + 1) % 32 == 0:
+            print(f"# set up {i + 1}/{len(runs)} runs ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        quarz_amd.step_batched(sims)
+    torch.cuda.synchronize()
+    psl = pushes = 0.0
+    t0 = time.perf_counter()
+    kern_ms = 0.0
+    for _ in range(args.steps):
+        kern_ms += quarz_amd.step_batched(sims)
+        for s in sims:
+            st = s.stats()
+            psl += st["particle_slices"]
+            pushes += st["beam_pushed"]
+        print(f"# batched step done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the plain path: the same runs, one launch per run, one after the other
+    ppsl = 0.0
+    t1 = time.perf_counter()
+    for _ in range(args.plain_steps):
+        for i, s in enumerate(sims):
+            s.step()
+            ppsl += s.stats()["particle_slices"]
+            if (i + 1) % 64 == 0:
+                print(f"# plain: {i + 1}/{len(sims)} runs stepped ({time.perf_counter() - t1:.0f} s)", file=sys.stderr,
+                      flush=True)
+    torch.cuda.synchronize()
+    pwall = time.perf_counter() - t1
+    fp, fn = FLOPS[0]
+    node_slices = sum(s.M * s.K for s in sims)
+    flops = fp * psl / args.steps + fn * node_slices
+    achieved = flops / (kern_ms / args.steps / 1e3)
+    Ns = sorted(s.M - 1 for s in sims)
+    out = {
+        "metric": METRIC,
+        "value": psl / wall,
+        "unit": "particle-slices/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: beams sampled from per-run seeds, plasma loaded at the box head",
+        "config": {"workload": f"c5: {configs.DESCRIPTION['c5']}", "runs": len(sims),
+                   "cells_min_median_max": [Ns[0], Ns[len(Ns) // 2], Ns[-1]],
+                   "slices_total": sum(s.K for s in sims), "parallelism": "one workgroup per run, one launch"},
+        "beam_pushes_per_s": pushes / wall,
+        "batched_sweep_ms": kern_ms / args.steps,
+        "plain_value": ppsl / pwall,
+        "plain_steps": args.plain_steps,
+        "batched_vs_plain": (psl / wall) / (ppsl / pwall),
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel (batched)",
+                     "flops_per_launch": flops,
+                     "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,11 +123,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--runs", type=int, default=256, help="c5: number of independent runs")
+    ap.add_argument("--plain-steps", type=int, default=1, help="c5: steps of the one-launch-per-run comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
               f"see DESIGN.md for how the path shards)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c5":
+        bench_batched(args)
+        return
     import torch
     import quarz_amd
     from quarz_amd import configs
@@ -103,9 +181,10 @@ def main():
         "beam_deposit_ms": dep_ms / args.steps,
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
-        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel",
-                     "flops_per_launch": flops_per_sweep},
+                     "flops_per_launch": flops_per_sweep,
+                     "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
     }
     if not args.no_cpu:
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)

@@ -121,8 +121,17 @@ int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
                     int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt, int move,
                     void* stream);
 
-/* The 17 weighted beam sums relative to gref and xiref (Beam::sums, src/Beam.cpp:1133-1155). */
+/* The 17 weighted beam sums relative to gref and xiref (Beam::sums, src/Beam.cpp:1133-1155): the particles
+ * are copied to the host and summed in index order, as the reference does (bit-identical). */
 int qz_beam_sums(int np, const qz_beam_soa* p, double gref, double xiref, double* out17_host, void* stream);
+
+/* The same 17 sums reduced on the device into out17 (device memory), without copying the particles:
+ * a fixed-shape two-level reduction, reproducible run to run, equal to qz_beam_sums up to the rounding
+ * of the summation order.  work: qz_beam_sums_workspace_bytes(np) bytes of device memory.  Replaces the
+ * host copy of Beam::sums for large beams (Beam::sums, src/Beam.cpp:1133-1155). */
+size_t qz_beam_sums_workspace_bytes(int np);
+int qz_beam_sums_device(int np, const qz_beam_soa* p, double gref, double xiref, double* out17, void* work,
+                        void* stream);
 
 /* One quasi-static sweep over all slices of one run: per slice the mode deposit, the psi / E_z / B_z and
  * B+ solves, the field store and the Adams-Bashforth push, in one persistent kernel
@@ -143,6 +152,10 @@ int qz_sim_info(const qz_sim* s, int* info10);
 /* one step of Simulation::run (src/Simulation.cpp:1376-1459): sweep, then the beam push unless last */
 int qz_sim_step(qz_sim* s);
 int qz_sim_step_fields(qz_sim* s);   /* the sweep only */
+/* One step of n independent runs (same modes) whose sweeps share one persistent launch, one workgroup per
+ * run (qz_sweep_batched); beam deposits and pushes run per run.  batch_ms (optional) receives the time of
+ * the batched sweep launch.  Not in the reference: it batches what Simulation::run does for one run. */
+int qz_sim_step_batched(qz_sim** sims, int n, double* batch_ms);
 int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
 /* what: fld diag bsrc nodes, beam<b>.<x|y|px|py|pz|xi|w>, sp<s>.<x|y|px|py|dl|w>; returns the count */
 long qz_sim_get(qz_sim* s, const char* what, double* host_out, long cap);

@@ -54,6 +54,10 @@ def lib():
                                       C.c_int, C.c_int, C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_int, C.c_void_p]
         L.qz_beam_sums.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, DP, C.c_void_p]
+        L.qz_beam_sums_workspace_bytes.restype = C.c_size_t
+        L.qz_beam_sums_workspace_bytes.argtypes = [C.c_int]
+        L.qz_beam_sums_device.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]
         L.qz_sweep_workspace_doubles.restype = C.c_size_t
         L.qz_sweep_workspace_doubles.argtypes = [C.c_int]
         L.qz_sim_create.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
@@ -61,6 +65,7 @@ def lib():
         L.qz_sim_info.argtypes = [C.c_void_p, IP]
         L.qz_sim_step.argtypes = [C.c_void_p]
         L.qz_sim_step_fields.argtypes = [C.c_void_p]
+        L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
         L.qz_sim_push_beams.argtypes = [C.c_void_p]
         L.qz_sim_get.restype = C.c_long
         L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
@@ -199,6 +204,15 @@ class Simulation:
         check(lib().qz_sim_write_fields(self.h, path.encode()))
 
 
+def step_batched(sims):
+    """One step of every Simulation in `sims` (same modes), their sweeps in one persistent launch.
+    Returns the batched sweep time in ms."""
+    arr = (C.c_void_p * len(sims))(*[s.h.value for s in sims])
+    ms = C.c_double(0)
+    check(lib().qz_sim_step_batched(arr, len(sims), C.byref(ms)))
+    return ms.value
+
+
 # ---------------------------------------------------------------------------- kernel-level entry points
 def _ptr(t):
     return C.c_void_p(t.data_ptr())
@@ -235,3 +249,21 @@ def beam_advance(grid, box, modes, qm, pusher, p, fld, dt_kick, dt, move, i0=0, 
     nc = fld.shape[-1]
     check(lib().qz_beam_advance(C.byref(grid), C.byref(box), modes, qm, pusher, i0, p["x"].numel(), C.byref(soa),
                                 _ptr(fld), nc, dt_kick, dt, int(move), stream))
+
+
+def beam_sums(p, gref, xiref, stream=None):
+    """Beam::sums of device SoA tensors, summed on the host in index order (bit-identical to the reference)."""
+    soa = beam_soa(p)
+    out = np.zeros(17)
+    check(lib().qz_beam_sums(p["x"].numel(), C.byref(soa), gref, xiref, out.ctypes.data_as(DP), stream))
+    return out
+
+
+def beam_sums_device(p, gref, xiref, out, work, stream=None):
+    """Beam::sums reduced on the device into `out` (17 doubles, device); work from beam_sums_workspace_bytes."""
+    soa = beam_soa(p)
+    check(lib().qz_beam_sums_device(p["x"].numel(), C.byref(soa), gref, xiref, _ptr(out), _ptr(work), stream))
+
+
+def beam_sums_workspace_bytes(np_):
+    return lib().qz_beam_sums_workspace_bytes(np_)

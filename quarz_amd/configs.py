@@ -107,10 +107,85 @@ witness.cut = 3
 witness.nparticles = 500000
 """
 
+# c5: many independent m = 0 runs for the batched launch.  Per run (seeded): N from {128, 256, 384, 512, 768,
+# 1024} stretched cells from 1e-3 at the axis to 0.05 at the wall (R = 8, or the largest R a growth ratio of
+# 1.05 reaches with N cells, 3.37 for N = 128), K uniform in [1000, 8000] slices at dxi = 0.01, driver n_b
+# uniform in [0.5, 4] (sigma_r 0.3, sigma_xi 1, xi0 -2, gamma 2000, 1% spread, emit 2e-3), witness delayed by
+# U[-6, -3] behind the driver (n_b 1, sigma_r 0.1, sigma_xi 0.3, gamma 2000, emit 1e-3), beam particles
+# U[5e4, 5e5] (2/3 driver, 1/3 witness), 4 plasma particles per cell, Vay, dt 20.  The witness shape and
+# the particle split are this workload's choices; the rest follows the batched-scan description.
+_C5 = """
+xi.convention = z-t
+modes = 0
+grid.type = file
+grid.file = {grid_file}
+xi.min = {xi_min}
+xi.max = 0
+xi.step = 0.01
+time.dt = 20
+time.steps = {steps}
+pusher.beam = vay
+pusher.ab_order = 3
+plasma.species = electrons
+electrons.ppc = 4
+beams = driver witness
+driver.n0 = {n0}
+driver.sigma_r = 0.3
+driver.sigma_xi = 1.0
+driver.xi0 = -2
+driver.gamma = 2000
+driver.espread = 0.01
+driver.emit_n = 0.002
+driver.cut = 3
+driver.nparticles = {nd}
+driver.seed = {seed_d}
+witness.n0 = 1
+witness.sigma_r = 0.1
+witness.sigma_xi = 0.3
+witness.xi0 = {xi_w}
+witness.gamma = 2000
+witness.emit_n = 0.001
+witness.cut = 3
+witness.nparticles = {nw}
+witness.seed = {seed_w}
+"""
+
+
+def c5_runs(nruns=256, seed=2024, steps=5, workdir=None):
+    """Decks of the batched scan: a list of (deck text, summary dict), one per run."""
+    import random
+    rng = random.Random(seed)
+    workdir = workdir or tempfile.mkdtemp(prefix="quarz_amd_c5_")
+    grids = {}
+    out = []
+    for i in range(nruns):
+        n = rng.choice([128, 256, 384, 512, 768, 1024])
+        if n not in grids:
+            r_reach = sum(min(1e-3 * 1.05 ** c, 0.05) for c in range(n))
+            R = min(8.0, r_reach)
+            nodes, q = stretched_nodes(n, R, 1e-3, 0.05)
+            path = os.path.join(workdir, f"grid_c5_{n}.txt")
+            with open(path, "w") as f:
+                f.write(f"# {n} cells, 1e-3 at the axis, 0.05 at the wall, R = {R:.6g}, q = {q:.6g}\n")
+                f.writelines(f"{x:.17g}\n" for x in nodes)
+            grids[n] = path
+        K = rng.randint(1000, 8000)
+        n0 = rng.uniform(0.5, 4.0)
+        delay = rng.uniform(-6.0, -3.0)
+        nb = rng.randint(50000, 500000)
+        nd = (2 * nb) // 3
+        deck = _C5.format(grid_file=grids[n], xi_min=f"{-(K - 1) * 0.01:.2f}", steps=steps, n0=f"{n0:.6f}", nd=nd,
+                          nw=nb - nd, xi_w=f"{-2.0 + delay:.6f}", seed_d=seed + 2 * i, seed_w=seed + 2 * i + 1)
+        out.append((deck, {"N": n, "K": K, "n0": n0, "delay": delay, "nb": nb}))
+    return out
+
+
 DESCRIPTION = {
     "c1": "m0 blowout, uniform 256 cells R=8, ppc4, 1601 slices, driver 1e5, vay, dt20",
     "c2": "m0+1 stretched 512 cells (5e-4..0.05) R=10, 8 azimuths, 4001 slices, driver 1e6 + witness 5e5, "
           "picard 3, vay, dt10",
+    "c5": "256 independent m0 runs, N in {128..1024} stretched (1e-3..0.05), K in [1000, 8000], "
+          "driver n_b in [0.5, 4] + witness, 5e4..5e5 beam particles, ppc 4, one batched launch per step",
 }
 
 

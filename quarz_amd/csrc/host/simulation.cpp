@@ -630,6 +630,19 @@ Simulation::~Simulation() {
 
 // src/Simulation.cpp:636-795 (one rank, no sub-slicing)
 void Simulation::step_fields() {
+    prepare_sweep();
+    hip_check(hipEventRecord(ev[0], stream), "event");
+    qz_check(qz_sweep(&sweep_args, stream));
+    hip_check(hipEventRecord(ev[1], stream), "event");
+    hip_check(hipEventSynchronize(ev[1]), "sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
+    finish_sweep(ms);
+}
+
+// everything of step_fields before the slice loop (src/Simulation.cpp:636-726): beam sources, plasma load,
+// background; fills sweep_args
+void Simulation::prepare_sweep() {
     // ---- beams -> (xi, r) arrays
     hip_check(hipEventRecord(ev[4], stream), "event");
     hip_check(hipMemsetAsync(bsrc.p, 0, sizeof(Real) * bsrc.n, stream), "bsrc reset");
@@ -694,14 +707,17 @@ void Simulation::step_fields() {
     A.lost = lost.p;
     A.ws = ws.p;
     A.stats = counters.p;
-    hip_check(hipEventRecord(ev[0], stream), "event");
-    qz_check(qz_sweep(&A, stream));
-    hip_check(hipEventRecord(ev[1], stream), "event");
+    sweep_args = A;
+}
+
+// after the slice loop: removed-particle count, work counters, timings (ms_sweep measured by the caller)
+void Simulation::finish_sweep(float sweep_ms) {
+    const int nsp = sweep_args.nspecies;
     std::vector<int> hl;
     lost.download(hl, stream);
     lost_last = 0;
     for (int i = 0; i < nsp; ++i) lost_last += hl[i];
-    hip_check(hipEventElapsedTime(&ms_sweep, ev[0], ev[1]), "elapsed");
+    ms_sweep = sweep_ms;
     hip_check(hipEventElapsedTime(&ms_beam_dep, ev[4], ev[5]), "elapsed");
     std::vector<unsigned long long> cnt;
     counters.download(cnt, stream);
@@ -749,6 +765,11 @@ void Simulation::push_beams() {
 void Simulation::step() {
     if (done) throw std::runtime_error("the run has ended (time.steps reached)");
     step_fields();
+    after_sweep();
+}
+
+// the rest of one iteration of Simulation::run after the sweep: output, beam push, step counter
+void Simulation::after_sweep() {
     if (!outdir.empty() && out_every > 0 && step_n % out_every == 0) {
         char nm[32];
         std::snprintf(nm, sizeof(nm), "/fields_%06d.bin", step_n);
@@ -873,6 +894,49 @@ extern "C" int qz_sim_info(const qz_sim* h, int* info) {
     return QZ_OK;
 }
 extern "C" int qz_sim_step(qz_sim* h) { QZ_SIM_TRY(h->s->step()) }
+
+// One step of n independent runs with their sweeps in one persistent launch (one workgroup per run).
+// Every run's preparation is enqueued on its own stream; the launch waits for all of them on the first
+// run's stream; every run then finishes and pushes its beams on its own stream.
+extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
+    if (!hs || n <= 0) return qz_fail(QZ_EINVAL, "qz_sim_step_batched: no runs");
+    QZ_SIM_TRY({
+        for (int i = 0; i < n; ++i) {
+            if (!hs[i]) throw std::runtime_error("qz_sim_step_batched: null run");
+            if (hs[i]->s->done) throw std::runtime_error("qz_sim_step_batched: a run has ended (time.steps reached)");
+            if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_sim_step_batched: runs differ in modes");
+        }
+        std::vector<qz_sweep_args> args(n);
+        qz::Simulation& s0 = *hs[0]->s;
+        for (int i = 0; i < n; ++i) {
+            qz::Simulation& s = *hs[i]->s;
+            s.prepare_sweep();
+            args[i] = s.sweep_args;
+            qz::hip_check(hipEventRecord(s.ev[0], s.stream), "event");
+            if (i > 0) qz::hip_check(hipStreamWaitEvent(s0.stream, s.ev[0], 0), "stream wait");
+        }
+        struct Events {   // released on every exit path
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+        } ev;
+        qz::hip_check(hipEventCreate(&ev.e[0]), "event");
+        qz::hip_check(hipEventCreate(&ev.e[1]), "event");
+        hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
+        qz::hip_check(hipEventRecord(e0, s0.stream), "event");
+        const int rc = qz_sweep_batched(args.data(), n, s0.stream);
+        qz::hip_check(hipEventRecord(e1, s0.stream), "event");
+        qz::hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        qz::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
+        if (batch_ms) *batch_ms = ms;
+        for (int i = 0; i < n; ++i) {
+            qz::Simulation& s = *hs[i]->s;
+            s.finish_sweep(ms);
+            s.after_sweep();
+        }
+    })
+}
 extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
 extern "C" int qz_sim_push_beams(qz_sim* h) {
     QZ_SIM_TRY({

@@ -119,7 +119,10 @@ public:
     ~Simulation();
 
     void step_fields();      // the quasi-static sweep of the current step n
+    void prepare_sweep();    // step_fields before the slice loop; fills sweep_args
+    void finish_sweep(float sweep_ms);   // step_fields after the slice loop
     void push_beams();       // the beam push of step n (not on the last step)
+    void after_sweep();      // output, beam push and step counter after the sweep
     void step();             // one iteration of Simulation::run
     void write_fields(const std::string& path);
 
@@ -141,6 +144,7 @@ public:
     DevBuf<int> lost;
     DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..10] sweep phase ticks, [14] beam deposited, [15] pushed
     DevBuf<char> dep_work;
+    qz_sweep_args sweep_args{};
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;

@@ -365,6 +365,76 @@ extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* b
 }
 
 // Beam::sums (src/Beam.cpp:1133-1155): copied to the host and summed in index order like the reference
+// ---------------------------------------------------------------------------- on-device beam sums
+// The 17 sums of Beam::sums (src/Beam.cpp:1133-1155) as a fixed-shape reduction: block b sums particles
+// [b * kSumChunk, (b + 1) * kSumChunk) with thread t taking every kSumBlock-th particle from t, then a tree in
+// LDS; a second kernel adds the block partials in block order.  The order depends only on np, so the result
+// is reproducible run to run (it differs from the sequential host loop in the last bits).
+constexpr int kSumBlock = 256;
+constexpr int kSumChunk = 16 * kSumBlock;
+
+__global__ __launch_bounds__(kSumBlock) void beam_sums_partial(int np, qz_beam_soa P, Real gref, Real xiref,
+                                                               Real* __restrict__ part) {
+    __shared__ Real sh[17][kSumBlock];
+    const int t = threadIdx.x;
+    const long lo = static_cast<long>(blockIdx.x) * kSumChunk;
+    const long hi = min(static_cast<long>(np), lo + kSumChunk);
+    Real acc[17];
+#pragma unroll
+    for (int c = 0; c < 17; ++c) acc[c] = Real(0);
+    for (long i = lo + t; i < hi; i += kSumBlock) {
+        const Real w = P.w[i];
+        if (w == Real(0)) continue;
+        acc[16] += Real(1);
+        const Real PX = P.px[i], PY = P.py[i], PZ = P.pz[i];
+        const Real g = dsqrt(Real(1) + PX * PX + PY * PY + PZ * PZ);
+        const Real X = P.x[i], Y = P.y[i];
+        const Real G = g - gref, XI = P.xi[i] - xiref;
+        const Real v[15] = {G, G * G, XI, XI * XI, X * X + Y * Y, X, Y, X * X, Y * Y, PX, PY, PX * PX, PY * PY,
+                            X * PX, Y * PY};
+        acc[0] += w;
+#pragma unroll
+        for (int c = 0; c < 15; ++c) acc[c + 1] += w * v[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 17; ++c) sh[c][t] = acc[c];
+    __syncthreads();
+    for (int h = kSumBlock / 2; h > 0; h >>= 1) {
+        if (t < h)
+            for (int c = 0; c < 17; ++c) sh[c][t] += sh[c][t + h];
+        __syncthreads();
+    }
+    if (t < 17) part[static_cast<size_t>(blockIdx.x) * 17 + t] = sh[t][0];
+}
+
+__global__ void beam_sums_final(int nb, const Real* __restrict__ part, Real* __restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= 17) return;
+    Real s = Real(0);
+    for (int b = 0; b < nb; ++b) s += part[static_cast<size_t>(b) * 17 + c];
+    out[c] = s;
+}
+
+extern "C" size_t qz_beam_sums_workspace_bytes(int np) {
+    const long nb = (static_cast<long>(np) + kSumChunk - 1) / kSumChunk;
+    return static_cast<size_t>(nb > 0 ? nb : 1) * 17 * sizeof(Real);
+}
+
+extern "C" int qz_beam_sums_device(int np, const qz_beam_soa* p, double gref, double xiref, double* out17, void* work,
+                                   void* stream) {
+    if (!p || !out17 || !work || np < 0) return qz_fail(QZ_EINVAL, "qz_beam_sums_device: bad argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nb = static_cast<int>((static_cast<long>(np) + kSumChunk - 1) / kSumChunk);
+    Real* part = static_cast<Real*>(work);
+    if (nb > 0) {
+        hipLaunchKernelGGL(beam_sums_partial, dim3(nb), dim3(kSumBlock), 0, st, np, *p, gref, xiref, part);
+        QZ_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(beam_sums_final, dim3(1), dim3(64), 0, st, nb, part, out17);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
 extern "C" int qz_beam_sums(int np, const qz_beam_soa* p, double gref, double xiref, double* out, void* stream) {
     if (!p || !out || np < 0) return qz_fail(QZ_EINVAL, "qz_beam_sums: bad argument");
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -196,3 +196,70 @@ def test_betatron_m1_steps_bit_exact(qa, oracle, decks):
     for k in qa.BEAM_ARRAYS:
         a, b = g.get(f"beam0.{k}"), o.get(f"beam0.{k}")
         assert _same(a, b), _report(f"witness {k}", a, b)
+
+
+# ------------------------------------------------------------------------------------------ batched runs
+def _batched_vs_plain(qa, deck, overrides, steps):
+    plain = [qa.Simulation(deck, ov) for ov in overrides]
+    batch = [qa.Simulation(deck, ov) for ov in overrides]
+    for _ in range(steps):
+        for s in plain:
+            s.step()
+        qa.step_batched(batch)
+    for i, (a, b) in enumerate(zip(batch, plain)):
+        for w in ("fld", "diag", "bsrc"):
+            ga, gb = getattr(a, w), getattr(b, w)
+            assert _same(ga, gb), _report(f"run {i} {w}", ga, gb)
+        for bi in range(a.nbeams):
+            for k in qa.BEAM_ARRAYS:
+                ga, gb = a.get(f"beam{bi}.{k}"), b.get(f"beam{bi}.{k}")
+                assert _same(ga, gb), _report(f"run {i} beam{bi}.{k}", ga, gb)
+        assert a.lost() == b.lost()
+
+
+def test_batched_m0_equals_plain(qa, decks):
+    """Runs of different grids, slice counts and particle counts in one launch: each bit-identical to its
+    own plain launch (the batched form adds no cross-run interaction)."""
+    ov = [{"time.steps": 3, "driver.nparticles": 20000, "xi.min": -8, "grid.dr": 0.04},
+          {"time.steps": 3, "driver.nparticles": 30000, "xi.min": -6, "grid.dr": 0.02, "electrons.ppc": 2},
+          {"time.steps": 3, "driver.nparticles": 10000, "xi.min": -9, "grid.dr": 0.08, "electrons.ppc": 8}]
+    _batched_vs_plain(qa, f"{decks}/gain.in", ov, 3)
+
+
+def test_batched_m1_equals_plain(qa, decks):
+    ov = [{"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
+           "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 2},
+          {"modes": 1, "xi.min": -4, "grid.dr": 0.05, "driver.n0": 0.3, "driver.x0": 0.1,
+           "driver.nparticles": 10000, "driver.sigma_r": 0.4, "electrons.ppc": 1, "time.steps": 2}]
+    _batched_vs_plain(qa, f"{decks}/linear.in", ov, 2)
+
+
+# ------------------------------------------------------------------------------------------ beam sums
+@pytest.mark.parametrize("np_", [0, 1, 4095, 4097, 300001])
+def test_beam_sums_device_matches_host(torch_dev, qa, np_):
+    """On-device Beam::sums against the index-ordered host sums (bit-identical to the reference's loop):
+    the live count exactly, every weighted sum to 1e-12 of the sum of its absolute terms."""
+    torch = torch_dev
+    rng = np.random.default_rng(np_ + 11)
+    host = {k: rng.normal(0, 1, np_) for k in qa.BEAM_ARRAYS}
+    host["pz"] = 2000 + 20 * rng.normal(0, 1, np_)
+    host["w"] = rng.uniform(0.5, 1.5, np_) * (rng.uniform(0, 1, np_) > 0.1)
+    p = {k: torch.tensor(v, dtype=torch.float64, device="cuda") for k, v in host.items()}
+    gref, xiref = 2000.0, -2.0
+    ref = qa.beam_sums(p, gref, xiref)
+    out = torch.zeros(17, dtype=torch.float64, device="cuda")
+    work = torch.zeros(max(1, qa.beam_sums_workspace_bytes(np_) // 8), dtype=torch.float64, device="cuda")
+    qa.beam_sums_device(p, gref, xiref, out, work)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert got[16] == ref[16] == np.count_nonzero(host["w"])
+    # absolute scale of every sum: the same sums over |terms|
+    w = host["w"]
+    g = np.sqrt(1 + host["px"] ** 2 + host["py"] ** 2 + host["pz"] ** 2)
+    G, XI = g - gref, host["xi"] - xiref
+    X, Y, PX, PY = host["x"], host["y"], host["px"], host["py"]
+    terms = [np.ones_like(w), G, G * G, XI, XI * XI, X * X + Y * Y, X, Y, X * X, Y * Y, PX, PY, PX * PX, PY * PY,
+             X * PX, Y * PY]
+    for c, v in enumerate(terms):
+        scale = np.abs(w * v).sum()
+        assert abs(got[c] - ref[c]) <= 1e-12 * scale + 1e-300, (c, got[c], ref[c], scale)
