@@ -46,6 +46,7 @@ template <class T> void DevBuf<T>::upload(const std::vector<T>& h, hipStream_t s
 template <class T> void DevBuf<T>::download(std::vector<T>& h, hipStream_t st) const {
     h.resize(n);
     if (n) hip_check(hipMemcpyAsync(h.data(), p, sizeof(T) * n, hipMemcpyDeviceToHost, st), "download");
+    hip_check(hipStreamSynchronize(st), "download sync");   // the host reads h right after
     hip_check(hipStreamSynchronize(st), "sync");
 }
 template struct DevBuf<Real>;
@@ -667,9 +668,9 @@ void Simulation::prepare_sweep() {
             if (s->mobile) s->deposit_rho(grid, a, b);
         for (size_t j = 0; j < a.size(); ++j) { bgt[j] -= a[j]; bgr[j] -= b[j]; }
     }
-    std::vector<Real> bgall(bgt);
-    bgall.insert(bgall.end(), bgr.begin(), bgr.end());
-    bg.upload(bgall, stream);
+    bg_host = bgt;   // a member: the asynchronous upload reads it after this function returns
+    bg_host.insert(bg_host.end(), bgr.begin(), bgr.end());
+    bg.upload(bg_host, stream);
     hip_check(hipMemsetAsync(lost.p, 0, sizeof(int) * lost.n, stream), "lost reset");
     // ---- the sweep
     qz_sweep_args A{};

@@ -145,6 +145,7 @@ public:
     DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..10] sweep phase ticks, [14] beam deposited, [15] pushed
     DevBuf<char> dep_work;
     qz_sweep_args sweep_args{};
+    std::vector<Real> bg_host;   // host side of the background upload (kept alive for the async copy)
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
