@@ -228,6 +228,26 @@ __device__ __forceinline__ void wave_exclusive_scan(int* __restrict__ v, int n, 
     }
 }
 
+// ((0 + v[e0]) + v[e0+1]) + ... + v[e1-1]: the sequential sum of one node list, with the LDS loads issued
+// eight at a time ahead of the dependent adds
+__device__ __forceinline__ Real sum_list(const Real* v, int e0, int e1) {
+    Real acc = Real(0);
+    int e = e0;
+    for (; e + 8 <= e1; e += 8) {
+        const Real x0 = v[e], x1 = v[e + 1], x2 = v[e + 2], x3 = v[e + 3];
+        const Real x4 = v[e + 4], x5 = v[e + 5], x6 = v[e + 6], x7 = v[e + 7];
+        acc += x0; acc += x1; acc += x2; acc += x3;
+        acc += x4; acc += x5; acc += x6; acc += x7;
+    }
+    if (e + 4 <= e1) {
+        const Real x0 = v[e], x1 = v[e + 1], x2 = v[e + 2], x3 = v[e + 3];
+        acc += x0; acc += x1; acc += x2; acc += x3;
+        e += 4;
+    }
+    for (; e < e1; ++e) acc += v[e];
+    return acc;
+}
+
 // entries < key in the index-sorted range idx[lo, hi)
 __device__ __forceinline__ int count_less(const unsigned short* __restrict__ idx, int lo, int hi, int key) {
     int a = lo, b = hi;
@@ -522,8 +542,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     const int g = item / M, n = item - g * M;
                     const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
                     const Real* vb = B + g * 2 * L;
-                    Real acc = Real(0);
-                    for (int e = e0; e < e1; ++e) acc += vb[e];
+                    const Real acc = sum_list(vb, e0, e1);
                     const int comp = s_comp[c0 + g];
                     Real* Dn = Dg + static_cast<size_t>(n) * ncd;
                     if (comp == W_NE0) Dn[D_NE0] = acc;
@@ -800,8 +819,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     const int g = item / M, n = item - g * M;
                     const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
                     const Real* vb = VB + g * 2 * L;
-                    Real acc = Real(0);
-                    for (int e = e0; e < e1; ++e) acc += vb[e];
+                    const Real acc = sum_list(vb, e0, e1);
                     WS(W_SR1 + c0 + g)[n] = acc;
                 }
                 __syncthreads();

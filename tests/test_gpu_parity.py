@@ -220,17 +220,19 @@ def _batched_vs_plain(qa, deck, overrides, steps):
 def test_batched_m0_equals_plain(qa, decks):
     """Runs of different grids, slice counts and particle counts in one launch: each bit-identical to its
     own plain launch (the batched form adds no cross-run interaction)."""
-    ov = [{"time.steps": 3, "driver.nparticles": 20000, "xi.min": -8, "grid.dr": 0.04},
-          {"time.steps": 3, "driver.nparticles": 30000, "xi.min": -6, "grid.dr": 0.02, "electrons.ppc": 2},
-          {"time.steps": 3, "driver.nparticles": 10000, "xi.min": -9, "grid.dr": 0.08, "electrons.ppc": 8}]
+    ov = [{"time.steps": 3, "driver.nparticles": 20000, "xi.min": -8},
+          {"time.steps": 3, "driver.nparticles": 30000, "xi.min": -6, "grid.type": "uniform", "grid.dr": 0.02,
+           "electrons.ppc": 2},
+          {"time.steps": 3, "driver.nparticles": 10000, "xi.min": -9, "grid.type": "uniform", "grid.dr": 0.08,
+           "electrons.ppc": 8}]
     _batched_vs_plain(qa, f"{decks}/gain.in", ov, 3)
 
 
 def test_batched_m1_equals_plain(qa, decks):
     ov = [{"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
-           "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 2},
+           "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 2, "time.dt": 5},
           {"modes": 1, "xi.min": -4, "grid.dr": 0.05, "driver.n0": 0.3, "driver.x0": 0.1,
-           "driver.nparticles": 10000, "driver.sigma_r": 0.4, "electrons.ppc": 1, "time.steps": 2}]
+           "driver.nparticles": 10000, "driver.sigma_r": 0.4, "electrons.ppc": 1, "time.steps": 2, "time.dt": 5}]
     _batched_vs_plain(qa, f"{decks}/linear.in", ov, 2)
 
 
