@@ -64,6 +64,7 @@ typedef struct qz_species_state {
 } qz_species_state;
 
 #define QZ_MAX_SPECIES 2
+#define QZ_NPHASE 16      /* sweep phase clocks: see qz_sim_phase_ticks */
 
 /* Everything one quasi-static sweep reads and writes (Simulation::step_fields, src/Simulation.cpp:636-795). */
 typedef struct qz_sweep_args {
@@ -95,7 +96,7 @@ typedef struct qz_sweep_args {
     int* lost;                 /* one counter per species (particles removed as trapped) */
     double* ws;                /* per-run workspace of qz_sweep_workspace_doubles(M) doubles */
     unsigned long long* stats; /* optional: stats[0] += live particles of every slice (particle-slices),
-                                  stats[1..10] += wall-clock ticks of the sweep phases */
+                                  stats[1..QZ_NPHASE] += wall-clock ticks of the sweep phases */
 } qz_sweep_args;
 
 const char* qz_last_error(void);
@@ -166,9 +167,11 @@ int qz_sim_timing(const qz_sim* s, double* out2);
 /* work of the last step: particle-slices of the sweep, beam particles pushed, beam particles deposited,
  * sweep kernel ms, beam deposit ms, beam push ms */
 int qz_sim_stats(const qz_sim* s, double* out6);
-/* wall-clock ticks (100 MHz) of the phases of the last sweep, measured by one thread of the run:
- * geometry, sort, deposit, combine, B_z rhs, psi/E_z/B_z solves, W+, S+ deposit, Picard B+, store+push */
-int qz_sim_phase_ticks(const qz_sim* s, double* out10);
+/* wall-clock ticks (100 MHz) of the QZ_NPHASE phases of the last sweep, measured by the first thread of the
+ * run (a phase includes that thread's wait at the barrier that ends it): geometry, sort, deposit
+ * contributions, deposit node sums, combine, B_z rhs, psi/E_z/B_z solves, W+, S+ gather, S+ contributions,
+ * S+ node sums, Picard staging, Picard rhs, Picard solves, store, push */
+int qz_sim_phase_ticks(const qz_sim* s, double* out16);
 long qz_sim_lost(const qz_sim* s);
 double qz_sim_time(const qz_sim* s);
 int qz_sim_write_fields(qz_sim* s, const char* path);

@@ -148,11 +148,12 @@ class Simulation:
         check(lib().qz_sim_stats(self.h, out.ctypes.data_as(DP)))
         return dict(zip(["particle_slices", "beam_pushed", "beam_deposited", "sweep_ms", "beam_dep_ms", "beam_push_ms"], out))
 
-    PHASES = ["geometry", "sort", "deposit", "combine", "bz_rhs", "solves", "wplus", "s_deposit", "picard", "store_push"]
+    PHASES = ["geometry", "sort", "dep_contrib", "dep_sums", "combine", "bz_rhs", "solves", "wplus", "s_gather",
+              "s_contrib", "s_sums", "picard_stage", "picard_rhs", "picard_solve", "store", "push"]
 
     def phase_ms(self):
         """Milliseconds of each sweep phase of the last sweep (one thread's wall clock, 100 MHz ticks)."""
-        out = np.zeros(10)
+        out = np.zeros(len(self.PHASES))
         check(lib().qz_sim_phase_ticks(self.h, out.ctypes.data_as(DP)))
         return dict(zip(self.PHASES, out * 1e-5))
 

@@ -602,7 +602,7 @@ Simulation::Simulation(const std::string& deck, const std::string& overrides) { 
     bg.alloc(6 * static_cast<size_t>(M));
     ws.alloc(qz_sweep_workspace_doubles(M));
     lost.alloc(QZ_MAX_SPECIES);
-    counters.alloc(16);
+    counters.alloc(24);
     int maxnp = 0;
     for (const auto& b : beams) maxnp = std::max(maxnp, b->Np);
     dep_work.alloc(qz_beam_deposit_workspace_bytes(std::max(maxnp, 1), K, grid.N));
@@ -651,7 +651,7 @@ void Simulation::prepare_sweep() {
     hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * counters.n, stream), "counter reset");
     for (const auto& b : beams) {
         const qz_beam_soa s = b->soa();
-        qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 14, stream));
+        qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 20, stream));
         qz_check(qz_beam_deposit(&g, &box, m1 ? 1 : 0, b->q, b->Np, &s, bsrc.p, NCB, dep_work.p, stream));
     }
     hip_check(hipEventRecord(ev[5], stream), "event");
@@ -723,8 +723,8 @@ void Simulation::finish_sweep(float sweep_ms) {
     std::vector<unsigned long long> cnt;
     counters.download(cnt, stream);
     particle_slices = static_cast<double>(cnt[0]);
-    for (int i = 0; i < 10; ++i) phase_ticks[i] = static_cast<double>(cnt[1 + i]);
-    beam_deposited = static_cast<double>(cnt[14]);
+    for (int i = 0; i < QZ_NPHASE; ++i) phase_ticks[i] = static_cast<double>(cnt[1 + i]);
+    beam_deposited = static_cast<double>(cnt[20]);
     swept = true;
 }
 
@@ -736,9 +736,9 @@ void Simulation::push_beams() {
     if (!have_prev) { dt_prev = dt; have_prev = true; }
     const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
     const qz_radial_grid g = grid.view();
-    hip_check(hipMemsetAsync(counters.p + 15, 0, sizeof(unsigned long long), stream), "counter reset");
+    hip_check(hipMemsetAsync(counters.p + 21, 0, sizeof(unsigned long long), stream), "counter reset");
     for (auto& b : beams)
-        if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 15, stream));
+        if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 21, stream));
     hip_check(hipEventRecord(ev[2], stream), "event");
     for (auto& b : beams) {
         if (b->rigid) continue;
@@ -756,7 +756,7 @@ void Simulation::push_beams() {
     hip_check(hipEventElapsedTime(&ms_beam, ev[2], ev[3]), "elapsed");
     {
         unsigned long long v = 0;
-        hip_check(hipMemcpy(&v, counters.p + 15, sizeof(v), hipMemcpyDeviceToHost), "counter");
+        hip_check(hipMemcpy(&v, counters.p + 21, sizeof(v), hipMemcpyDeviceToHost), "counter");
         beam_pushed = static_cast<double>(v);
     }
     t += dt;
@@ -1024,9 +1024,9 @@ extern "C" int qz_sim_stats(const qz_sim* h, double* out6) {
     out6[5] = s.ms_beam;
     return QZ_OK;
 }
-extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out10) {
-    if (!h || !out10) return qz_fail(QZ_EINVAL, "qz_sim_phase_ticks: null argument");
-    for (int i = 0; i < 10; ++i) out10[i] = h->s->phase_ticks[i];
+extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out) {
+    if (!h || !out) return qz_fail(QZ_EINVAL, "qz_sim_phase_ticks: null argument");
+    for (int i = 0; i < QZ_NPHASE; ++i) out[i] = h->s->phase_ticks[i];
     return QZ_OK;
 }
 extern "C" long qz_sim_lost(const qz_sim* h) { return h->s->lost_last; }

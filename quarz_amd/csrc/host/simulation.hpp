@@ -142,14 +142,14 @@ public:
     int M = 0, K = 0, NCF = 0, NCB = 0, NCD = 0;
     DevBuf<Real> fld, bsrc, diag, bg, ws;
     DevBuf<int> lost;
-    DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..10] sweep phase ticks, [14] beam deposited, [15] pushed
+    DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..16] sweep phase ticks, [20] beam deposited, [21] pushed
     DevBuf<char> dep_work;
     qz_sweep_args sweep_args{};
     std::vector<Real> bg_host;   // host side of the background upload (kept alive for the async copy)
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
-    double phase_ticks[10] = {};
+    double phase_ticks[QZ_NPHASE] = {};
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     std::string outdir;

@@ -120,30 +120,42 @@ __device__ __forceinline__ Real dep_value(int comp, int cc, int j, Real t, Real 
 //   dp_0 = d_0 / b_0,  dp_i = (d_i - a_i dp_{i-1}) m_i,  x_{M-1} = dp_{M-1},  x_i = dp_i - c'_i x_{i+1}
 // d_i is the prepared right-hand side of FieldSolver::solve (src/FieldSolver.cpp:253-259): 0 on Dirichlet
 // rows, scale * rhs otherwise, minus fcw on the flux row.  w holds rhs on entry and dp afterwards; the
-// solution goes to x (which may be w).  Rows are loaded four at a time, one group ahead of the arithmetic.
-__device__ __forceinline__ void thomas_lds(const Real* a, const Real* m, const Real* cp, Real b0, Real* w, Real* x, Real scale,
-                           bool dir0, bool dirN, bool fluxN, Real fcw, int M) {
+// solution goes to x (which may be w).  Rows go in groups of four through two register sets: one group is
+// loaded while the other is computed, so the LDS latency stays off the recurrence.
+struct FwdRows { Real d0, d1, d2, d3, a0, a1, a2, a3, m0, m1, m2, m3; };
+struct BwdRows { Real p0, p1, p2, p3, c0, c1, c2, c3; };
+
+__device__ __forceinline__ void thomas_lds(const Real* a, const Real* m, const Real* cp, Real b0, Real* w, Real* x,
+                                           Real scale, bool dir0, bool dirN, bool fluxN, Real fcw, int M) {
     const int N = M - 1;
     Real prev = (dir0 ? Real(0) : scale * w[0]) / b0;
     w[0] = prev;
+    auto ldf = [&](FwdRows& g, int r) {
+        g.d0 = w[r]; g.d1 = w[r + 1]; g.d2 = w[r + 2]; g.d3 = w[r + 3];
+        g.a0 = a[r]; g.a1 = a[r + 1]; g.a2 = a[r + 2]; g.a3 = a[r + 3];
+        g.m0 = m[r]; g.m1 = m[r + 1]; g.m2 = m[r + 2]; g.m3 = m[r + 3];
+    };
+    auto stepf = [&](const FwdRows& g, int r) {
+        prev = (scale * g.d0 - g.a0 * prev) * g.m0; w[r] = prev;
+        prev = (scale * g.d1 - g.a1 * prev) * g.m1; w[r + 1] = prev;
+        prev = (scale * g.d2 - g.a2 * prev) * g.m2; w[r + 2] = prev;
+        prev = (scale * g.d3 - g.a3 * prev) * g.m3; w[r + 3] = prev;
+    };
     const int nq = (N - 1) >> 2;   // full groups of four rows in 1 .. N-1
     int i = 1;
     if (nq > 0) {
-        Real d0 = w[1], d1 = w[2], d2 = w[3], d3 = w[4];
-        Real a0 = a[1], a1 = a[2], a2 = a[3], a3 = a[4];
-        Real m0 = m[1], m1 = m[2], m2 = m[3], m3 = m[4];
-        for (int q = 0; q < nq; ++q, i += 4) {
-            const int nx = (q + 1 < nq) ? i + 4 : i;   // next group (the last one reloads its own rows)
-            const Real e0 = w[nx], e1 = w[nx + 1], e2 = w[nx + 2], e3 = w[nx + 3];
-            const Real b0_ = a[nx], b1 = a[nx + 1], b2 = a[nx + 2], b3 = a[nx + 3];
-            const Real n0 = m[nx], n1 = m[nx + 1], n2 = m[nx + 2], n3 = m[nx + 3];
-            prev = (scale * d0 - a0 * prev) * m0; w[i] = prev;
-            prev = (scale * d1 - a1 * prev) * m1; w[i + 1] = prev;
-            prev = (scale * d2 - a2 * prev) * m2; w[i + 2] = prev;
-            prev = (scale * d3 - a3 * prev) * m3; w[i + 3] = prev;
-            d0 = e0; d1 = e1; d2 = e2; d3 = e3;
-            a0 = b0_; a1 = b1; a2 = b2; a3 = b3;
-            m0 = n0; m1 = n1; m2 = n2; m3 = n3;
+        FwdRows A, B;
+        ldf(A, 1);
+        int q = 0;
+        for (; q + 2 <= nq; q += 2, i += 8) {
+            ldf(B, i + 4);
+            stepf(A, i);
+            ldf(A, (q + 2 < nq) ? i + 8 : i + 4);   // the last pass reloads rows it already holds
+            stepf(B, i + 4);
+        }
+        if (q < nq) {
+            stepf(A, i);
+            i += 4;
         }
     }
     for (; i < N; ++i) {
@@ -161,22 +173,32 @@ __device__ __forceinline__ void thomas_lds(const Real* a, const Real* m, const R
     }
     Real xn = prev;
     x[N] = xn;
-    // backward: rows N-1 .. 0, four at a time from the top, loads one group ahead
+    // backward: rows N-1 .. 0, groups of four from the top
+    auto ldb = [&](BwdRows& g, int r) {
+        g.p0 = w[r]; g.p1 = w[r - 1]; g.p2 = w[r - 2]; g.p3 = w[r - 3];
+        g.c0 = cp[r]; g.c1 = cp[r - 1]; g.c2 = cp[r - 2]; g.c3 = cp[r - 3];
+    };
+    auto stepb = [&](const BwdRows& g, int r) {
+        xn = g.p0 - g.c0 * xn; x[r] = xn;
+        xn = g.p1 - g.c1 * xn; x[r - 1] = xn;
+        xn = g.p2 - g.c2 * xn; x[r - 2] = xn;
+        xn = g.p3 - g.c3 * xn; x[r - 3] = xn;
+    };
     i = N - 1;
     const int nb = N >> 2;
     if (nb > 0) {
-        Real p0 = w[i], p1 = w[i - 1], p2 = w[i - 2], p3 = w[i - 3];
-        Real c0 = cp[i], c1 = cp[i - 1], c2 = cp[i - 2], c3 = cp[i - 3];
-        for (int q = 0; q < nb; ++q, i -= 4) {
-            const int nx = (q + 1 < nb) ? i - 4 : i;
-            const Real e0 = w[nx], e1 = w[nx - 1], e2 = w[nx - 2], e3 = w[nx - 3];
-            const Real f0 = cp[nx], f1 = cp[nx - 1], f2 = cp[nx - 2], f3 = cp[nx - 3];
-            xn = p0 - c0 * xn; x[i] = xn;
-            xn = p1 - c1 * xn; x[i - 1] = xn;
-            xn = p2 - c2 * xn; x[i - 2] = xn;
-            xn = p3 - c3 * xn; x[i - 3] = xn;
-            p0 = e0; p1 = e1; p2 = e2; p3 = e3;
-            c0 = f0; c1 = f1; c2 = f2; c3 = f3;
+        BwdRows A, B;
+        ldb(A, i);
+        int q = 0;
+        for (; q + 2 <= nb; q += 2, i -= 8) {
+            ldb(B, i - 4);
+            stepb(A, i);
+            ldb(A, (q + 2 < nb) ? i - 8 : i - 4);
+            stepb(B, i - 4);
+        }
+        if (q < nb) {
+            stepb(A, i);
+            i -= 4;
         }
     }
     for (; i >= 0; --i) {
@@ -246,16 +268,6 @@ __device__ __forceinline__ Real sum_list(const Real* v, int e0, int e1) {
     }
     for (; e < e1; ++e) acc += v[e];
     return acc;
-}
-
-// entries < key in the index-sorted range idx[lo, hi)
-__device__ __forceinline__ int count_less(const unsigned short* __restrict__ idx, int lo, int hi, int key) {
-    int a = lo, b = hi;
-    while (a < b) {
-        const int mid = (a + b) >> 1;
-        if (static_cast<int>(idx[mid]) < key) a = mid + 1; else b = mid;
-    }
-    return a - lo;
 }
 
 // Opaque register fence: the compiler must assume the values changed here, so nothing derived from them is
@@ -350,7 +362,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
     int* s_misc = reinterpret_cast<int*>(smem + plan.misc);      // [0] live count
     Real* s_bw0 = reinterpret_cast<Real*>(smem + plan.misc + 16);                        // 4 doubles
     int* s_comp = reinterpret_cast<int*>(smem + plan.misc + 48);                         // 24 ints
-    unsigned long long* s_ph = reinterpret_cast<unsigned long long*>(smem + plan.misc + 160);   // 11 clocks
+    unsigned long long* s_ph = reinterpret_cast<unsigned long long*>(smem + plan.misc + 160);   // phase clocks + last
     Real* B = reinterpret_cast<Real*>(smem + plan.region);
     const int Bcap = (kLdsBytes - plan.region) / 8;
     auto BA = [&](int slot) -> Real* { return B + static_cast<size_t>(slot) * Mp; };
@@ -379,8 +391,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         for (int a = W_BPR1; a <= W_BPI2; ++a) WS(a)[j] = Real(0);   // B+ := 0 at the head (Simulation.cpp:705-708)
     }
     if (tid == 0) {
-        for (int i = 0; i < 11; ++i) s_ph[i] = 0;
-        s_ph[10] = wall_clock64();
+        for (int i = 0; i < QZ_NPHASE; ++i) s_ph[i] = 0;
+        s_ph[QZ_NPHASE] = wall_clock64();
     }
     const int nS = M1 ? 6 : 2;
     const Real dmin = A.delta_min, qsamax = A.max_qsa;
@@ -389,8 +401,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
     do {                                                     \
         if (tid == 0 && stats) {                             \
             const unsigned long long tnow = wall_clock64();  \
-            s_ph[i] += tnow - s_ph[10];                      \
-            s_ph[10] = tnow;                                 \
+            s_ph[i] += tnow - s_ph[QZ_NPHASE];               \
+            s_ph[QZ_NPHASE] = tnow;                          \
         }                                                    \
     } while (0)
     __syncthreads();
@@ -413,9 +425,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         auto slot_of = [&](int pp) { return w * (PT * 64) + pp * 64 + lane; };
         // momenta, Delta and weight of cached particle pp (species arrays, L2-resident); the slot is made
         // opaque so the addresses are formed at each use instead of being held in registers
-        auto load_mom = [&](int pp, Real& PX, Real& PY, Real& D, Real& wi) {
-            int gi = slot_of(pp);
-            asm volatile("" : "+v"(gi));
+        auto load_mom = [&](int pp, int ln, Real& PX, Real& PY, Real& D, Real& wi) {
+            const int gi = w * (PT * 64) + pp * 64 + ln;
             const bool s1 = gi >= np0;
             const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
             const int li = s1 ? gi - np0 : gi;
@@ -426,6 +437,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         for (int pp = 0; pp < PT; ++pp) {
             const int gi = slot_of(pp);
             cj[pp] = -1;
+            ct[pp] = Real(0);
             if (gi < NP) {
                 const bool s1 = gi >= np0;
                 const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
@@ -437,10 +449,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     const Real r = dsqrt(X * X + Y * Y);
                     Real c, s_;
                     if (r > Real(0)) { c = X / r; s_ = Y / r; } else { c = Real(1); s_ = Real(0); }
-                    int j; Real t;
-                    grid_weights(s_r, s_ih, N, R, r, j, t);
-                    cj[pp] = j;
-                    ct[pp] = t;
+                    cj[pp] = 0;
+                    ct[pp] = r;   // the radius until the cell search below
                     cc[pp] = c;
                     cs[pp] = s_;
                     cir[pp] = r > Real(0) ? Real(1) / r : Real(0);
@@ -448,6 +458,39 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     cgam[pp] = gam;
                     cinvd[pp] = Real(1) / D;
                 }
+            }
+        }
+        // cell and weight of every particle, the binary searches of all cached particles advanced together
+        // (RadialGrid::locate / weights, src/RadialGrid.hpp:35-50: r <= 0 -> cell 0, r >= R -> cell N-1)
+        {
+            int lo[PT], hi[PT];
+#pragma unroll
+            for (int pp = 0; pp < PT; ++pp) {
+                const Real r = ct[pp];
+                lo[pp] = 0;
+                hi[pp] = N;
+                if (r <= Real(0)) hi[pp] = 1;
+                else if (r >= R) lo[pp] = N - 1;
+            }
+            int nbits = 1;
+            while ((1 << nbits) < N) ++nbits;
+            for (int it = 0; it < nbits; ++it) {
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp) {
+                    if (cj[pp] >= 0 && hi[pp] - lo[pp] > 1) {
+                        const int mid = (lo[pp] + hi[pp]) >> 1;
+                        if (s_r[mid] <= ct[pp]) lo[pp] = mid; else hi[pp] = mid;
+                    }
+                }
+            }
+#pragma unroll
+            for (int pp = 0; pp < PT; ++pp) {
+                if (cj[pp] < 0) continue;
+                const int j = lo[pp];
+                Real t = (ct[pp] - s_r[j]) * s_ih[j];
+                t = t < Real(0) ? Real(0) : (t > Real(1) ? Real(1) : t);
+                cj[pp] = j;
+                ct[pp] = t;
             }
         }
         QZ_PHASE(0);
@@ -512,18 +555,50 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index order
+        // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index
+        // order: a particle's slot counts the smaller indices of the neighbour cell (binary searches of all
+        // cached particles advanced together)
+        {
+            int am[PT], bm[PT], ap[PT], bp[PT];
 #pragma unroll
-        for (int pp = 0; pp < PT; ++pp) {
-            const int j = cj[pp];
-            cpos[pp] = 0;
-            if (j >= 0) {
-                const int gi = slot_of(pp);
-                const int less_m = count_less(s_sidx, s_cs[j], s_cs[j + 1], gi);
-                const int less_p = count_less(s_sidx, s_cs[j + 2], s_cs[j + 3], gi);
-                const int pl = (s_cs[j] + s_cs[j + 1]) + lrank[pp] + less_m;
-                const int pr = (s_cs[j + 1] + s_cs[j + 2]) + lrank[pp] + less_p;
-                cpos[pp] = pl | (pr << 16);
+            for (int pp = 0; pp < PT; ++pp) {
+                const int j = cj[pp];
+                am[pp] = bm[pp] = ap[pp] = bp[pp] = 0;
+                if (j >= 0) {
+                    am[pp] = s_cs[j]; bm[pp] = s_cs[j + 1];
+                    ap[pp] = s_cs[j + 2]; bp[pp] = s_cs[j + 3];
+                }
+            }
+            int lom[PT], lop[PT];
+#pragma unroll
+            for (int pp = 0; pp < PT; ++pp) { lom[pp] = am[pp]; lop[pp] = ap[pp]; }
+            bool more = true;
+            while (more) {
+                more = false;
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp) {
+                    const int key = slot_of(pp);
+                    if (am[pp] < bm[pp]) {
+                        const int mid = (am[pp] + bm[pp]) >> 1;
+                        if (static_cast<int>(s_sidx[mid]) < key) am[pp] = mid + 1; else bm[pp] = mid;
+                        more |= am[pp] < bm[pp];
+                    }
+                    if (ap[pp] < bp[pp]) {
+                        const int mid = (ap[pp] + bp[pp]) >> 1;
+                        if (static_cast<int>(s_sidx[mid]) < key) ap[pp] = mid + 1; else bp[pp] = mid;
+                        more |= ap[pp] < bp[pp];
+                    }
+                }
+            }
+#pragma unroll
+            for (int pp = 0; pp < PT; ++pp) {
+                const int j = cj[pp];
+                cpos[pp] = 0;
+                if (j >= 0) {
+                    const int pl = (s_cs[j] + s_cs[j + 1]) + lrank[pp] + (am[pp] - lom[pp]);
+                    const int pr = (s_cs[j + 1] + s_cs[j + 2]) + lrank[pp] + (ap[pp] - lop[pp]);
+                    cpos[pp] = pl | (pr << 16);
+                }
             }
         }
         const int G = max(1, min(Bcap / max(2 * L, 1), 8));
@@ -556,6 +631,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     } else WS(comp)[n] = acc;
                 }
                 __syncthreads();
+                QZ_PHASE(3);
                 c0 += gcur;
                 gcur = 0;
             };
@@ -564,12 +640,17 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 constexpr int C = decltype(cst)::value;
                 Real* vb = B + gcur * 2 * L;
                 fence_cache();
+                int ln = lane;
+                asm volatile("" : "+v"(ln));   // addresses formed here, loads issued together
+                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp)
+                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
                 for (int pp = 0; pp < PT; ++pp) {
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    Real PX, PY, D, wi;
-                    load_mom(pp, PX, PY, D, wi);
+                    const Real PX = mPX[pp], PY = mPY[pp], D = mD[pp], wi = mW[pp];
                     const bool s1 = slot_of(pp) >= np0;
                     const Real q = s1 ? q1 : q0, qm = s1 ? qm1 : qm0;
                     const int pl = cpos[pp] & 0xffff, pr = cpos[pp] >> 16;
@@ -580,6 +661,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     vb[pr] = dep_value(C, 1, j, ct[pp], cc[pp], cs[pp], cir[pp], cinvd[pp], cgam[pp], PX, PY, pz, wi, q,
                                        qm, s_iv[j + 1], ih);
                 }
+                QZ_PHASE(2);
                 if (++gcur == G) reduce();
             };
             comp_pass(IC<W_CHI0>{});
@@ -622,7 +704,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(2);
+        QZ_PHASE(3);
 
         // ================================================================ 4. combine_sources (Simulation.cpp:408-457)
         {
@@ -680,7 +762,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(3);
+        QZ_PHASE(4);
         // B_z right-hand side (compute_bz, Simulation.cpp:506-526)
         for (int j = tid; j < M; j += kBS) {
             const Real t1 = stn.grad_odd(BA(LY::JP + 1), j), t2 = stn.over_r(BA(LY::JP + 1), j);
@@ -696,7 +778,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(4);
+        QZ_PHASE(5);
 
         // ================================================================ 5a. psi, E_z, B_z solves || chi factors
         if (w == 0) {
@@ -725,7 +807,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(5);
+        QZ_PHASE(6);
 
         // ================================================================ 5b. field outputs, W+ (compute_wplus, Simulation.cpp:477-498)
         {
@@ -772,7 +854,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 BA(LY::SBZ + 1)[j] = WS(W_BZ1R)[j]; BA(LY::SBZ + 2)[j] = WS(W_BZ1I)[j];
             }
         }
-        QZ_PHASE(6);
+        QZ_PHASE(7);
 
         // ================================================================ 5c. S+ deposit (PlasmaSpecies::deposit_S, PlasmaSpecies.cpp:516-558)
         {
@@ -783,12 +865,18 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             // gather W+, E_z, B_z (PlasmaSpecies.cpp:75-89, without B+) and form the particle's a_x, a_y
             Real cax[PT], cay[PT];
             fence_cache();
+            {
+                int ln = lane;
+                asm volatile("" : "+v"(ln));
+                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp)
+                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
             for (int pp = 0; pp < PT; ++pp) {
                 const int j = cj[pp];
                 if (j < 0) continue;
-                Real PX, PY, D, wi;
-                load_mom(pp, PX, PY, D, wi);
+                const Real PX = mPX[pp], PY = mPY[pp];
                 const bool s1 = slot_of(pp) >= np0;
                 const Real qm = s1 ? qm1 : qm0;
                 const Real t = ct[pp], c = cc[pp], sn = cs[pp];
@@ -812,6 +900,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 cax[pp] = (qm * (gam * Wx + PY * Bz) * invd + fp * c) * invd - PX * dD * invd * invd;
                 cay[pp] = (qm * (gam * Wy - PX * Bz) * invd + fp * sn) * invd - PY * dD * invd * invd;
             }
+            }
+            QZ_PHASE(8);
             int gcur = 0, c0 = 0;
             auto reduce = [&]() {
                 __syncthreads();
@@ -823,6 +913,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     WS(W_SR1 + c0 + g)[n] = acc;
                 }
                 __syncthreads();
+                QZ_PHASE(10);
                 c0 += gcur;
                 gcur = 0;
             };
@@ -833,12 +924,17 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 Real* vb = VB + gcur * 2 * L;
                 fence_cache();
                 fence_regs(cax); fence_regs(cay);
+                int ln = lane;
+                asm volatile("" : "+v"(ln));
+                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
+#pragma unroll
+                for (int pp = 0; pp < PT; ++pp)
+                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
                 for (int pp = 0; pp < PT; ++pp) {
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    Real PX, PY, D, wi;
-                    load_mom(pp, PX, PY, D, wi);
+                    const Real PX = mPX[pp], PY = mPY[pp], wi = mW[pp];
                     const bool s1 = slot_of(pp) >= np0;
                     const Real qq = s1 ? q1 : q0;
                     const Real t = ct[pp], c = cc[pp], sn = cs[pp];
@@ -868,6 +964,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                         vb[side == 0 ? pl : pr] = v;
                     }
                 }
+                QZ_PHASE(9);
                 if (++gcur == GS) reduce();
             };
             s_pass(IC<0>{});
@@ -881,7 +978,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             if (gcur > 0) reduce();
         }
         __syncthreads();
-        QZ_PHASE(7);
+        QZ_PHASE(10);
 
         // ================================================================ 5d. gz and the Picard iterations (Simulation.cpp:535-591)
         for (int j = tid; j < M; j += kBS) {
@@ -913,8 +1010,13 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         {
             const Real fcw = A.flux_coef * WS(W_JZ0)[N];
             const int niter = M1 ? A.picard : 1;
+            __syncthreads();
+            QZ_PHASE(11);
             for (int it = 0; it < niter; ++it) {
-                __syncthreads();
+                if (it > 0) {
+                    __syncthreads();
+                    QZ_PHASE(13);
+                }
                 for (int j = tid; j < M; j += kBS) {
                     Real r1 = -BA(LY::PGZ + 1)[j], i1 = BA(LY::PGZ + 0)[j];
                     if (M1) {
@@ -934,6 +1036,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     BA(LY::PRB + 0)[j] = r1; BA(LY::PRB + 1)[j] = i1;
                 }
                 __syncthreads();
+                QZ_PHASE(12);
                 // lanes: 0 B_r1 (L1D), 1 B_i1 (L1F, flux wall J_z(R)); m1: 2 B_r0, 3 B_i0 (L0), 4 B_r2, 5 B_i2 (L2D)
                 if (w == 0) {
                     const int nj = M1 ? 6 : 2;
@@ -948,7 +1051,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(8);
+        QZ_PHASE(13);
 
         // ================================================================ 6. store (Simulation.cpp:594-633) + push
         for (int j = tid; j < M; j += kBS)
@@ -992,6 +1095,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 }
             }
         }
+        QZ_PHASE(14);
         if (k < K - 1) {
             // ---- Adams-Bashforth push (PlasmaSpecies::push, src/PlasmaSpecies.cpp:600-667)
             const int slot = k % ord;
@@ -1093,10 +1197,10 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        QZ_PHASE(9);
+        QZ_PHASE(15);
     }
     if (tid == 0 && stats)
-        for (int i = 0; i < 10; ++i) atomicAdd(&stats[1 + i], s_ph[i]);
+        for (int i = 0; i < QZ_NPHASE; ++i) atomicAdd(&stats[1 + i], s_ph[i]);
 #undef QZ_PHASE
 }
 
