@@ -218,7 +218,38 @@ __device__ __forceinline__ Real thomas_factor(const Real* a, const Real* b, cons
     Real cprev = c[0] / bw0;
     cp[0] = cprev;
     m[0] = Real(0);
-    for (int i = 1; i < N; ++i) {
+    // rows 1 .. N-1 in groups of four through two register sets, as thomas_lds
+    struct Rows { Real w0, w1, w2, w3, a0, a1, a2, a3, c0, c1, c2, c3; };
+    auto ld = [&](Rows& g, int r) {
+        g.w0 = b[r] - chi[r]; g.w1 = b[r + 1] - chi[r + 1]; g.w2 = b[r + 2] - chi[r + 2]; g.w3 = b[r + 3] - chi[r + 3];
+        g.a0 = a[r]; g.a1 = a[r + 1]; g.a2 = a[r + 2]; g.a3 = a[r + 3];
+        g.c0 = c[r]; g.c1 = c[r + 1]; g.c2 = c[r + 2]; g.c3 = c[r + 3];
+    };
+    auto step = [&](const Rows& g, int r) {
+        Real mi;
+        mi = Real(1) / (g.w0 - g.a0 * cprev); cprev = g.c0 * mi; m[r] = mi; cp[r] = cprev;
+        mi = Real(1) / (g.w1 - g.a1 * cprev); cprev = g.c1 * mi; m[r + 1] = mi; cp[r + 1] = cprev;
+        mi = Real(1) / (g.w2 - g.a2 * cprev); cprev = g.c2 * mi; m[r + 2] = mi; cp[r + 2] = cprev;
+        mi = Real(1) / (g.w3 - g.a3 * cprev); cprev = g.c3 * mi; m[r + 3] = mi; cp[r + 3] = cprev;
+    };
+    const int nq = (N - 1) >> 2;
+    int i = 1;
+    if (nq > 0) {
+        Rows A, B;
+        ld(A, 1);
+        int q = 0;
+        for (; q + 2 <= nq; q += 2, i += 8) {
+            ld(B, i + 4);
+            step(A, i);
+            ld(A, (q + 2 < nq) ? i + 8 : i + 4);
+            step(B, i + 4);
+        }
+        if (q < nq) {
+            step(A, i);
+            i += 4;
+        }
+    }
+    for (; i < N; ++i) {
         const Real mi = Real(1) / ((b[i] - chi[i]) - a[i] * cprev);
         cprev = c[i] * mi;
         m[i] = mi;
@@ -268,6 +299,16 @@ __device__ __forceinline__ Real sum_list(const Real* v, int e0, int e1) {
     }
     for (; e < e1; ++e) acc += v[e];
     return acc;
+}
+
+// entries < key in the index-sorted range idx[lo, hi)
+__device__ __forceinline__ int count_less(const unsigned short* __restrict__ idx, int lo, int hi, int key) {
+    int a = lo, b = hi;
+    while (a < b) {
+        const int mid = (a + b) >> 1;
+        if (static_cast<int>(idx[mid]) < key) a = mid + 1; else b = mid;
+    }
+    return a - lo;
 }
 
 // Opaque register fence: the compiler must assume the values changed here, so nothing derived from them is
@@ -425,8 +466,9 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         auto slot_of = [&](int pp) { return w * (PT * 64) + pp * 64 + lane; };
         // momenta, Delta and weight of cached particle pp (species arrays, L2-resident); the slot is made
         // opaque so the addresses are formed at each use instead of being held in registers
-        auto load_mom = [&](int pp, int ln, Real& PX, Real& PY, Real& D, Real& wi) {
-            const int gi = w * (PT * 64) + pp * 64 + ln;
+        auto load_mom = [&](int pp, Real& PX, Real& PY, Real& D, Real& wi) {
+            int gi = slot_of(pp);
+            asm volatile("" : "+v"(gi));
             const bool s1 = gi >= np0;
             const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
             const int li = s1 ? gi - np0 : gi;
@@ -437,7 +479,6 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         for (int pp = 0; pp < PT; ++pp) {
             const int gi = slot_of(pp);
             cj[pp] = -1;
-            ct[pp] = Real(0);
             if (gi < NP) {
                 const bool s1 = gi >= np0;
                 const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
@@ -449,8 +490,10 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     const Real r = dsqrt(X * X + Y * Y);
                     Real c, s_;
                     if (r > Real(0)) { c = X / r; s_ = Y / r; } else { c = Real(1); s_ = Real(0); }
-                    cj[pp] = 0;
-                    ct[pp] = r;   // the radius until the cell search below
+                    int j; Real t;
+                    grid_weights(s_r, s_ih, N, R, r, j, t);
+                    cj[pp] = j;
+                    ct[pp] = t;
                     cc[pp] = c;
                     cs[pp] = s_;
                     cir[pp] = r > Real(0) ? Real(1) / r : Real(0);
@@ -458,39 +501,6 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                     cgam[pp] = gam;
                     cinvd[pp] = Real(1) / D;
                 }
-            }
-        }
-        // cell and weight of every particle, the binary searches of all cached particles advanced together
-        // (RadialGrid::locate / weights, src/RadialGrid.hpp:35-50: r <= 0 -> cell 0, r >= R -> cell N-1)
-        {
-            int lo[PT], hi[PT];
-#pragma unroll
-            for (int pp = 0; pp < PT; ++pp) {
-                const Real r = ct[pp];
-                lo[pp] = 0;
-                hi[pp] = N;
-                if (r <= Real(0)) hi[pp] = 1;
-                else if (r >= R) lo[pp] = N - 1;
-            }
-            int nbits = 1;
-            while ((1 << nbits) < N) ++nbits;
-            for (int it = 0; it < nbits; ++it) {
-#pragma unroll
-                for (int pp = 0; pp < PT; ++pp) {
-                    if (cj[pp] >= 0 && hi[pp] - lo[pp] > 1) {
-                        const int mid = (lo[pp] + hi[pp]) >> 1;
-                        if (s_r[mid] <= ct[pp]) lo[pp] = mid; else hi[pp] = mid;
-                    }
-                }
-            }
-#pragma unroll
-            for (int pp = 0; pp < PT; ++pp) {
-                if (cj[pp] < 0) continue;
-                const int j = lo[pp];
-                Real t = (ct[pp] - s_r[j]) * s_ih[j];
-                t = t < Real(0) ? Real(0) : (t > Real(1) ? Real(1) : t);
-                cj[pp] = j;
-                ct[pp] = t;
             }
         }
         QZ_PHASE(0);
@@ -555,50 +565,18 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index
-        // order: a particle's slot counts the smaller indices of the neighbour cell (binary searches of all
-        // cached particles advanced together)
-        {
-            int am[PT], bm[PT], ap[PT], bp[PT];
+        // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index order
 #pragma unroll
-            for (int pp = 0; pp < PT; ++pp) {
-                const int j = cj[pp];
-                am[pp] = bm[pp] = ap[pp] = bp[pp] = 0;
-                if (j >= 0) {
-                    am[pp] = s_cs[j]; bm[pp] = s_cs[j + 1];
-                    ap[pp] = s_cs[j + 2]; bp[pp] = s_cs[j + 3];
-                }
-            }
-            int lom[PT], lop[PT];
-#pragma unroll
-            for (int pp = 0; pp < PT; ++pp) { lom[pp] = am[pp]; lop[pp] = ap[pp]; }
-            bool more = true;
-            while (more) {
-                more = false;
-#pragma unroll
-                for (int pp = 0; pp < PT; ++pp) {
-                    const int key = slot_of(pp);
-                    if (am[pp] < bm[pp]) {
-                        const int mid = (am[pp] + bm[pp]) >> 1;
-                        if (static_cast<int>(s_sidx[mid]) < key) am[pp] = mid + 1; else bm[pp] = mid;
-                        more |= am[pp] < bm[pp];
-                    }
-                    if (ap[pp] < bp[pp]) {
-                        const int mid = (ap[pp] + bp[pp]) >> 1;
-                        if (static_cast<int>(s_sidx[mid]) < key) ap[pp] = mid + 1; else bp[pp] = mid;
-                        more |= ap[pp] < bp[pp];
-                    }
-                }
-            }
-#pragma unroll
-            for (int pp = 0; pp < PT; ++pp) {
-                const int j = cj[pp];
-                cpos[pp] = 0;
-                if (j >= 0) {
-                    const int pl = (s_cs[j] + s_cs[j + 1]) + lrank[pp] + (am[pp] - lom[pp]);
-                    const int pr = (s_cs[j + 1] + s_cs[j + 2]) + lrank[pp] + (ap[pp] - lop[pp]);
-                    cpos[pp] = pl | (pr << 16);
-                }
+        for (int pp = 0; pp < PT; ++pp) {
+            const int j = cj[pp];
+            cpos[pp] = 0;
+            if (j >= 0) {
+                const int gi = slot_of(pp);
+                const int less_m = count_less(s_sidx, s_cs[j], s_cs[j + 1], gi);
+                const int less_p = count_less(s_sidx, s_cs[j + 2], s_cs[j + 3], gi);
+                const int pl = (s_cs[j] + s_cs[j + 1]) + lrank[pp] + less_m;
+                const int pr = (s_cs[j + 1] + s_cs[j + 2]) + lrank[pp] + less_p;
+                cpos[pp] = pl | (pr << 16);
             }
         }
         const int G = max(1, min(Bcap / max(2 * L, 1), 8));
@@ -640,17 +618,12 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 constexpr int C = decltype(cst)::value;
                 Real* vb = B + gcur * 2 * L;
                 fence_cache();
-                int ln = lane;
-                asm volatile("" : "+v"(ln));   // addresses formed here, loads issued together
-                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
-#pragma unroll
-                for (int pp = 0; pp < PT; ++pp)
-                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
                 for (int pp = 0; pp < PT; ++pp) {
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    const Real PX = mPX[pp], PY = mPY[pp], D = mD[pp], wi = mW[pp];
+                    Real PX, PY, D, wi;
+                    load_mom(pp, PX, PY, D, wi);
                     const bool s1 = slot_of(pp) >= np0;
                     const Real q = s1 ? q1 : q0, qm = s1 ? qm1 : qm0;
                     const int pl = cpos[pp] & 0xffff, pr = cpos[pp] >> 16;
@@ -865,18 +838,12 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             // gather W+, E_z, B_z (PlasmaSpecies.cpp:75-89, without B+) and form the particle's a_x, a_y
             Real cax[PT], cay[PT];
             fence_cache();
-            {
-                int ln = lane;
-                asm volatile("" : "+v"(ln));
-                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
-#pragma unroll
-                for (int pp = 0; pp < PT; ++pp)
-                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
             for (int pp = 0; pp < PT; ++pp) {
                 const int j = cj[pp];
                 if (j < 0) continue;
-                const Real PX = mPX[pp], PY = mPY[pp];
+                Real PX, PY, D, wi;
+                load_mom(pp, PX, PY, D, wi);
                 const bool s1 = slot_of(pp) >= np0;
                 const Real qm = s1 ? qm1 : qm0;
                 const Real t = ct[pp], c = cc[pp], sn = cs[pp];
@@ -899,7 +866,6 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 const Real dD = qm * ((PX * Wx + PY * Wy) * invd - Ez);
                 cax[pp] = (qm * (gam * Wx + PY * Bz) * invd + fp * c) * invd - PX * dD * invd * invd;
                 cay[pp] = (qm * (gam * Wy - PX * Bz) * invd + fp * sn) * invd - PY * dD * invd * invd;
-            }
             }
             QZ_PHASE(8);
             int gcur = 0, c0 = 0;
@@ -924,17 +890,12 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 Real* vb = VB + gcur * 2 * L;
                 fence_cache();
                 fence_regs(cax); fence_regs(cay);
-                int ln = lane;
-                asm volatile("" : "+v"(ln));
-                Real mPX[PT], mPY[PT], mD[PT], mW[PT];
-#pragma unroll
-                for (int pp = 0; pp < PT; ++pp)
-                    if (cj[pp] >= 0) load_mom(pp, ln, mPX[pp], mPY[pp], mD[pp], mW[pp]);
 #pragma unroll
                 for (int pp = 0; pp < PT; ++pp) {
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    const Real PX = mPX[pp], PY = mPY[pp], wi = mW[pp];
+                    Real PX, PY, D, wi;
+                    load_mom(pp, PX, PY, D, wi);
                     const bool s1 = slot_of(pp) >= np0;
                     const Real qq = s1 ? q1 : q0;
                     const Real t = ct[pp], c = cc[pp], sn = cs[pp];
