@@ -27,6 +27,9 @@ sys.path.insert(0, ROOT)
 METRIC = "plasma particle-slices/s & beam pushes/s (FP64, 1 GPU); batched vs plain"
 FP64_PEAK = 78.6e12
 FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
+# HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
+# same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
+MEASURED_TRAFFIC = {"c2": 753.0e6}
 
 
 def cpu_baseline(deck, nthreads):
@@ -182,7 +185,8 @@ def main():
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel",
+                     "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
+                     "kernel": "sweep_kernel",
                      "flops_per_launch": flops_per_sweep,
                      "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
     }
