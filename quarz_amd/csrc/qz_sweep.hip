@@ -281,16 +281,45 @@ __device__ __forceinline__ void wave_exclusive_scan(int* __restrict__ v, int n, 
     }
 }
 
-// ((0 + v[e0]) + v[e0+1]) + ... + v[e1-1]: the sequential sum of one node list, with the LDS loads issued
-// eight at a time ahead of the dependent adds
+// ((0 + v[e0]) + v[e0+1]) + ... + v[e1-1]: the sequential sum of one node list.  Lists are long where the
+// plasma is compressed (hundreds of entries in a blowout sheath), so the loads run one batch of eight ahead of
+// the dependent adds: a long list costs about one add latency per entry.
+struct Batch8 {
+    Real a0, a1, a2, a3, a4, a5, a6, a7;
+    __device__ __forceinline__ void load(const Real* v) {
+        a0 = v[0]; a1 = v[1]; a2 = v[2]; a3 = v[3]; a4 = v[4]; a5 = v[5]; a6 = v[6]; a7 = v[7];
+    }
+    __device__ __forceinline__ void add_to(Real& acc) const {
+        acc += a0; acc += a1; acc += a2; acc += a3; acc += a4; acc += a5; acc += a6; acc += a7;
+    }
+};
+
 __device__ __forceinline__ Real sum_list(const Real* v, int e0, int e1) {
     Real acc = Real(0);
     int e = e0;
-    for (; e + 8 <= e1; e += 8) {
-        const Real x0 = v[e], x1 = v[e + 1], x2 = v[e + 2], x3 = v[e + 3];
-        const Real x4 = v[e + 4], x5 = v[e + 5], x6 = v[e + 6], x7 = v[e + 7];
-        acc += x0; acc += x1; acc += x2; acc += x3;
-        acc += x4; acc += x5; acc += x6; acc += x7;
+    // 24 entries per iteration, all loads issued before the adds: the adds of the first batch cover the
+    // latency of the other two, leaving one exposed LDS round trip per 24 entries
+    for (; e + 24 <= e1; e += 24) {
+        Batch8 x, y, z;
+        x.load(v + e);
+        y.load(v + e + 8);
+        z.load(v + e + 16);
+        x.add_to(acc);
+        y.add_to(acc);
+        z.add_to(acc);
+    }
+    if (e + 16 <= e1) {
+        Batch8 x, y;
+        x.load(v + e);
+        y.load(v + e + 8);
+        x.add_to(acc);
+        y.add_to(acc);
+        e += 16;
+    } else if (e + 8 <= e1) {
+        Batch8 x;
+        x.load(v + e);
+        x.add_to(acc);
+        e += 8;
     }
     if (e + 4 <= e1) {
         const Real x0 = v[e], x1 = v[e + 1], x2 = v[e + 2], x3 = v[e + 3];
@@ -386,6 +415,15 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
     const int nsp = A.nspecies;
     const int np0 = nsp > 0 ? A.sp[0].np : 0, np1 = nsp > 1 ? A.sp[1].np : 0;
     const int NP = np0 + np1;
+    // species arrays read once: a particle's array is a per-lane choice between two uniform pointers
+    const Real* const px0 = nsp > 0 ? A.sp[0].px : nullptr;
+    const Real* const py0 = nsp > 0 ? A.sp[0].py : nullptr;
+    const Real* const dl0 = nsp > 0 ? A.sp[0].dl : nullptr;
+    const Real* const w0 = nsp > 0 ? A.sp[0].w : nullptr;
+    const Real* const px1 = nsp > 1 ? A.sp[1].px : px0;
+    const Real* const py1 = nsp > 1 ? A.sp[1].py : py0;
+    const Real* const dl1 = nsp > 1 ? A.sp[1].dl : dl0;
+    const Real* const w1 = nsp > 1 ? A.sp[1].w : w0;
     const Real q0 = nsp > 0 ? A.sp[0].q : Real(0), q1 = nsp > 1 ? A.sp[1].q : Real(0);
     const Real qm0 = nsp > 0 ? A.sp[0].q / A.sp[0].m : Real(0), qm1 = nsp > 1 ? A.sp[1].q / A.sp[1].m : Real(0);
     const bool has_neg = (nsp > 0 && q0 < 0) || (nsp > 1 && q1 < 0);
@@ -470,9 +508,17 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             int gi = slot_of(pp);
             asm volatile("" : "+v"(gi));
             const bool s1 = gi >= np0;
-            const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
             const int li = s1 ? gi - np0 : gi;
-            PX = st.px[li]; PY = st.py[li]; D = st.dl[li]; wi = st.w[li];
+            PX = (s1 ? px1 : px0)[li]; PY = (s1 ? py1 : py0)[li]; D = (s1 ? dl1 : dl0)[li]; wi = (s1 ? w1 : w0)[li];
+        };
+        // the same for PB consecutive cached particles from p0, all loads issued before any is used
+        constexpr int PB = PT < 4 ? PT : 4;
+        auto load_batch = [&](int p0, Real (&PX)[PB], Real (&PY)[PB], Real (&D)[PB], Real (&wi)[PB]) {
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                PX[u] = PY[u] = D[u] = wi[u] = Real(0);
+                if (cj[p0 + u] >= 0) load_mom(p0 + u, PX[u], PY[u], D[u], wi[u]);
+            }
         };
         // ================================================================ 1. geometry (PlasmaSpecies.cpp:52-70)
 #pragma unroll
@@ -619,11 +665,15 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 Real* vb = B + gcur * 2 * L;
                 fence_cache();
 #pragma unroll
-                for (int pp = 0; pp < PT; ++pp) {
+                for (int p0 = 0; p0 < PT; p0 += PB) {
+                Real PXb[PB], PYb[PB], Db[PB], wb[PB];
+                load_batch(p0, PXb, PYb, Db, wb);
+#pragma unroll
+                for (int u = 0; u < PB; ++u) {
+                    const int pp = p0 + u;
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    Real PX, PY, D, wi;
-                    load_mom(pp, PX, PY, D, wi);
+                    const Real PX = PXb[u], PY = PYb[u], D = Db[u], wi = wb[u];
                     const bool s1 = slot_of(pp) >= np0;
                     const Real q = s1 ? q1 : q0, qm = s1 ? qm1 : qm0;
                     const int pl = cpos[pp] & 0xffff, pr = cpos[pp] >> 16;
@@ -633,6 +683,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                                        qm, s_iv[j], ih);
                     vb[pr] = dep_value(C, 1, j, ct[pp], cc[pp], cs[pp], cir[pp], cinvd[pp], cgam[pp], PX, PY, pz, wi, q,
                                        qm, s_iv[j + 1], ih);
+                }
                 }
                 QZ_PHASE(2);
                 if (++gcur == G) reduce();
@@ -839,11 +890,15 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             Real cax[PT], cay[PT];
             fence_cache();
 #pragma unroll
-            for (int pp = 0; pp < PT; ++pp) {
+            for (int p0 = 0; p0 < PT; p0 += PB) {
+            Real PXb[PB], PYb[PB], Db[PB], wb[PB];
+            load_batch(p0, PXb, PYb, Db, wb);
+#pragma unroll
+            for (int u = 0; u < PB; ++u) {
+                const int pp = p0 + u;
                 const int j = cj[pp];
                 if (j < 0) continue;
-                Real PX, PY, D, wi;
-                load_mom(pp, PX, PY, D, wi);
+                const Real PX = PXb[u], PY = PYb[u];
                 const bool s1 = slot_of(pp) >= np0;
                 const Real qm = s1 ? qm1 : qm0;
                 const Real t = ct[pp], c = cc[pp], sn = cs[pp];
@@ -866,6 +921,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 const Real dD = qm * ((PX * Wx + PY * Wy) * invd - Ez);
                 cax[pp] = (qm * (gam * Wx + PY * Bz) * invd + fp * c) * invd - PX * dD * invd * invd;
                 cay[pp] = (qm * (gam * Wy - PX * Bz) * invd + fp * sn) * invd - PY * dD * invd * invd;
+            }
             }
             QZ_PHASE(8);
             int gcur = 0, c0 = 0;
@@ -891,11 +947,15 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 fence_cache();
                 fence_regs(cax); fence_regs(cay);
 #pragma unroll
-                for (int pp = 0; pp < PT; ++pp) {
+                for (int p0 = 0; p0 < PT; p0 += PB) {
+                Real PXb[PB], PYb[PB], Db[PB], wb[PB];
+                load_batch(p0, PXb, PYb, Db, wb);
+#pragma unroll
+                for (int u = 0; u < PB; ++u) {
+                    const int pp = p0 + u;
                     const int j = cj[pp];
                     if (j < 0) continue;
-                    Real PX, PY, D, wi;
-                    load_mom(pp, PX, PY, D, wi);
+                    const Real PX = PXb[u], PY = PYb[u], wi = wb[u];
                     const bool s1 = slot_of(pp) >= np0;
                     const Real qq = s1 ? q1 : q0;
                     const Real t = ct[pp], c = cc[pp], sn = cs[pp];
@@ -924,6 +984,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                         const Real v = (SC & 1) == 0 ? iv * (Ar * ck + Ai * sk) : iv * (Ai * ck - Ar * sk);
                         vb[side == 0 ? pl : pr] = v;
                     }
+                }
                 }
                 QZ_PHASE(9);
                 if (++gcur == GS) reduce();
