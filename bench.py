@@ -10,6 +10,10 @@ device synchronisations.  Prints one JSON line:
   roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops
                  679 per particle-slice + 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP
                  events on the simulation's stream
+  beam_roofline  the beam deposit and push against MI355X HBM bandwidth (8 TB/s), algorithmic bytes per step:
+                 deposit 56 B read per live beam particle + bsrc (K M NC_B doubles) zeroed and written once;
+                 push 56 B read + 48 B written per particle + fld (K M NC_F doubles) read once; times from HIP
+                 events around the deposit (bsrc reset, live count, deposit) and the push launches
   cpu_baseline   the oracle's OpenMP build of the same step (the reference's loop structure) timed on this
                  host's cores on one full sweep of the same deck
 The inputs (beam particles, grid, operators) are resident in HBM before the timed region; the plasma is
@@ -26,6 +30,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "plasma particle-slices/s & beam pushes/s (FP64, 1 GPU); batched vs plain"
 FP64_PEAK = 78.6e12
+HBM_PEAK = 8.0e12
 FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
 # same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
@@ -146,7 +151,7 @@ def main():
     for _ in range(args.warmup):
         sim.step()
     torch.cuda.synchronize()
-    psl = pushes = 0.0
+    psl = pushes = dep_particles = 0.0
     sweep_ms = beam_ms = dep_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -154,6 +159,7 @@ def main():
         st = sim.stats()
         psl += st["particle_slices"]
         pushes += st["beam_pushed"]
+        dep_particles += st["beam_deposited"]
         sweep_ms += st["sweep_ms"]
         beam_ms += st["beam_push_ms"]
         dep_ms += st["beam_dep_ms"]
@@ -190,6 +196,21 @@ def main():
                      "flops_per_launch": flops_per_sweep,
                      "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
     }
+    nb_dep, nb_push = dep_particles / args.steps, pushes / args.steps
+    ncf, ncb = (18, 15) if modes else (6, 5)
+    kmn = 8.0 * sim.K * sim.M
+    parts = {"deposit": (56.0 * nb_dep + 2.0 * kmn * ncb, dep_ms), "push": (104.0 * nb_push + kmn * ncf, beam_ms)}
+    beam = {"bound": "hbm", "peak": HBM_PEAK / 1e9, "unit": "GB/s"}
+    tot_b = tot_t = 0.0
+    for name, (b, ms) in parts.items():
+        t = ms / args.steps / 1e3
+        tot_b += b
+        tot_t += t
+        beam[name] = {"achieved": b / t / 1e9 if t > 0 else None, "frac": b / t / HBM_PEAK if t > 0 else None,
+                      "bytes_per_step": b}
+    beam["achieved"] = tot_b / tot_t / 1e9 if tot_t > 0 else None
+    beam["frac"] = tot_b / tot_t / HBM_PEAK if tot_t > 0 else None
+    out["beam_roofline"] = beam
     if not args.no_cpu:
         ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(deck, min(16, ncpu))

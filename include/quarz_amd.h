@@ -176,6 +176,19 @@ long qz_sim_lost(const qz_sim* s);
 double qz_sim_time(const qz_sim* s);
 int qz_sim_write_fields(qz_sim* s, const char* path);
 
+/* ---- a fixed set of runs stepped together (not in the reference: it batches Simulation::run) ----
+ * Every step runs all sweeps in one persistent launch (as qz_sim_step_batched).  With use_graph, once a step
+ * repeats the device work of the step before in every run (every beam started, the same dt, a beam push to
+ * do, no field output), the whole step -- each run's beam deposit and plasma reload, the batched sweep, each
+ * run's beam push -- is captured once as a HIP graph and replayed on later steps.  step_ms receives the
+ * time of the batched sweep (eager steps) or of the whole replayed step.  The runs must stay alive and must
+ * not be stepped by other calls while the batch exists. */
+typedef struct qz_batch qz_batch;
+int qz_batch_create(qz_sim** sims, int n, int use_graph, qz_batch** out);
+int qz_batch_step(qz_batch* b, double* step_ms);
+long qz_batch_replays(const qz_batch* b);   /* steps run as graph replays so far */
+void qz_batch_destroy(qz_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,12 @@ def lib():
         L.qz_sim_step.argtypes = [C.c_void_p]
         L.qz_sim_step_fields.argtypes = [C.c_void_p]
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
+        L.qz_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.qz_batch_step.argtypes = [C.c_void_p, DP]
+        L.qz_batch_replays.argtypes = [C.c_void_p]
+        L.qz_batch_replays.restype = C.c_long
+        L.qz_batch_destroy.argtypes = [C.c_void_p]
+        L.qz_batch_destroy.restype = None
         L.qz_sim_push_beams.argtypes = [C.c_void_p]
         L.qz_sim_get.restype = C.c_long
         L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
@@ -212,6 +218,40 @@ def step_batched(sims):
     ms = C.c_double(0)
     check(lib().qz_sim_step_batched(arr, len(sims), C.byref(ms)))
     return ms.value
+
+
+class Batch:
+    """A fixed set of Simulations (same modes) stepped together: all sweeps in one persistent launch, and with
+    graph=True the steady-state step (beam deposits, plasma reloads, the batched sweep, beam pushes) replayed
+    as one captured HIP graph.  The Simulations must outlive the Batch and must not be stepped otherwise."""
+
+    def __init__(self, sims, graph=True):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
+        self.h = C.c_void_p()
+        check(lib().qz_batch_create(arr, len(self.sims), 1 if graph else 0, C.byref(self.h)))
+
+    def step(self):
+        """One step of every run; returns the batched sweep time (eager steps) or the whole step's time
+        (graph replays) in ms."""
+        ms = C.c_double(0)
+        check(lib().qz_batch_step(self.h, C.byref(ms)))
+        return ms.value
+
+    @property
+    def replays(self):
+        return lib().qz_batch_replays(self.h)
+
+    def close(self):
+        if self.h and self.h.value:
+            lib().qz_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------- kernel-level entry points

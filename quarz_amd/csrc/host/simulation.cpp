@@ -281,6 +281,17 @@ void PlasmaSpecies::init(const Config& cfg, const std::string& nm, const RadialG
 // rings at the sub-ring volume centroids of every cell (src/PlasmaSpecies.cpp:202-295)
 void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st) {
     (void)z;
+    if (have_template) {   // the rings of the first load (device copies: capturable, no host work)
+        const size_t bytes = sizeof(Real) * static_cast<size_t>(Np);
+        if (Np) {
+            const DevBuf<Real>* src[6] = {&t_x, &t_y, &t_px, &t_py, &t_dl, &t_w};
+            DevBuf<Real>* dst[6] = {&x, &y, &px, &py, &dl, &w};
+            for (int c = 0; c < 6; ++c)
+                hip_check(hipMemcpyAsync(dst[c]->p, src[c]->p, bytes, hipMemcpyDeviceToDevice, st), "plasma copy");
+        }
+        qz_check(qz_zero_async(hist.p, sizeof(Real) * hist.n, st));
+        return;
+    }
     std::mt19937_64 rng(seed);
     std::normal_distribution<double> gauss(0.0, 1.0);
     const double fz = 1.0 * density;
@@ -309,7 +320,11 @@ void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, h
     }
     Np = i;
     x.upload(hx, st); y.upload(hy, st); px.upload(hpx, st); py.upload(hpy, st); dl.upload(hdl, st); w.upload(hw, st);
-    hip_check(hipMemsetAsync(hist.p, 0, sizeof(Real) * hist.n, st), "hist reset");
+    t_x.alloc(cap); t_y.alloc(cap); t_px.alloc(cap); t_py.alloc(cap); t_dl.alloc(cap); t_w.alloc(cap);
+    t_x.upload(hx, st); t_y.upload(hy, st); t_px.upload(hpx, st); t_py.upload(hpy, st); t_dl.upload(hdl, st);
+    t_w.upload(hw, st);
+    have_template = true;
+    qz_check(qz_zero_async(hist.p, sizeof(Real) * hist.n, st));
 }
 
 // host deposit of the loaded rings (background / neutralization), serial index order
@@ -643,35 +658,38 @@ void Simulation::step_fields() {
 
 // everything of step_fields before the slice loop (src/Simulation.cpp:636-726): beam sources, plasma load,
 // background; fills sweep_args
-void Simulation::prepare_sweep() {
+void Simulation::prepare_sweep(bool timed) {
     // ---- beams -> (xi, r) arrays
-    hip_check(hipEventRecord(ev[4], stream), "event");
-    hip_check(hipMemsetAsync(bsrc.p, 0, sizeof(Real) * bsrc.n, stream), "bsrc reset");
+    if (timed) hip_check(hipEventRecord(ev[4], stream), "event");
+    qz_check(qz_zero_async(bsrc.p, sizeof(Real) * bsrc.n, stream));
     const qz_radial_grid g = grid.view();
-    hip_check(hipMemsetAsync(counters.p, 0, sizeof(unsigned long long) * counters.n, stream), "counter reset");
+    qz_check(qz_zero_async(counters.p, sizeof(unsigned long long) * counters.n, stream));
     for (const auto& b : beams) {
         const qz_beam_soa s = b->soa();
         qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 20, stream));
         qz_check(qz_beam_deposit(&g, &box, m1 ? 1 : 0, b->q, b->Np, &s, bsrc.p, NCB, dep_work.p, stream));
     }
-    hip_check(hipEventRecord(ev[5], stream), "event");
+    if (timed) hip_check(hipEventRecord(ev[5], stream), "event");
     // ---- plasma at the head of the box, background
-    std::vector<Real> bgt(3 * static_cast<size_t>(M), 0.0), bgr(3 * static_cast<size_t>(M), 0.0);
     const Real z_front = t - box.xi_min;
     unsigned long long sd = seed;
     for (auto& s : species) s->load(grid, z_front, sd++, stream);
-    for (auto& s : species)
-        if (!s->mobile) s->deposit_rho(grid, bgt, bgr);
-    if (neutralize) {
-        std::vector<Real> a(3 * static_cast<size_t>(M), 0.0), b(3 * static_cast<size_t>(M), 0.0);
+    if (!bg_ready) {   // the loaded rings are the same every step, and so is the background
+        std::vector<Real> bgt(3 * static_cast<size_t>(M), 0.0), bgr(3 * static_cast<size_t>(M), 0.0);
         for (auto& s : species)
-            if (s->mobile) s->deposit_rho(grid, a, b);
-        for (size_t j = 0; j < a.size(); ++j) { bgt[j] -= a[j]; bgr[j] -= b[j]; }
+            if (!s->mobile) s->deposit_rho(grid, bgt, bgr);
+        if (neutralize) {
+            std::vector<Real> a(3 * static_cast<size_t>(M), 0.0), b(3 * static_cast<size_t>(M), 0.0);
+            for (auto& s : species)
+                if (s->mobile) s->deposit_rho(grid, a, b);
+            for (size_t j = 0; j < a.size(); ++j) { bgt[j] -= a[j]; bgr[j] -= b[j]; }
+        }
+        bg_host = bgt;   // a member: the asynchronous upload reads it after this function returns
+        bg_host.insert(bg_host.end(), bgr.begin(), bgr.end());
+        bg.upload(bg_host, stream);
+        bg_ready = true;
     }
-    bg_host = bgt;   // a member: the asynchronous upload reads it after this function returns
-    bg_host.insert(bg_host.end(), bgr.begin(), bgr.end());
-    bg.upload(bg_host, stream);
-    hip_check(hipMemsetAsync(lost.p, 0, sizeof(int) * lost.n, stream), "lost reset");
+    qz_check(qz_zero_async(lost.p, sizeof(int) * lost.n, stream));
     // ---- the sweep
     qz_sweep_args A{};
     A.grid = g;
@@ -712,14 +730,15 @@ void Simulation::prepare_sweep() {
 }
 
 // after the slice loop: removed-particle count, work counters, timings (ms_sweep measured by the caller)
-void Simulation::finish_sweep(float sweep_ms) {
+void Simulation::finish_sweep(float sweep_ms, bool timed) {
     const int nsp = sweep_args.nspecies;
     std::vector<int> hl;
     lost.download(hl, stream);
     lost_last = 0;
     for (int i = 0; i < nsp; ++i) lost_last += hl[i];
     ms_sweep = sweep_ms;
-    hip_check(hipEventElapsedTime(&ms_beam_dep, ev[4], ev[5]), "elapsed");
+    ms_beam_dep = 0;
+    if (timed) hip_check(hipEventElapsedTime(&ms_beam_dep, ev[4], ev[5]), "elapsed");
     std::vector<unsigned long long> cnt;
     counters.download(cnt, stream);
     particle_slices = static_cast<double>(cnt[0]);
@@ -734,12 +753,27 @@ void Simulation::push_beams() {
     ms_beam = 0;
     if (last) return;
     if (!have_prev) { dt_prev = dt; have_prev = true; }
+    enqueue_push(true);
+    hip_check(hipEventRecord(ev[3], stream), "event");
+    hip_check(hipEventSynchronize(ev[3]), "sync");
+    hip_check(hipEventElapsedTime(&ms_beam, ev[2], ev[3]), "elapsed");
+    {
+        unsigned long long v = 0;
+        hip_check(hipMemcpy(&v, counters.p + 21, sizeof(v), hipMemcpyDeviceToHost), "counter");
+        beam_pushed = static_cast<double>(v);
+    }
+    t += dt;
+    dt_prev = dt;
+}
+
+// device work of one beam push: live count, then the leapfrog start-up (first push only) and the kick-drift
+void Simulation::enqueue_push(bool timed) {
     const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
     const qz_radial_grid g = grid.view();
-    hip_check(hipMemsetAsync(counters.p + 21, 0, sizeof(unsigned long long), stream), "counter reset");
+    qz_check(qz_zero_async(counters.p + 21, sizeof(unsigned long long), stream));
     for (auto& b : beams)
         if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, counters.p + 21, stream));
-    hip_check(hipEventRecord(ev[2], stream), "event");
+    if (timed) hip_check(hipEventRecord(ev[2], stream), "event");
     for (auto& b : beams) {
         if (b->rigid) continue;
         const qz_beam_soa s = b->soa();
@@ -751,16 +785,18 @@ void Simulation::push_beams() {
         }
         qz_check(qz_beam_advance(&g, &box, m1 ? 1 : 0, qm, b->pusher, 0, b->Np, &s, fld.p, NCF, dkick, ddrift, 1, stream));
     }
-    hip_check(hipEventRecord(ev[3], stream), "event");
-    hip_check(hipEventSynchronize(ev[3]), "sync");
-    hip_check(hipEventElapsedTime(&ms_beam, ev[2], ev[3]), "elapsed");
-    {
-        unsigned long long v = 0;
-        hip_check(hipMemcpy(&v, counters.p + 21, sizeof(v), hipMemcpyDeviceToHost), "counter");
-        beam_pushed = static_cast<double>(v);
-    }
-    t += dt;
-    dt_prev = dt;
+}
+
+// the device work of the next step equals that of the step before: background and plasma template made,
+// every beam started (no leapfrog start-up), the same dt, a beam push to do and no field output
+bool Simulation::capturable() const {
+    if (done || !bg_ready || !have_prev || dt_prev != dt || step_n >= nsteps) return false;
+    if (!outdir.empty() && out_every > 0 && step_n % out_every == 0) return false;
+    for (const auto& s : species)
+        if (s->mobile && !s->have_template) return false;
+    for (const auto& b : beams)
+        if (!b->rigid && !b->started) return false;
+    return true;
 }
 
 void Simulation::step() {
@@ -938,6 +974,123 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
         }
     })
 }
+
+// A fixed set of runs stepped together.  Every step runs all sweeps in one persistent launch; once a step
+// repeats the device work of the one before in every run (Simulation::capturable), the whole step (each
+// run's beam deposit and plasma reload, the batched sweep, each run's beam push) is captured once as a HIP
+// graph on the batch's stream and replayed.  Not in the reference.
+struct qz_batch {
+    std::vector<qz_sim*> sims;
+    bool use_graph = true;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    qz_sweep_args* dargs = nullptr;   // device copy of the runs' sweep arguments, read by the graph's launch
+    std::vector<qz_sweep_args> hargs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long replays = 0;
+    ~qz_batch() {
+        if (st) (void)hipStreamSynchronize(st);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (dargs) (void)hipFree(dargs);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out) {
+    if (!hs || n <= 0 || !out) return qz_fail(QZ_EINVAL, "qz_batch_create: no runs");
+    QZ_SIM_TRY({
+        std::unique_ptr<qz_batch> b(new qz_batch());
+        for (int i = 0; i < n; ++i) {
+            if (!hs[i]) throw std::runtime_error("qz_batch_create: null run");
+            if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_batch_create: runs differ in modes");
+            b->sims.push_back(hs[i]);
+        }
+        b->use_graph = use_graph != 0;
+        qz::hip_check(hipStreamCreate(&b->st), "stream");
+        qz::hip_check(hipEventCreate(&b->e0), "event");
+        qz::hip_check(hipEventCreate(&b->e1), "event");
+        *out = b.release();
+    })
+}
+
+extern "C" void qz_batch_destroy(qz_batch* b) { delete b; }
+
+// capture of one step of every run on the batch stream (each run's stream is switched to it meanwhile)
+static void capture_batch(qz_batch* b) {
+    const int n = static_cast<int>(b->sims.size());
+    b->hargs.resize(n);
+    for (int i = 0; i < n; ++i) b->hargs[i] = b->sims[i]->s->sweep_args;   // the same every step
+    qz::hip_check(hipMalloc(reinterpret_cast<void**>(&b->dargs), sizeof(qz_sweep_args) * n), "batch arguments");
+    qz::hip_check(hipMemcpy(b->dargs, b->hargs.data(), sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice), "batch arguments");
+    std::vector<hipStream_t> saved(n);
+    for (int i = 0; i < n; ++i) saved[i] = b->sims[i]->s->stream;
+    qz::hip_check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture");
+    try {
+        for (int i = 0; i < n; ++i) {
+            b->sims[i]->s->stream = b->st;
+            b->sims[i]->s->prepare_sweep(false);
+            b->sims[i]->s->stream = saved[i];
+        }
+        if (qz_sweep_launch(b->dargs, b->hargs.data(), n, b->st) != QZ_OK) throw std::runtime_error(qz_last_error());
+        for (int i = 0; i < n; ++i) {
+            b->sims[i]->s->stream = b->st;
+            b->sims[i]->s->enqueue_push(false);
+            b->sims[i]->s->stream = saved[i];
+        }
+    } catch (...) {
+        for (int i = 0; i < n; ++i) b->sims[i]->s->stream = saved[i];
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(b->st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    qz::hip_check(hipStreamEndCapture(b->st, &b->graph), "end capture");
+    qz::hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+}
+
+extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
+    if (!b) return qz_fail(QZ_EINVAL, "qz_batch_step: null batch");
+    QZ_SIM_TRY({
+        const int n = static_cast<int>(b->sims.size());
+        bool replay = b->use_graph;
+        for (int i = 0; i < n; ++i) {
+            const qz::Simulation& s = *b->sims[i]->s;
+            if (s.done) throw std::runtime_error("qz_batch_step: a run has ended (time.steps reached)");
+            replay = replay && s.capturable();
+        }
+        if (!replay) {   // start-up, last or output steps: the eager batched step
+            const int rc = qz_sim_step_batched(b->sims.data(), n, step_ms);
+            if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
+            return QZ_OK;
+        }
+        if (!b->exec) capture_batch(b);
+        qz::hip_check(hipEventRecord(b->e0, b->st), "event");
+        qz::hip_check(hipGraphLaunch(b->exec, b->st), "graph launch");
+        qz::hip_check(hipEventRecord(b->e1, b->st), "event");
+        qz::hip_check(hipEventSynchronize(b->e1), "sync");
+        float ms = 0;
+        qz::hip_check(hipEventElapsedTime(&ms, b->e0, b->e1), "elapsed");
+        ++b->replays;
+        if (step_ms) *step_ms = ms;
+        for (int i = 0; i < n; ++i) {   // the host side of the step (Simulation::finish_sweep, after_sweep)
+            qz::Simulation& s = *b->sims[i]->s;
+            s.finish_sweep(ms, false);
+            std::vector<unsigned long long> cnt;
+            s.counters.download(cnt, s.stream);
+            s.beam_pushed = static_cast<double>(cnt[21]);
+            s.ms_beam = 0;
+            s.t += s.dt;
+            s.dt_prev = s.dt;
+            ++s.step_n;
+        }
+    })
+}
+extern "C" long qz_batch_replays(const qz_batch* b) { return b ? b->replays : -1; }
+
 extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
 extern "C" int qz_sim_push_beams(qz_sim* h) {
     QZ_SIM_TRY({

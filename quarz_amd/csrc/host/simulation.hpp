@@ -78,6 +78,10 @@ struct PlasmaSpecies {
     int Np = 0, cap = 0;
     std::vector<Real> hx, hy, hpx, hpy, hdl, hw;   // host copy of the last load
     DevBuf<Real> x, y, px, py, dl, w, hist;
+    // The load does not depend on the step (uniform profile, the same seed every step): the first load is
+    // kept in HBM and later loads are device copies of it.
+    DevBuf<Real> t_x, t_y, t_px, t_py, t_dl, t_w;
+    bool have_template = false;
 
     void init(const Config& cfg, const std::string& nm, const RadialGrid& g, bool mode1);
     void load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st);
@@ -119,9 +123,13 @@ public:
     ~Simulation();
 
     void step_fields();      // the quasi-static sweep of the current step n
-    void prepare_sweep();    // step_fields before the slice loop; fills sweep_args
-    void finish_sweep(float sweep_ms);   // step_fields after the slice loop
+    // step_fields before the slice loop; fills sweep_args.  With timed = false it records no events: all its
+    // work is then device work on `stream` (capturable) once the background and the plasma template exist.
+    void prepare_sweep(bool timed = true);
+    void finish_sweep(float sweep_ms, bool timed = true);   // step_fields after the slice loop
     void push_beams();       // the beam push of step n (not on the last step)
+    void enqueue_push(bool timed);       // the device work of push_beams (live count, kick and drift launches)
+    bool capturable() const;             // the next step's device work is the same as the last one's
     void after_sweep();      // output, beam push and step counter after the sweep
     void step();             // one iteration of Simulation::run
     void write_fields(const std::string& path);
@@ -146,6 +154,7 @@ public:
     DevBuf<char> dep_work;
     qz_sweep_args sweep_args{};
     std::vector<Real> bg_host;   // host side of the background upload (kept alive for the async copy)
+    bool bg_ready = false;       // the background is the same every step: computed and uploaded once
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;

@@ -8,6 +8,7 @@
 // adds their contributions sequentially, in particle-index order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "qz_device.hpp"
@@ -92,12 +93,27 @@ __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, R
 }
 
 // live particles (w != 0) of a beam, added to *count
-__global__ void beam_count_live_kernel(int np, const Real* __restrict__ w, unsigned long long* __restrict__ count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long live = (i < np && w[i] != Real(0)) ? 1ull : 0ull;
-    unsigned long long s = live;
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(count, s);
+// zero fill: 16-byte stores over n16 chunks, then 4-byte stores over the ntail words after them
+__global__ void __launch_bounds__(256) zero_kernel(uint4* __restrict__ p, size_t n16, unsigned int* __restrict__ tail,
+                                                   int ntail) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; i < n16; i += stride) p[i] = make_uint4(0, 0, 0, 0);
+    if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < ntail) tail[threadIdx.x] = 0u;
+}
+
+// live particles (w != 0): a grid-stride count per thread, one atomic per block of 256 threads
+__global__ void __launch_bounds__(256) beam_count_live_kernel(int np, const Real* __restrict__ w,
+                                                              unsigned long long* __restrict__ count) {
+    __shared__ unsigned int part[4];
+    unsigned int c = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < np; i += gridDim.x * 256) c += w[i] != Real(0) ? 1u : 0u;
+    for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long s = static_cast<unsigned long long>(part[0]) + part[1] + part[2] + part[3];
+        if (s) atomicAdd(count, s);
+    }
 }
 
 // deposit step 1: cell key, per-particle record and bucket counts
@@ -300,10 +316,31 @@ DepWork dep_layout(int np, int nloc, int N) {
 }
 }  // namespace
 
+int qz_zero_async(void* p, size_t bytes, hipStream_t st) {
+    if (bytes == 0) return QZ_OK;
+    if (!p || bytes % 4 != 0) return qz_fail(QZ_EINVAL, "zero fill: null pointer or size not a multiple of 4");
+    char* c = static_cast<char*>(p);
+    size_t head = (16 - reinterpret_cast<uintptr_t>(c) % 16) % 16;   // bytes before the first 16-byte boundary
+    if (head > bytes) head = bytes;
+    if (head) {   // at most 3 words: a separate tiny launch keeps the main kernel aligned
+        hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, st, static_cast<uint4*>(nullptr), size_t(0),
+                           reinterpret_cast<unsigned int*>(c), static_cast<int>(head / 4));
+        QZ_HIP(hipGetLastError());
+    }
+    const size_t body = bytes - head;
+    const size_t n16 = body / 16;
+    const int ntail = static_cast<int>((body % 16) / 4);
+    const size_t nb = std::min<size_t>(std::max<size_t>((n16 + 255) / 256, 1), 2048);
+    hipLaunchKernelGGL(zero_kernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, st, reinterpret_cast<uint4*>(c + head),
+                       n16, reinterpret_cast<unsigned int*>(c + head + 16 * n16), ntail);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st) {
     if (np <= 0) return QZ_OK;
-    const int tb = 256;
-    hipLaunchKernelGGL(beam_count_live_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, w, count);
+    const int nb = std::min((np + 255) / 256, 512);
+    hipLaunchKernelGGL(beam_count_live_kernel, dim3(nb), dim3(256), 0, st, np, w, count);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
 }
@@ -330,8 +367,12 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
     int* bsum = reinterpret_cast<int*>(w + L.bsum);
     int* total = reinterpret_cast<int*>(w + L.total);
     const int ncells = (box->nloc + 1) * grid->N;
-    QZ_HIP(hipMemsetAsync(counts, 0, 4 * (static_cast<size_t>(ncells) + 1), st));
-    QZ_HIP(hipMemsetAsync(fill, 0, 4 * (static_cast<size_t>(ncells) + 1), st));
+    {
+        const int rc = qz_zero_async(counts, 4 * (static_cast<size_t>(ncells) + 1), st);
+        if (rc) return rc;
+        const int rc2 = qz_zero_async(fill, 4 * (static_cast<size_t>(ncells) + 1), st);
+        if (rc2) return rc2;
+    }
     const int tb = 256;
     hipLaunchKernelGGL(beam_key_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q, np, *p, key, rec, counts);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total

@@ -236,6 +236,52 @@ def test_batched_m1_equals_plain(qa, decks):
     _batched_vs_plain(qa, f"{decks}/linear.in", ov, 2)
 
 
+def _graph_vs_plain(qa, deck, overrides, steps, replays):
+    """A Batch with graph replays against the same runs stepped plainly, compared bit for bit after every
+    step: the eager start-up step, the step that captures the graph, each replay (with the host work of the
+    step and the comparison's downloads between replays) and the eager last step."""
+    plain = [qa.Simulation(deck, ov) for ov in overrides]
+    runs = [qa.Simulation(deck, ov) for ov in overrides]
+    batch = qa.Batch(runs, graph=True)
+    try:
+        for st in range(steps):
+            for s in plain:
+                s.step()
+            batch.step()
+            for i, (a, b) in enumerate(zip(runs, plain)):
+                for w in ("fld", "diag", "bsrc"):
+                    ga, gb = getattr(a, w), getattr(b, w)
+                    assert _same(ga, gb), _report(f"step {st} run {i} {w}", ga, gb)
+                for bi in range(a.nbeams):
+                    for k in qa.BEAM_ARRAYS:
+                        ga, gb = a.get(f"beam{bi}.{k}"), b.get(f"beam{bi}.{k}")
+                        assert _same(ga, gb), _report(f"step {st} run {i} beam{bi}.{k}", ga, gb)
+                assert a.lost() == b.lost()
+                assert a.t == b.t
+        assert batch.replays == replays
+    finally:
+        batch.close()
+
+
+def test_graph_batch_m0_equals_plain(qa, decks):
+    """time.steps = 5: step 0 starts the leapfrog (eager), steps 1-4 are graph replays (the first of them
+    captures), step 5 is the last sweep without a push (eager)."""
+    ov = [{"time.steps": 5, "driver.nparticles": 20000, "xi.min": -8},
+          {"time.steps": 5, "driver.nparticles": 30000, "xi.min": -6, "grid.type": "uniform", "grid.dr": 0.02,
+           "electrons.ppc": 2},
+          {"time.steps": 5, "driver.nparticles": 10000, "xi.min": -9, "grid.type": "uniform", "grid.dr": 0.08,
+           "electrons.ppc": 8}]
+    _graph_vs_plain(qa, f"{decks}/gain.in", ov, 6, 4)
+
+
+def test_graph_batch_m1_equals_plain(qa, decks):
+    ov = [{"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
+           "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 4, "time.dt": 5},
+          {"modes": 1, "xi.min": -4, "grid.dr": 0.05, "driver.n0": 0.3, "driver.x0": 0.1,
+           "driver.nparticles": 10000, "driver.sigma_r": 0.4, "electrons.ppc": 1, "time.steps": 4, "time.dt": 5}]
+    _graph_vs_plain(qa, f"{decks}/linear.in", ov, 5, 3)
+
+
 # ------------------------------------------------------------------------------------------ beam sums
 @pytest.mark.parametrize("np_", [0, 1, 4095, 4097, 300001])
 def test_beam_sums_device_matches_host(torch_dev, qa, np_):
