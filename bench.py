@@ -56,21 +56,30 @@ def bench_batched(args):
     import torch
     import quarz_amd
     from quarz_amd import configs
-    runs = configs.c5_runs(args.runs, steps=args.warmup + args.steps + args.plain_steps + 1)
+    warmup = max(args.warmup, 2)   # the leapfrog start-up step (eager), then the step that captures the graph
+    runs = configs.c5_runs(args.runs, steps=warmup + args.steps + args.plain_steps + 1)
     sims = []
     t0 = time.perf_counter()
     for i, (deck, _) in enumerate(runs):
         sims.append(quarz_amd.Simulation(deck))
         if (i + 1) % 32 == 0:
             print(f"# set up {i + 1}/{len(runs)} runs ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        quarz_amd.step_batched(sims)
+    # launch shapes of the beam kernels, tuned on the run with the most beam particles
+    big = max(range(len(runs)), key=lambda i: runs[i][1]["nb"])
+    shapes = sims[big].autotune()
+    batch = quarz_amd.Batch(sims, graph=True)
+    te = time.perf_counter()
+    kern_ms = batch.step()   # eager: the batched sweep launch time
+    torch.cuda.synchronize()
+    eager_wall = time.perf_counter() - te
+    for _ in range(warmup - 1):
+        batch.step()
     torch.cuda.synchronize()
     psl = pushes = 0.0
     t0 = time.perf_counter()
-    kern_ms = 0.0
+    graph_ms = 0.0
     for _ in range(args.steps):
-        kern_ms += quarz_amd.step_batched(sims)
+        graph_ms += batch.step()
         for s in sims:
             st = s.stats()
             psl += st["particle_slices"]
@@ -78,6 +87,8 @@ def bench_batched(args):
         print(f"# batched step done ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    replays = batch.replays
+    batch.close()
     # the plain path: the same runs, one launch per run, one after the other
     ppsl = 0.0
     t1 = time.perf_counter()
@@ -93,7 +104,7 @@ def bench_batched(args):
     fp, fn = FLOPS[0]
     node_slices = sum(s.M * s.K for s in sims)
     flops = fp * psl / args.steps + fn * node_slices
-    achieved = flops / (kern_ms / args.steps / 1e3)
+    achieved = flops / (kern_ms / 1e3)
     Ns = sorted(s.M - 1 for s in sims)
     out = {
         "metric": METRIC,
@@ -101,7 +112,7 @@ def bench_batched(args):
         "unit": "particle-slices/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -112,7 +123,11 @@ def bench_batched(args):
                    "cells_min_median_max": [Ns[0], Ns[len(Ns) // 2], Ns[-1]],
                    "slices_total": sum(s.K for s in sims), "parallelism": "one workgroup per run, one launch"},
         "beam_pushes_per_s": pushes / wall,
-        "batched_sweep_ms": kern_ms / args.steps,
+        "batched_sweep_ms": kern_ms,
+        "graph_step_ms": graph_ms / args.steps,
+        "graph_replays": replays,
+        "eager_step_ms": eager_wall * 1e3,
+        "launch_shapes": shapes,
         "plain_value": ppsl / pwall,
         "plain_steps": args.plain_steps,
         "batched_vs_plain": (psl / wall) / (ppsl / pwall),
@@ -148,6 +163,7 @@ def main():
     total = args.warmup + args.steps + 1
     sim = quarz_amd.Simulation(deck, {"time.steps": total})
     modes = sim.m1
+    shapes = sim.autotune()   # launch shapes of the beam kernels for this run
     for _ in range(args.warmup):
         sim.step()
     torch.cuda.synchronize()
@@ -190,6 +206,7 @@ def main():
         "beam_deposit_ms": dep_ms / args.steps,
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
+        "launch_shapes": shapes,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
                      "kernel": "sweep_kernel",

@@ -189,6 +189,18 @@ int qz_batch_step(qz_batch* b, double* step_ms);
 long qz_batch_replays(const qz_batch* b);   /* steps run as graph replays so far */
 void qz_batch_destroy(qz_batch* b);
 
+/* ---- launch shapes of the beam kernels (not in the reference) ----
+ * Threads per block (a multiple of 64 in [64, 1024]) of: the beam push (Beam::advance), the deposit's
+ * per-particle kernels (cell keys and scatter), its bucket sort and its per-node sums.  One table for the
+ * process, 256 each until set.  Results do not depend on the shapes. */
+enum { QZ_K_BEAM_ADVANCE = 0, QZ_K_BEAM_PARTICLES = 1, QZ_K_BEAM_SORT = 2, QZ_K_BEAM_NODES = 3, QZ_K_COUNT = 4 };
+int qz_set_block(int kernel, int threads);
+int qz_get_block(int kernel);
+/* Autotuner step: times each of ncand block sizes for `kernel` on scratch copies of the run's largest beam
+ * (the push, or the whole deposit into a scratch source array), reps launches each after one warm-up launch;
+ * ms_out[i] receives candidate i's mean time, and the fastest is set.  The run's own state is not changed. */
+int qz_sim_autotune(qz_sim* s, int kernel, const int* candidates, int ncand, int reps, double* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

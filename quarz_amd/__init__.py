@@ -6,6 +6,7 @@ This module is a thin ctypes driver over it: `Simulation` mirrors the reference'
 There is no CPU fallback: without the built library every call raises.
 """
 import ctypes as C
+import json
 import os
 
 import numpy as np
@@ -72,6 +73,9 @@ def lib():
         L.qz_batch_replays.restype = C.c_long
         L.qz_batch_destroy.argtypes = [C.c_void_p]
         L.qz_batch_destroy.restype = None
+        L.qz_set_block.argtypes = [C.c_int, C.c_int]
+        L.qz_get_block.argtypes = [C.c_int]
+        L.qz_sim_autotune.argtypes = [C.c_void_p, C.c_int, IP, C.c_int, C.c_int, DP]
         L.qz_sim_push_beams.argtypes = [C.c_void_p]
         L.qz_sim_get.restype = C.c_long
         L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
@@ -157,6 +161,36 @@ class Simulation:
     PHASES = ["geometry", "sort", "dep_contrib", "dep_sums", "combine", "bz_rhs", "solves", "wplus", "s_gather",
               "s_contrib", "s_sums", "picard_stage", "picard_rhs", "picard_solve", "store", "push"]
 
+    def autotune(self, candidates=(64, 128, 256, 512, 1024), reps=3, cache=None):
+        """Choose the beam kernels' threads per block for this run's shapes: one kernel at a time (the others
+        at their current choice), every candidate timed on scratch copies of the largest beam.  The choice
+        is applied to the process (set_block) and returned as {kernel: threads}.  With `cache` (a JSON file),
+        a stored choice for the same device and shapes is applied without timing, and new ones are stored."""
+        sizes = sorted(self.get(f"beam{b}.w").size for b in range(self.nbeams))
+        key = f"{_device_name()}|m{int(self.m1)}|M{self.M}|K{self.K}|nb{sizes[-1] if sizes else 0}"
+        table = {}
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+        if key in table:
+            for name, threads in table[key].items():
+                set_block(name, threads)
+            return dict(table[key])
+        cand = (C.c_int * len(candidates))(*candidates)
+        out = np.zeros(len(candidates))
+        chosen, times = {}, {}
+        for k, name in enumerate(KERNELS):
+            check(lib().qz_sim_autotune(self.h, k, cand, len(candidates), reps, out.ctypes.data_as(DP)))
+            chosen[name] = get_block(name)
+            times[name] = {int(c): float(t) for c, t in zip(candidates, out)}
+        self.autotune_ms = times
+        if cache:
+            table[key] = chosen
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            with open(cache, "w") as f:
+                json.dump(table, f, indent=1)
+        return chosen
+
     def phase_ms(self):
         """Milliseconds of each sweep phase of the last sweep (one thread's wall clock, 100 MHz ticks)."""
         out = np.zeros(len(self.PHASES))
@@ -218,6 +252,27 @@ def step_batched(sims):
     ms = C.c_double(0)
     check(lib().qz_sim_step_batched(arr, len(sims), C.byref(ms)))
     return ms.value
+
+
+# beam kernels with a tunable launch shape (QZ_K_* of quarz_amd.h, in that order)
+KERNELS = ("beam_advance", "beam_particles", "beam_sort", "beam_nodes")
+
+
+def set_block(kernel, threads):
+    """Threads per block of a beam kernel (name in KERNELS): a multiple of 64 in [64, 1024], process-wide."""
+    check(lib().qz_set_block(KERNELS.index(kernel), int(threads)))
+
+
+def get_block(kernel):
+    return lib().qz_get_block(KERNELS.index(kernel))
+
+
+def _device_name():
+    try:
+        import torch
+        return torch.cuda.get_device_name(0)
+    except Exception:
+        return "gpu"
 
 
 class Batch:

@@ -980,9 +980,12 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
 // run's beam deposit and plasma reload, the batched sweep, each run's beam push) is captured once as a HIP
 // graph on the batch's stream and replayed.  Not in the reference.
 struct qz_batch {
+    static constexpr int kBranches = 8;   // streams the per-run work of a captured step is spread over
     std::vector<qz_sim*> sims;
     bool use_graph = true;
     hipStream_t st = nullptr;
+    hipStream_t br[kBranches] = {};
+    hipEvent_t fork[2] = {}, join[2][kBranches] = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;
     qz_sweep_args* dargs = nullptr;   // device copy of the runs' sweep arguments, read by the graph's launch
     std::vector<qz_sweep_args> hargs;
@@ -996,6 +999,13 @@ struct qz_batch {
         if (dargs) (void)hipFree(dargs);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        for (int f = 0; f < 2; ++f) {
+            if (fork[f]) (void)hipEventDestroy(fork[f]);
+            for (auto e : join[f])
+                if (e) (void)hipEventDestroy(e);
+        }
+        for (auto b : br)
+            if (b) (void)hipStreamDestroy(b);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -1011,6 +1021,11 @@ extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out
         }
         b->use_graph = use_graph != 0;
         qz::hip_check(hipStreamCreate(&b->st), "stream");
+        for (auto& x : b->br) qz::hip_check(hipStreamCreate(&x), "stream");
+        for (int f = 0; f < 2; ++f) {
+            qz::hip_check(hipEventCreateWithFlags(&b->fork[f], hipEventDisableTiming), "event");
+            for (auto& e : b->join[f]) qz::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        }
         qz::hip_check(hipEventCreate(&b->e0), "event");
         qz::hip_check(hipEventCreate(&b->e1), "event");
         *out = b.release();
@@ -1019,7 +1034,9 @@ extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out
 
 extern "C" void qz_batch_destroy(qz_batch* b) { delete b; }
 
-// capture of one step of every run on the batch stream (each run's stream is switched to it meanwhile)
+// capture of one step of every run: the per-run work (preparation, then beam push) on branch streams forked
+// from and joined back to the batch stream around the batched sweep, so that runs overlap in the graph
+// (each run's stream is switched to its branch meanwhile)
 static void capture_batch(qz_batch* b) {
     const int n = static_cast<int>(b->sims.size());
     b->hargs.resize(n);
@@ -1028,19 +1045,25 @@ static void capture_batch(qz_batch* b) {
     qz::hip_check(hipMemcpy(b->dargs, b->hargs.data(), sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice), "batch arguments");
     std::vector<hipStream_t> saved(n);
     for (int i = 0; i < n; ++i) saved[i] = b->sims[i]->s->stream;
+    const int nb = std::min(n, qz_batch::kBranches);
+    auto per_run = [&](int phase, auto&& work) {   // fork, per-run work on the branches, join
+        qz::hip_check(hipEventRecord(b->fork[phase], b->st), "fork");
+        for (int j = 0; j < nb; ++j) qz::hip_check(hipStreamWaitEvent(b->br[j], b->fork[phase], 0), "fork");
+        for (int i = 0; i < n; ++i) {
+            b->sims[i]->s->stream = b->br[i % nb];
+            work(*b->sims[i]->s);
+            b->sims[i]->s->stream = saved[i];
+        }
+        for (int j = 0; j < nb; ++j) {
+            qz::hip_check(hipEventRecord(b->join[phase][j], b->br[j]), "join");
+            qz::hip_check(hipStreamWaitEvent(b->st, b->join[phase][j], 0), "join");
+        }
+    };
     qz::hip_check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture");
     try {
-        for (int i = 0; i < n; ++i) {
-            b->sims[i]->s->stream = b->st;
-            b->sims[i]->s->prepare_sweep(false);
-            b->sims[i]->s->stream = saved[i];
-        }
+        per_run(0, [](qz::Simulation& s) { s.prepare_sweep(false); });
         if (qz_sweep_launch(b->dargs, b->hargs.data(), n, b->st) != QZ_OK) throw std::runtime_error(qz_last_error());
-        for (int i = 0; i < n; ++i) {
-            b->sims[i]->s->stream = b->st;
-            b->sims[i]->s->enqueue_push(false);
-            b->sims[i]->s->stream = saved[i];
-        }
+        per_run(1, [](qz::Simulation& s) { s.enqueue_push(false); });
     } catch (...) {
         for (int i = 0; i < n; ++i) b->sims[i]->s->stream = saved[i];
         hipGraph_t g = nullptr;
@@ -1090,6 +1113,62 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
     })
 }
 extern "C" long qz_batch_replays(const qz_batch* b) { return b ? b->replays : -1; }
+
+// Autotuner step for one beam kernel (quarz_amd.h): candidates timed on scratch copies of the largest beam
+extern "C" int qz_sim_autotune(qz_sim* h, int kernel, const int* cand, int ncand, int reps, double* ms_out) {
+    if (!h || !cand || ncand < 1 || reps < 1 || !ms_out) return qz_fail(QZ_EINVAL, "qz_sim_autotune: bad arguments");
+    if (kernel < 0 || kernel >= QZ_K_COUNT) return qz_fail(QZ_EINVAL, "qz_sim_autotune: unknown kernel");
+    QZ_SIM_TRY({
+        qz::Simulation& s = *h->s;
+        const bool push = kernel == QZ_K_BEAM_ADVANCE;
+        const qz::Beam* bm = nullptr;
+        for (const auto& b : s.beams)
+            if ((!push || !b->rigid) && (!bm || b->Np > bm->Np)) bm = b.get();
+        for (int i = 0; i < ncand; ++i) ms_out[i] = 0;
+        if (!bm || bm->Np == 0) return QZ_OK;   // nothing to time: the current shape stays
+        const size_t np = static_cast<size_t>(bm->Np);
+        qz::DevBuf<qz::Real> a[7];
+        for (int c = 0; c < 7; ++c) {
+            a[c].alloc(np);
+            qz::hip_check(hipMemcpyAsync(a[c].p, bm->a[c].p, 8 * np, hipMemcpyDeviceToDevice, s.stream), "scratch copy");
+        }
+        const qz_beam_soa soa{a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a[6].p};
+        qz::DevBuf<qz::Real> src;
+        if (!push) src.alloc(s.bsrc.n);
+        const qz_radial_grid g = s.grid.view();
+        const int m1 = s.m1 ? 1 : 0;
+        auto run = [&]() {
+            if (push) {
+                qz::qz_check(qz_beam_advance(&g, &s.box, m1, bm->q / bm->m, bm->pusher, 0, bm->Np, &soa, s.fld.p, s.NCF,
+                                             s.dt, s.dt, 1, s.stream));
+            } else {
+                qz::qz_check(qz_zero_async(src.p, sizeof(qz::Real) * src.n, s.stream));
+                qz::qz_check(qz_beam_deposit(&g, &s.box, m1, bm->q, bm->Np, &soa, src.p, s.NCB, s.dep_work.p, s.stream));
+            }
+        };
+        struct Events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+        } ev;
+        qz::hip_check(hipEventCreate(&ev.e[0]), "event");
+        qz::hip_check(hipEventCreate(&ev.e[1]), "event");
+        int best = -1;
+        for (int i = 0; i < ncand; ++i) {
+            if (qz_set_block(kernel, cand[i]) != QZ_OK) throw std::runtime_error(qz_last_error());
+            run();   // warm-up
+            qz::hip_check(hipEventRecord(ev.e[0], s.stream), "event");
+            for (int r = 0; r < reps; ++r) run();
+            qz::hip_check(hipEventRecord(ev.e[1], s.stream), "event");
+            qz::hip_check(hipEventSynchronize(ev.e[1]), "sync");
+            float ms = 0;
+            qz::hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "elapsed");
+            ms_out[i] = ms / reps;
+            if (best < 0 || ms_out[i] < ms_out[best]) best = i;
+        }
+        if (qz_set_block(kernel, cand[best]) != QZ_OK) throw std::runtime_error(qz_last_error());
+        qz::hip_check(hipStreamSynchronize(s.stream), "sync");
+    })
+}
 
 extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
 extern "C" int qz_sim_push_beams(qz_sim* h) {

@@ -316,6 +316,20 @@ DepWork dep_layout(int np, int nloc, int N) {
 }
 }  // namespace
 
+// Threads per block of the beam kernels (QZ_K_*), chosen by the autotuner (qz_sim_autotune) or set with
+// qz_set_block; 256 until then.  One table for the process.
+static int g_block[QZ_K_COUNT] = {256, 256, 256, 256};
+
+extern "C" int qz_set_block(int kernel, int threads) {
+    if (kernel < 0 || kernel >= QZ_K_COUNT) return qz_fail(QZ_EINVAL, "qz_set_block: unknown kernel");
+    if (threads < 64 || threads > 1024 || threads % 64 != 0)
+        return qz_fail(QZ_EINVAL, "qz_set_block: threads must be a multiple of 64 in [64, 1024]");
+    g_block[kernel] = threads;
+    return QZ_OK;
+}
+
+extern "C" int qz_get_block(int kernel) { return (kernel < 0 || kernel >= QZ_K_COUNT) ? -1 : g_block[kernel]; }
+
 int qz_zero_async(void* p, size_t bytes, hipStream_t st) {
     if (bytes == 0) return QZ_OK;
     if (!p || bytes % 4 != 0) return qz_fail(QZ_EINVAL, "zero fill: null pointer or size not a multiple of 4");
@@ -373,7 +387,7 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
         const int rc2 = qz_zero_async(fill, 4 * (static_cast<size_t>(ncells) + 1), st);
         if (rc2) return rc2;
     }
-    const int tb = 256;
+    const int tb = g_block[QZ_K_BEAM_PARTICLES], tsort = g_block[QZ_K_BEAM_SORT], tnode = g_block[QZ_K_BEAM_NODES];
     hipLaunchKernelGGL(beam_key_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q, np, *p, key, rec, counts);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
     const int nb = (n1 + 1023) / 1024;
@@ -381,9 +395,9 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
     hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
     hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
     hipLaunchKernelGGL(beam_scatter_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, key, start, fill, list);
-    hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tb - 1) / tb), dim3(tb), 0, st, ncells, start, list);
+    hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tsort - 1) / tsort), dim3(tsort), 0, st, ncells, start, list);
     const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
-    hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tb - 1) / tb)), dim3(tb), 0, st, *grid, *box,
+    hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tnode - 1) / tnode)), dim3(tnode), 0, st, *grid, *box,
                        modes == 1 ? 1 : 0, start, list, rec, bsrc, nc);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
@@ -398,7 +412,7 @@ extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* b
     if (nc != (modes == 1 ? F_NC1 : F_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_advance: nc must be 6 (m = 0) or 18 (m = 1)");
     if (pusher < 0 || pusher > 3) return qz_fail(QZ_EINVAL, "qz_beam_advance: pusher must be 0 vay, 1 boris, 2 hc, 3 imp");
     if (np - i0 <= 0) return QZ_OK;
-    const int tb = 256;
+    const int tb = g_block[QZ_K_BEAM_ADVANCE];
     hipLaunchKernelGGL(beam_advance_kernel, dim3((np - i0 + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
                        *grid, *box, modes == 1 ? 1 : 0, qm, pusher, i0, np, *p, fld, nc, dt_kick, dt, move);
     QZ_HIP(hipGetLastError());

@@ -32,3 +32,20 @@ def test_library_exports_every_declared_symbol():
 def test_last_error_is_empty_string_initially():
     import quarz_amd
     assert isinstance(quarz_amd.last_error(), str)
+
+
+def test_set_block_validates_and_round_trips():
+    """The launch-shape table is host state: no GPU call is made."""
+    import quarz_amd
+    for k in quarz_amd.KERNELS:
+        before = quarz_amd.get_block(k)
+        assert before == 256
+        for bad in (0, 32, 100, 2048):
+            try:
+                quarz_amd.set_block(k, bad)
+                raise AssertionError(f"{k}: {bad} accepted")
+            except quarz_amd.QuarzError:
+                pass
+        quarz_amd.set_block(k, 128)
+        assert quarz_amd.get_block(k) == 128
+        quarz_amd.set_block(k, before)

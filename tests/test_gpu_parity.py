@@ -282,6 +282,52 @@ def test_graph_batch_m1_equals_plain(qa, decks):
     _graph_vs_plain(qa, f"{decks}/linear.in", ov, 5, 3)
 
 
+# ------------------------------------------------------------------------------------------ launch shapes
+def _state(qa, sim):
+    out = {w: getattr(sim, w) for w in ("fld", "diag", "bsrc")}
+    for bi in range(sim.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            out[f"beam{bi}.{k}"] = sim.get(f"beam{bi}.{k}")
+    return out
+
+
+def test_launch_shapes_and_autotune_keep_results(qa, decks):
+    """Beam kernels at 64 and 1024 threads per block give the results of the default 256 bit for bit; the
+    autotuner leaves the run's state untouched and picks one of its candidates."""
+    ov = {"time.steps": 3, "driver.nparticles": 30000, "xi.min": -8}
+    ref = qa.Simulation(f"{decks}/gain.in", ov)
+    for _ in range(2):
+        ref.step()
+    want = _state(qa, ref)
+    try:
+        for threads in (64, 1024):
+            for k in qa.KERNELS:
+                qa.set_block(k, threads)
+            s = qa.Simulation(f"{decks}/gain.in", ov)
+            for _ in range(2):
+                s.step()
+            got = _state(qa, s)
+            for name in want:
+                assert _same(got[name], want[name]), _report(f"{threads} threads {name}", got[name], want[name])
+        for k in qa.KERNELS:
+            qa.set_block(k, 256)
+        s = qa.Simulation(f"{decks}/gain.in", ov)
+        s.step()
+        before = _state(qa, s)
+        chosen = s.autotune(candidates=(64, 256), reps=1)
+        after = _state(qa, s)
+        for name in before:
+            assert _same(after[name], before[name]), _report(f"autotune changed {name}", after[name], before[name])
+        assert set(chosen) == set(qa.KERNELS) and all(v in (64, 256) for v in chosen.values())
+        s.step()
+        got = _state(qa, s)
+        for name in want:
+            assert _same(got[name], want[name]), _report(f"tuned {name}", got[name], want[name])
+    finally:
+        for k in qa.KERNELS:
+            qa.set_block(k, 256)
+
+
 # ------------------------------------------------------------------------------------------ beam sums
 @pytest.mark.parametrize("np_", [0, 1, 4095, 4097, 300001])
 def test_beam_sums_device_matches_host(torch_dev, qa, np_):
