@@ -128,9 +128,9 @@ def bench_batched(args):
         "graph_replays": replays,
         "eager_step_ms": eager_wall * 1e3,
         "launch_shapes": shapes,
-        "plain_value": ppsl / pwall,
+        "plain_value": ppsl / pwall if ppsl > 0 else None,
         "plain_steps": args.plain_steps,
-        "batched_vs_plain": (psl / wall) / (ppsl / pwall),
+        "batched_vs_plain": (psl / wall) / (ppsl / pwall) if ppsl > 0 else None,
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel (batched)",
                      "flops_per_launch": flops,
