@@ -200,6 +200,8 @@ int qz_get_block(int kernel);
  * (the push, or the whole deposit into a scratch source array), reps launches each after one warm-up launch;
  * ms_out[i] receives candidate i's mean time, and the fastest is set.  The run's own state is not changed. */
 int qz_sim_autotune(qz_sim* s, int kernel, const int* candidates, int ncand, int reps, double* ms_out);
+/* "<gfx arch name>|<compute units>CU" of the current device, the key under which launch shapes are cached */
+int qz_device_tag(char* out, int n);
 
 #ifdef __cplusplus
 }

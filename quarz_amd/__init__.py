@@ -76,6 +76,7 @@ def lib():
         L.qz_set_block.argtypes = [C.c_int, C.c_int]
         L.qz_get_block.argtypes = [C.c_int]
         L.qz_sim_autotune.argtypes = [C.c_void_p, C.c_int, IP, C.c_int, C.c_int, DP]
+        L.qz_device_tag.argtypes = [C.c_char_p, C.c_int]
         L.qz_sim_push_beams.argtypes = [C.c_void_p]
         L.qz_sim_get.restype = C.c_long
         L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
@@ -268,11 +269,9 @@ def get_block(kernel):
 
 
 def _device_name():
-    try:
-        import torch
-        return torch.cuda.get_device_name(0)
-    except Exception:
-        return "gpu"
+    buf = C.create_string_buffer(256)
+    check(lib().qz_device_tag(buf, 256))
+    return buf.value.decode()
 
 
 class Batch:

@@ -1,6 +1,8 @@
 // C-ABI glue: error reporting, device information, the batched Thomas solver and the sweep entry points.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -86,6 +88,16 @@ extern "C" int qz_tridiag_thomas_batched(const double* a, const double* b, const
 }
 
 extern "C" size_t qz_sweep_workspace_doubles(int M) { return qz_sweep_workspace_doubles_impl(M); }
+
+extern "C" int qz_device_tag(char* out, int n) {
+    if (!out || n < 1) return qz_fail(QZ_EINVAL, "qz_device_tag: no buffer");
+    int dev = 0;
+    QZ_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t pr;
+    QZ_HIP(hipGetDeviceProperties(&pr, dev));
+    std::snprintf(out, static_cast<size_t>(n), "%s|%dCU", pr.gcnArchName, pr.multiProcessorCount);
+    return QZ_OK;
+}
 
 static int validate_sweep(const qz_sweep_args& a) {
     if (a.modes != 0 && a.modes != 1) return qz_fail(QZ_EINVAL, "sweep: modes must be 0 or 1");

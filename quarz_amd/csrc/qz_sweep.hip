@@ -442,6 +442,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
     Real* s_bw0 = reinterpret_cast<Real*>(smem + plan.misc + 16);                        // 4 doubles
     int* s_comp = reinterpret_cast<int*>(smem + plan.misc + 48);                         // 24 ints
     unsigned long long* s_ph = reinterpret_cast<unsigned long long*>(smem + plan.misc + 160);   // phase clocks + last
+    // order of the per-node sums (in the wave-counter area once the sort is done): M node ids
+    unsigned short* s_norder = reinterpret_cast<unsigned short*>(smem + plan.wcnt);
     Real* B = reinterpret_cast<Real*>(smem + plan.region);
     const int Bcap = (kLdsBytes - plan.region) / 8;
     auto BA = [&](int slot) -> Real* { return B + static_cast<size_t>(slot) * Mp; };
@@ -611,6 +613,32 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
+        // Order of the per-node sums: nodes whose lists exceed twice the mean length (a compressed sheath holds
+        // hundreds of entries) first, then the others, each group in node order.  A wave's item round costs its
+        // longest list, so a thread then sums at most one long list per pass.  Wave 0 builds it while the other
+        // waves search their list positions; it is read after the deposit's first barrier (s_misc[1] = number
+        // of long nodes).
+        if (w == 0) {
+            const unsigned long long lt_mask = (1ull << lane) - 1ull;
+            const int thr = max(8, (4 * L) / M);
+            int nlong = 0;
+            for (int t0 = 0; t0 < M; t0 += 64) {
+                const int n = t0 + lane;
+                nlong += __popcll(__ballot(n < M && s_cs[n + 2] - s_cs[n] > thr));
+            }
+            int cl = 0, csh = nlong;
+            for (int t0 = 0; t0 < M; t0 += 64) {
+                const int n = t0 + lane;
+                const bool valid = n < M;
+                const bool lg = valid && s_cs[n + 2] - s_cs[n] > thr;
+                const unsigned long long bl = __ballot(lg), bs = __ballot(valid && !lg);
+                if (lg) s_norder[cl + __popcll(bl & lt_mask)] = static_cast<unsigned short>(n);
+                else if (valid) s_norder[csh + __popcll(bs & lt_mask)] = static_cast<unsigned short>(n);
+                cl += __popcll(bl);
+                csh += __popcll(bs);
+            }
+            if (lane == 0) s_misc[1] = nlong;
+        }
         // node n's list holds cell n-1 (right entries) and cell n (left entries), merged in particle-index order
 #pragma unroll
         for (int pp = 0; pp < PT; ++pp) {
@@ -633,12 +661,25 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         // pass: contributions into LDS lists, a barrier, the per-node sums, a barrier.
         Real* __restrict__ Dg = A.diag + static_cast<size_t>(k) * M * A.ncd;
         const int ncd = A.ncd;
+        // (component, node) of sum item `item` of a pass over gcnt components, in the order of s_norder
+        auto item_node = [&](int item, int gcnt, int nl, int& g, int& n) {
+            if (item < gcnt * nl) {
+                g = item / nl;
+                n = s_norder[item - g * nl];
+            } else {
+                const int r = item - gcnt * nl, ns = M - nl;
+                g = r / ns;
+                n = s_norder[nl + (r - g * ns)];
+            }
+        };
         {
             int gcur = 0, c0 = 0;
             auto reduce = [&]() {
                 __syncthreads();
+                const int nl = s_misc[1];
                 for (int item = tid; item < M * gcur; item += kBS) {
-                    const int g = item / M, n = item - g * M;
+                    int g, n;
+                    item_node(item, gcur, nl, g, n);
                     const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
                     const Real* vb = B + g * 2 * L;
                     const Real acc = sum_list(vb, e0, e1);
@@ -927,8 +968,10 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             int gcur = 0, c0 = 0;
             auto reduce = [&]() {
                 __syncthreads();
+                const int nl = s_misc[1];
                 for (int item = tid; item < M * gcur; item += kBS) {
-                    const int g = item / M, n = item - g * M;
+                    int g, n;
+                    item_node(item, gcur, nl, g, n);
                     const int e0 = s_cs[n] + s_cs[n + 1], e1 = s_cs[n + 1] + s_cs[n + 2];
                     const Real* vb = VB + g * 2 * L;
                     const Real acc = sum_list(vb, e0, e1);
