@@ -66,7 +66,7 @@ def bench_batched(args):
             print(f"# set up {i + 1}/{len(runs)} runs ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
     # launch shapes of the beam kernels, tuned on the run with the most beam particles
     big = max(range(len(runs)), key=lambda i: runs[i][1]["nb"])
-    shapes = sims[big].autotune()
+    shapes = sims[big].autotune(cache=args.tune_cache)
     batch = quarz_amd.Batch(sims, graph=True)
     te = time.perf_counter()
     kern_ms = batch.step()   # eager: the batched sweep launch time
@@ -148,6 +148,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--runs", type=int, default=256, help="c5: number of independent runs")
     ap.add_argument("--plain-steps", type=int, default=1, help="c5: steps of the one-launch-per-run comparison")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
@@ -163,7 +165,7 @@ def main():
     total = args.warmup + args.steps + 1
     sim = quarz_amd.Simulation(deck, {"time.steps": total})
     modes = sim.m1
-    shapes = sim.autotune()   # launch shapes of the beam kernels for this run
+    shapes = sim.autotune(cache=args.tune_cache)   # launch shapes of the beam kernels for this run
     for _ in range(args.warmup):
         sim.step()
     torch.cuda.synchronize()

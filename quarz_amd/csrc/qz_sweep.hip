@@ -613,12 +613,14 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             }
         }
         __syncthreads();
-        // Order of the per-node sums: nodes whose lists exceed twice the mean length (a compressed sheath holds
-        // hundreds of entries) first, then the others, each group in node order.  A wave's item round costs its
-        // longest list, so a thread then sums at most one long list per pass.  Wave 0 builds it while the other
-        // waves search their list positions; it is read after the deposit's first barrier (s_misc[1] = number
-        // of long nodes).
-        if (w == 0) {
+        // Order of the per-node sums (m = 1): nodes whose lists exceed twice the mean length (a compressed sheath
+        // holds hundreds of entries) first, then the others, each group in node order.  A wave's item round costs
+        // its longest list, so a thread then sums at most one long list per pass: 3.4% off the c2 sweep (eleven
+        // passes of two components).  In m = 0 (four components, passes of up to eight) it costs more than it
+        // saves (+2.9% on the c5 batch), and the sums keep node order.  Wave 0 builds it while the other waves
+        // search their list positions; it is read after the deposit's first barrier (s_misc[1] = number of long
+        // nodes).
+        if (M1 && w == 0) {
             const unsigned long long lt_mask = (1ull << lane) - 1ull;
             const int thr = max(8, (4 * L) / M);
             int nlong = 0;
@@ -627,7 +629,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 nlong += __popcll(__ballot(n < M && s_cs[n + 2] - s_cs[n] > thr));
             }
             int cl = 0, csh = nlong;
-            for (int t0 = 0; t0 < M; t0 += 64) {
+            for (int t0 = 0; nlong > 0 && t0 < M; t0 += 64) {   // no long list: the sums take plain node order
                 const int n = t0 + lane;
                 const bool valid = n < M;
                 const bool lg = valid && s_cs[n + 2] - s_cs[n] > thr;
@@ -661,22 +663,26 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         // pass: contributions into LDS lists, a barrier, the per-node sums, a barrier.
         Real* __restrict__ Dg = A.diag + static_cast<size_t>(k) * M * A.ncd;
         const int ncd = A.ncd;
-        // (component, node) of sum item `item` of a pass over gcnt components, in the order of s_norder
+        // (component, node) of sum item `item` of a pass over gcnt <= 8 components, in the order of s_norder
+        // (plain node order when no list is long); the component count by subtraction, not division
         auto item_node = [&](int item, int gcnt, int nl, int& g, int& n) {
-            if (item < gcnt * nl) {
-                g = item / nl;
-                n = s_norder[item - g * nl];
+            int r = item;
+            g = 0;
+            if (r < gcnt * nl) {
+                while (r >= nl) { r -= nl; ++g; }
+                n = s_norder[r];
             } else {
-                const int r = item - gcnt * nl, ns = M - nl;
-                g = r / ns;
-                n = s_norder[nl + (r - g * ns)];
+                r -= gcnt * nl;
+                const int ns = M - nl;
+                while (r >= ns) { r -= ns; ++g; }
+                n = nl > 0 ? s_norder[nl + r] : r;
             }
         };
         {
             int gcur = 0, c0 = 0;
             auto reduce = [&]() {
                 __syncthreads();
-                const int nl = s_misc[1];
+                const int nl = M1 ? s_misc[1] : 0;
                 for (int item = tid; item < M * gcur; item += kBS) {
                     int g, n;
                     item_node(item, gcur, nl, g, n);
@@ -968,7 +974,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
             int gcur = 0, c0 = 0;
             auto reduce = [&]() {
                 __syncthreads();
-                const int nl = s_misc[1];
+                const int nl = M1 ? s_misc[1] : 0;
                 for (int item = tid; item < M * gcur; item += kBS) {
                     int g, n;
                     item_node(item, gcur, nl, g, n);
