@@ -34,7 +34,7 @@ HBM_PEAK = 8.0e12
 FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
 # same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
-MEASURED_TRAFFIC = {"c2": 753.0e6}
+MEASURED_TRAFFIC = {"c2": 841.9e6}
 
 
 def cpu_baseline(deck, nthreads):

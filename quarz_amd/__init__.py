@@ -162,7 +162,7 @@ class Simulation:
     PHASES = ["geometry", "sort", "dep_contrib", "dep_sums", "combine", "bz_rhs", "solves", "wplus", "s_gather",
               "s_contrib", "s_sums", "picard_stage", "picard_rhs", "picard_solve", "store", "push"]
 
-    def autotune(self, candidates=(64, 128, 256, 512, 1024), reps=3, cache=None):
+    def autotune(self, candidates=(64, 128, 256, 512, 1024), reps=10, cache=None):
         """Choose the beam kernels' threads per block for this run's shapes: one kernel at a time (the others
         at their current choice), every candidate timed on scratch copies of the largest beam.  The choice
         is applied to the process (set_block) and returned as {kernel: threads}.  With `cache` (a JSON file),
