@@ -72,6 +72,9 @@ typedef struct qz_sweep_args {
     int modes;                 /* 0: m = 0; 1: m = 0 and 1 */
     int picard;                /* Picard iterations of the chi_1 coupling (m = 1) */
     int nslices;               /* K slices, head first */
+    int k_begin, k_end;        /* slices of this launch, [k_begin, k_end); k_end = 0 means K.  A launch that starts
+                                  past 0 continues the state the launch ending at k_begin left: plasma arrays, AB
+                                  history and the B+ warm start in ws (B+ starts at zero only at slice 0) */
     int ab_order;              /* Adams-Bashforth order 1..5 */
     int ngp;                   /* beams.xi_shape = ngp */
     double dxi, delta_min, max_qsa;
@@ -121,6 +124,14 @@ int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
 int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher, int i0,
                     int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt, int move,
                     void* stream);
+/* The same push restricted to the particles whose gather slice (after clamping to the box) is in
+ * [s_lo, s_hi) and whose marker pstep[i] (np ints, caller-owned) differs from `step`; pushed particles get
+ * pstep[i] = step.  A step pushed range by range, in increasing slice order, then pushes every live
+ * particle exactly once, as Beam::advance does (particles only slip to later slices).  Not in the reference:
+ * it serves the pipelined run, where slice ranges are pushed as soon as their fields exist. */
+int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,
+                           int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt,
+                           int move, int s_lo, int s_hi, int step, int* pstep, void* stream);
 
 /* The 17 weighted beam sums relative to gref and xiref (Beam::sums, src/Beam.cpp:1133-1155): the particles
  * are copied to the host and summed in index order, as the reference does (bit-identical). */
@@ -158,6 +169,12 @@ int qz_sim_step_fields(qz_sim* s);   /* the sweep only */
  * the batched sweep launch.  Not in the reference: it batches what Simulation::run does for one run. */
 int qz_sim_step_batched(qz_sim** sims, int n, double* batch_ms);
 int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
+/* nrun steps of Simulation::run with up to `depth` steps in flight, each step split into nchunks slice ranges:
+ * step n sweeps a range as soon as step n-1 has pushed the beam particles that can reach it.  Bit-identical
+ * to nrun calls of qz_sim_step.  Needs a started run (after the first step), the same dt and no field output
+ * in those steps.  out2 (optional): particle-slices and beam particles pushed over the nrun steps.  Not in
+ * the reference: its MPI ranks pipeline slice blocks instead, with different arithmetic at the block edges. */
+int qz_sim_run_pipelined(qz_sim* s, int nrun, int nchunks, int depth, double* out2);
 /* what: fld diag bsrc nodes, beam<b>.<x|y|px|py|pz|xi|w>, sp<s>.<x|y|px|py|dl|w>; returns the count */
 long qz_sim_get(qz_sim* s, const char* what, double* host_out, long cap);
 int qz_sim_beam_diag(qz_sim* s, int b, double* out15);

@@ -77,6 +77,7 @@ def lib():
         L.qz_get_block.argtypes = [C.c_int]
         L.qz_sim_autotune.argtypes = [C.c_void_p, C.c_int, IP, C.c_int, C.c_int, DP]
         L.qz_device_tag.argtypes = [C.c_char_p, C.c_int]
+        L.qz_sim_run_pipelined.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, DP]
         L.qz_sim_push_beams.argtypes = [C.c_void_p]
         L.qz_sim_get.restype = C.c_long
         L.qz_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
@@ -191,6 +192,14 @@ class Simulation:
             with open(cache, "w") as f:
                 json.dump(table, f, indent=1)
         return chosen
+
+    def run_pipelined(self, steps, chunks=64, depth=8):
+        """`steps` steps with up to `depth` in flight, each split into `chunks` slice ranges (bit-identical to
+        calling step() `steps` times; the run must have taken its first step).  Returns (particle-slices,
+        beam particles pushed) over the steps."""
+        out = np.zeros(2)
+        check(lib().qz_sim_run_pipelined(self.h, int(steps), int(chunks), int(depth), out.ctypes.data_as(DP)))
+        return float(out[0]), float(out[1])
 
     def phase_ms(self):
         """Milliseconds of each sweep phase of the last sweep (one thread's wall clock, 100 MHz ticks)."""

@@ -818,6 +818,202 @@ void Simulation::after_sweep() {
     ++step_n;
 }
 
+// ---------------------------------------------------------------------------- pipelined steps
+// Steps of Simulation::run overlapped in time: step n sweeps slice range c as soon as step n-1 has pushed
+// the beam particles that can reach it.  The result is the serial one bit for bit:
+//  - the plasma is reloaded at the head every step and B+ restarts at zero there (src/Simulation.cpp:705-709),
+//    so step n depends on step n-1 only through the beam particles;
+//  - beam particles only slip to later slices (src/Beam.cpp:1357): the particles step n deposits into rows
+//    <= r were pushed by step n-1 from rows <= r, and a range pushed once its fields exist, with every
+//    particle marked (qz_beam_advance_slices), is pushed exactly as Beam::advance pushes it;
+//  - the deposit of a row range is the full deposit restricted to those rows (node sums in particle-index
+//    order), and step n overwrites rows of fld only after step n-1's pushes have read them.
+// Every step in flight has its own stream, plasma arrays, workspace, bsrc and counters (a context); beam
+// particles, fld and diag are shared.  Events order the ranges: deposit(n, c) waits for push(n-1, c+1).
+// The sweep launches read their argument blocks from one device array made before the first launch: a
+// stream-ordered allocation per launch (as qz_sweep makes) could be handed from one stream to another while
+// a kernel still reads it.
+namespace {
+struct StepCtx {
+    hipStream_t st = nullptr;
+    std::vector<std::array<DevBuf<Real>, 7>> sp;   // per mobile species: x y px py dl w hist
+    DevBuf<Real> ws, bsrc;
+    DevBuf<char> work;
+    std::vector<hipEvent_t> pushed;                 // per slice range: this context's step pushed it
+    ~StepCtx() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (auto e : pushed)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+}  // namespace
+
+void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total) {
+    if (nrun < 1) throw std::runtime_error("run_pipelined: nrun < 1");
+    if (!capturable())
+        throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no field "
+                                 "output, steps left)");
+    if (step_n + nrun - 1 > nsteps) throw std::runtime_error("run_pipelined: more steps than time.steps leaves");
+    if (!outdir.empty() && out_every > 0)
+        for (int n = step_n; n < step_n + nrun; ++n)
+            if (n % out_every == 0) throw std::runtime_error("run_pipelined: a field output falls in the steps");
+    nchunks = std::max(1, std::min(nchunks, K));
+    depth = std::max(1, std::min(depth, nrun));
+    std::vector<int> cb(nchunks + 1);
+    for (int c = 0; c <= nchunks; ++c) cb[c] = static_cast<int>((static_cast<long long>(K) * c) / nchunks);
+
+    std::vector<PlasmaSpecies*> mob;
+    for (auto& sp : species)
+        if (sp->mobile) mob.push_back(sp.get());
+    std::vector<std::unique_ptr<StepCtx>> ctx(depth);
+    for (auto& cp : ctx) {
+        cp.reset(new StepCtx());
+        hip_check(hipStreamCreate(&cp->st), "stream");
+        cp->sp.resize(mob.size());
+        for (size_t s = 0; s < mob.size(); ++s) {
+            for (int a = 0; a < 6; ++a) cp->sp[s][a].alloc(mob[s]->cap);
+            cp->sp[s][6].alloc(mob[s]->hist.n);
+        }
+        cp->ws.alloc(ws.n);
+        cp->bsrc.alloc(bsrc.n);
+        cp->work.alloc(dep_work.n);
+        cp->pushed.resize(nchunks);
+        for (auto& e : cp->pushed) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+    }
+    DevBuf<int> lostv(static_cast<size_t>(nrun) * QZ_MAX_SPECIES);
+    DevBuf<unsigned long long> cntv(static_cast<size_t>(nrun) * 24);
+    qz_check(qz_zero_async(lostv.p, sizeof(int) * lostv.n, stream));
+    qz_check(qz_zero_async(cntv.p, sizeof(unsigned long long) * cntv.n, stream));
+    for (auto& b : beams)
+        if (b->pstep.n != static_cast<size_t>(b->Np)) {
+            b->pstep.alloc(b->Np);
+            qz_check(qz_zero_async(b->pstep.p, sizeof(int) * b->pstep.n, stream));
+        }
+    hip_check(hipStreamSynchronize(stream), "sync");   // the run's own stream is done before the contexts start
+
+    // the sweep arguments of every (step, range): the run's, with the step context's own arrays
+    std::vector<qz_sweep_args> hargs(static_cast<size_t>(nrun) * nchunks);
+    for (int i = 0; i < nrun; ++i) {
+        StepCtx& C = *ctx[i % depth];
+        qz_sweep_args A = sweep_args;
+        A.bsrc = C.bsrc.p;
+        A.ws = C.ws.p;
+        A.lost = lostv.p + static_cast<size_t>(i) * QZ_MAX_SPECIES;
+        A.stats = cntv.p + static_cast<size_t>(i) * 24;
+        for (size_t s = 0; s < mob.size(); ++s) {
+            A.sp[s].x = C.sp[s][0].p; A.sp[s].y = C.sp[s][1].p; A.sp[s].px = C.sp[s][2].p;
+            A.sp[s].py = C.sp[s][3].p; A.sp[s].dl = C.sp[s][4].p; A.sp[s].w = C.sp[s][5].p;
+            A.sp[s].hist = C.sp[s][6].p;
+        }
+        for (int c = 0; c < nchunks; ++c) {
+            A.k_begin = cb[c];
+            A.k_end = cb[c + 1];
+            if (qz_validate_sweep(A) != QZ_OK) throw std::runtime_error(qz_last_error());
+            hargs[static_cast<size_t>(i) * nchunks + c] = A;
+        }
+    }
+    DevBuf<qz_sweep_args> dargs(hargs.size());
+    hip_check(hipMemcpy(dargs.p, hargs.data(), sizeof(qz_sweep_args) * hargs.size(), hipMemcpyHostToDevice), "arguments");
+
+    const qz_radial_grid g = grid.view();
+    const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
+    const int mm = m1 ? 1 : 0;
+    const size_t row = static_cast<size_t>(M) * NCB;
+    for (int i = 0; i < nrun; ++i) {
+        StepCtx& C = *ctx[i % depth];
+        StepCtx* P = i > 0 ? ctx[(i - 1) % depth].get() : nullptr;
+        const int n = step_n + i;
+        const bool last = n >= nsteps;   // the last sweep has no beam push (src/Simulation.cpp:1430-1446)
+        const int marker = n + 1;
+        hipStream_t st = C.st;
+        // plasma at the head (the rings of the first load) with a fresh AB history
+        for (size_t s = 0; s < mob.size(); ++s) {
+            const size_t bytes = sizeof(Real) * static_cast<size_t>(mob[s]->Np);
+            const DevBuf<Real>* src[6] = {&mob[s]->t_x, &mob[s]->t_y, &mob[s]->t_px, &mob[s]->t_py, &mob[s]->t_dl,
+                                          &mob[s]->t_w};
+            for (int a = 0; a < 6; ++a)
+                if (bytes) hip_check(hipMemcpyAsync(C.sp[s][a].p, src[a]->p, bytes, hipMemcpyDeviceToDevice, st), "plasma copy");
+            qz_check(qz_zero_async(C.sp[s][6].p, sizeof(Real) * C.sp[s][6].n, st));
+        }
+        unsigned long long* cnt = cntv.p + static_cast<size_t>(i) * 24;
+        auto push_range = [&](int c) {
+            const int lo = cb[c], hi = c == nchunks - 1 ? K : cb[c + 1];
+            for (auto& b : beams) {
+                if (b->rigid) continue;
+                const qz_beam_soa so = b->soa();
+                qz_check(qz_beam_advance_slices(&g, &box, mm, b->q / b->m, b->pusher, b->Np, &so, fld.p, NCF, dkick, ddrift,
+                                                1, lo, hi, marker, b->pstep.p, st));
+            }
+            hip_check(hipEventRecord(C.pushed[c], st), "event");
+        };
+        for (int c = 0; c < nchunks; ++c) {
+            const int c0 = cb[c], c1 = cb[c + 1];
+            // beam sources of rows [d0, d1]: up to row c1, which the range's last slice differentiates
+            if (P) hip_check(hipStreamWaitEvent(st, P->pushed[std::min(c + 1, nchunks - 1)], 0), "wait");
+            const int d0 = c == 0 ? 0 : c0 + 1, d1 = std::min(c1, K - 1);
+            if (c == nchunks - 1)   // every push of step n-1 is done: live counts as Simulation::step counts them
+                for (auto& b : beams) {
+                    qz_check(qz_beam_count_live(b->Np, b->a[6].p, cnt + 20, st));
+                    if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, cnt + 21, st));
+                }
+            if (d0 <= d1) {
+                qz_check(qz_zero_async(C.bsrc.p + d0 * row, sizeof(Real) * row * static_cast<size_t>(d1 - d0 + 1), st));
+                qz_beam_grid bx = box;
+                bx.k0 = box.k0 + d0;
+                bx.nloc = d1 - d0 + 1;
+                for (auto& b : beams) {
+                    const qz_beam_soa so = b->soa();
+                    qz_check(qz_beam_deposit(&g, &bx, mm, b->q, b->Np, &so, C.bsrc.p + d0 * row, NCB, C.work.p, st));
+                }
+            }
+            const size_t ai = static_cast<size_t>(i) * nchunks + c;
+            qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
+            if (!last && c > 0) push_range(c - 1);   // its last slice's gather reads row c0, just stored
+        }
+        if (!last) push_range(nchunks - 1);
+    }
+    for (auto& cp : ctx) hip_check(hipStreamSynchronize(cp->st), "sync");
+
+    // host side of the steps: totals, the last step's counters, the run's arrays as after the last step
+    std::vector<unsigned long long> hc;
+    cntv.download(hc, stream);
+    std::vector<int> hl;
+    lostv.download(hl, stream);
+    psl_total = pushed_total = 0;
+    for (int i = 0; i < nrun; ++i) {
+        psl_total += static_cast<double>(hc[static_cast<size_t>(i) * 24]);
+        if (step_n + i < nsteps) pushed_total += static_cast<double>(hc[static_cast<size_t>(i) * 24 + 21]);
+    }
+    const size_t li = static_cast<size_t>(nrun - 1);
+    particle_slices = static_cast<double>(hc[li * 24]);
+    for (int k = 0; k < QZ_NPHASE; ++k) phase_ticks[k] = static_cast<double>(hc[li * 24 + 1 + k]);
+    beam_deposited = static_cast<double>(hc[li * 24 + 20]);
+    beam_pushed = static_cast<double>(hc[li * 24 + 21]);
+    lost_last = 0;
+    for (size_t s = 0; s < mob.size(); ++s) lost_last += hl[li * QZ_MAX_SPECIES + s];
+    StepCtx& Lc = *ctx[(nrun - 1) % depth];
+    for (size_t s = 0; s < mob.size(); ++s) {
+        DevBuf<Real>* dst[7] = {&mob[s]->x, &mob[s]->y, &mob[s]->px, &mob[s]->py, &mob[s]->dl, &mob[s]->w, &mob[s]->hist};
+        for (int a = 0; a < 7; ++a)
+            hip_check(hipMemcpyAsync(dst[a]->p, Lc.sp[s][a].p, sizeof(Real) * dst[a]->n, hipMemcpyDeviceToDevice, stream),
+                      "plasma copy");
+    }
+    hip_check(hipMemcpyAsync(bsrc.p, Lc.bsrc.p, sizeof(Real) * bsrc.n, hipMemcpyDeviceToDevice, stream), "bsrc copy");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    ms_sweep = ms_beam = ms_beam_dep = 0;
+    swept = true;
+    for (int i = 0; i < nrun; ++i) {   // Simulation::push_beams and after_sweep, per step
+        if (step_n < nsteps) {
+            t += dt;
+            dt_prev = dt;
+        } else {
+            done = true;
+        }
+        ++step_n;
+    }
+}
+
 // fields_NNNNNN.bin version 3 (src/Simulation.cpp:873-993), all components, one rank
 void Simulation::write_fields(const std::string& fn) {
     std::vector<Real> F, D, B;
@@ -1170,6 +1366,13 @@ extern "C" int qz_sim_autotune(qz_sim* h, int kernel, const int* cand, int ncand
     })
 }
 
+extern "C" int qz_sim_run_pipelined(qz_sim* h, int nrun, int nchunks, int depth, double* out2) {
+    QZ_SIM_TRY({
+        double psl = 0, pushed = 0;
+        h->s->run_pipelined(nrun, nchunks, depth, psl, pushed);
+        if (out2) { out2[0] = psl; out2[1] = pushed; }
+    })
+}
 extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
 extern "C" int qz_sim_push_beams(qz_sim* h) {
     QZ_SIM_TRY({

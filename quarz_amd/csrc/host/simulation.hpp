@@ -109,6 +109,7 @@ struct Beam {
     double gref = 0, xiref = 0;
     int Np = 0;
     DevBuf<Real> a[7];   // x y px py pz xi w
+    DevBuf<int> pstep;   // step that last pushed each particle (pipelined runs)
     void init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st);
     void set_particles(const std::vector<double>& p7, hipStream_t st);
     qz_beam_soa soa() const { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a[6].p}; }
@@ -132,6 +133,10 @@ public:
     bool capturable() const;             // the next step's device work is the same as the last one's
     void after_sweep();      // output, beam push and step counter after the sweep
     void step();             // one iteration of Simulation::run
+    // nrun steps with up to `depth` steps in flight, every step split into nchunks slice ranges (see the
+    // definition); bit-identical to nrun calls of step().  Returns the particle-slices and beam particles
+    // pushed of all nrun steps.
+    void run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total);
     void write_fields(const std::string& path);
 
     bool m1 = false;

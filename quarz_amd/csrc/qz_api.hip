@@ -99,10 +99,13 @@ extern "C" int qz_device_tag(char* out, int n) {
     return QZ_OK;
 }
 
-static int validate_sweep(const qz_sweep_args& a) {
+int qz_validate_sweep(const qz_sweep_args& a) {
     if (a.modes != 0 && a.modes != 1) return qz_fail(QZ_EINVAL, "sweep: modes must be 0 or 1");
     if (a.grid.N < 2 || a.grid.N > 2048) return qz_fail(QZ_ERANGE, "sweep: need 2 <= N <= 2048 radial cells");
     if (a.nslices < 1) return qz_fail(QZ_EINVAL, "sweep: nslices < 1");
+    if (a.k_begin < 0 || a.k_end < 0 || a.k_end > a.nslices || (a.k_end > 0 && a.k_begin >= a.k_end) ||
+        (a.k_end == 0 && a.k_begin >= a.nslices))
+        return qz_fail(QZ_EINVAL, "sweep: slice range [k_begin, k_end) outside [0, nslices)");
     if (a.ab_order < 1 || a.ab_order > 5) return qz_fail(QZ_EINVAL, "sweep: ab_order must be 1..5");
     if (a.nspecies < 0 || a.nspecies > QZ_MAX_SPECIES) return qz_fail(QZ_ERANGE, "sweep: at most 2 mobile species");
     if (a.ncf != (a.modes ? 18 : 6) || a.ncb != (a.modes ? 15 : 5) || a.ncd < (a.modes ? 9 : 3))
@@ -116,7 +119,7 @@ static int validate_sweep(const qz_sweep_args& a) {
 static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
     if (!runs || n < 1) return qz_fail(QZ_EINVAL, "sweep: no runs");
     for (int i = 0; i < n; ++i) {
-        const int rc = validate_sweep(runs[i]);
+        const int rc = qz_validate_sweep(runs[i]);
         if (rc) return rc;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);

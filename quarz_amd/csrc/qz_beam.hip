@@ -19,9 +19,12 @@ namespace qz {
 // per-particle record of the deposit (the node-independent factors of src/Beam.cpp:738-757)
 enum BRec : int { R_WQ = 0, R_TK, R_T, R_OMVZ, R_VZ, R_VR, R_VT, R_C, R_S, R_VX, R_VY, R_N };
 
+// With pstep, only the particles whose gather slice k (after clamping) lies in [s_lo, s_hi) and that are not
+// yet marked with `step` are pushed, and they are marked: a step pushed slice range by slice range pushes
+// every particle once, also those that slip into a later range.
 __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, Real qm, int mode, int i0, int np,
                                     qz_beam_soa P, const Real* __restrict__ fld, int nc, Real dt_kick, Real dt,
-                                    int move) {
+                                    int move, int s_lo, int s_hi, int step, int* __restrict__ pstep) {
     const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     if (P.w[i] == Real(0)) return;
@@ -46,6 +49,10 @@ __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, R
         if (k < 0) { k = 0; s = 0; }
         if (k > nloc - 2) { k = nloc - 2; s = nloc - 1; }
         tk = s - k;
+    }
+    if (pstep) {
+        if (k < s_lo || k >= s_hi || pstep[i] == step) return;
+        pstep[i] = step;
     }
     const int k1 = ngp ? k : k + 1;
     int j; Real t;
@@ -414,7 +421,26 @@ extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* b
     if (np - i0 <= 0) return QZ_OK;
     const int tb = g_block[QZ_K_BEAM_ADVANCE];
     hipLaunchKernelGGL(beam_advance_kernel, dim3((np - i0 + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
-                       *grid, *box, modes == 1 ? 1 : 0, qm, pusher, i0, np, *p, fld, nc, dt_kick, dt, move);
+                       *grid, *box, modes == 1 ? 1 : 0, qm, pusher, i0, np, *p, fld, nc, dt_kick, dt, move, 0, 0, 0,
+                       static_cast<int*>(nullptr));
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
+extern "C" int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm,
+                                      int pusher, int np, const qz_beam_soa* p, const double* fld, int nc,
+                                      double dt_kick, double dt, int move, int s_lo, int s_hi, int step, int* pstep,
+                                      void* stream) {
+    if (!grid || !box || !p || !fld || !pstep) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: null argument");
+    if (np < 0 || grid->N < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: bad extent");
+    if (!box->ngp && box->nloc < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: linear shape needs >= 2 slices");
+    if (nc != (modes == 1 ? F_NC1 : F_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: nc must be 6 or 18");
+    if (pusher < 0 || pusher > 3) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: pusher must be 0..3");
+    if (np == 0 || s_lo >= s_hi) return QZ_OK;
+    const int tb = g_block[QZ_K_BEAM_ADVANCE];
+    hipLaunchKernelGGL(beam_advance_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
+                       *grid, *box, modes == 1 ? 1 : 0, qm, pusher, 0, np, *p, fld, nc, dt_kick, dt, move, s_lo, s_hi,
+                       step, pstep);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
 }

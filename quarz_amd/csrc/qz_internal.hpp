@@ -19,6 +19,7 @@ int qz_fail_hip(hipError_t e, const std::string& where);
 
 size_t qz_sweep_workspace_doubles_impl(int M);
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
+int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_OK or a QZ_E* code
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);
 // zero `bytes` (a multiple of 4) at p with a kernel: unlike a memset node it is safe inside a captured graph
 int qz_zero_async(void* p, size_t bytes, hipStream_t st);

@@ -406,6 +406,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
     Real* __restrict__ ws = A.ws;
     auto WS = [&](int arr) -> Real* { return ws + static_cast<size_t>(arr) * M; };
     const int K = A.nslices;
+    const int kb = A.k_begin, ke = A.k_end > 0 ? A.k_end : K;   // slices of this launch
     const Real dxi = A.dxi;
     const Real idxi = Real(1) / dxi;
     const Stencil stn{A.grad_m, A.grad_0, A.grad_p, A.inv_r, N};
@@ -469,7 +470,8 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         s_r[j] = A.grid.r[j];
         s_iv[j] = A.grid.inv_V[j];
         if (j < N) s_ih[j] = A.grid.inv_h[j];
-        for (int a = W_BPR1; a <= W_BPI2; ++a) WS(a)[j] = Real(0);   // B+ := 0 at the head (Simulation.cpp:705-708)
+        if (kb == 0)
+            for (int a = W_BPR1; a <= W_BPI2; ++a) WS(a)[j] = Real(0);   // B+ := 0 at the head (Simulation.cpp:705-708)
     }
     if (tid == 0) {
         for (int i = 0; i < QZ_NPHASE; ++i) s_ph[i] = 0;
@@ -497,7 +499,7 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         fence_regs(cinvd); fence_regs(cgam);
     };
 
-    for (int k = 0; k < K; ++k) {
+    for (int k = kb; k < ke; ++k) {
         // every per-thread value (addresses, lane roles) is formed inside the slice, not kept live across it
         int tid_k = threadIdx.x;
         asm volatile("" : "+v"(tid_k));

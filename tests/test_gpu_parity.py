@@ -282,6 +282,39 @@ def test_graph_batch_m1_equals_plain(qa, decks):
     _graph_vs_plain(qa, f"{decks}/linear.in", ov, 5, 3)
 
 
+# ------------------------------------------------------------------------------------------ pipelined steps
+@pytest.mark.parametrize("deck,ov,steps,chunks,depth", [
+    ("gain.in", {"time.steps": 4, "driver.nparticles": 20000, "xi.min": -8}, 4, 5, 3),
+    ("gain.in", {"time.steps": 6, "driver.nparticles": 20000, "xi.min": -8, "beams.xi_shape": "ngp"}, 3, 7, 2),
+    ("gain.in", {"time.steps": 3, "driver.nparticles": 10000, "xi.min": -6, "grid.type": "uniform",
+                 "grid.dr": 0.05}, 3, 1, 1),
+    ("linear.in", {"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
+                   "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 4,
+                   "time.dt": 5}, 4, 4, 4),
+])
+def test_pipelined_steps_equal_plain(qa, decks, deck, ov, steps, chunks, depth):
+    """Steps overlapped in time (step n sweeps a slice range once step n-1 has pushed the beam particles that
+    reach it) against the same steps taken one at a time: fields, diagnostics, the last beam sources, every
+    beam particle, the lost count and the time, bit for bit.  The cases end on the last sweep (no push) or
+    before it, with one to four steps in flight and one to seven ranges per step."""
+    a = qa.Simulation(f"{decks}/{deck}", ov)
+    b = qa.Simulation(f"{decks}/{deck}", ov)
+    a.step()
+    b.step()   # the leapfrog start-up step
+    for _ in range(steps):
+        a.step()
+    b.run_pipelined(steps, chunks=chunks, depth=depth)
+    for w in ("fld", "diag", "bsrc"):
+        ga, gb = getattr(b, w), getattr(a, w)
+        assert _same(ga, gb), _report(f"pipelined {w}", ga, gb)
+    for bi in range(a.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            ga, gb = b.get(f"beam{bi}.{k}"), a.get(f"beam{bi}.{k}")
+            assert _same(ga, gb), _report(f"pipelined beam{bi}.{k}", ga, gb)
+    assert b.lost() == a.lost()
+    assert b.t == a.t
+
+
 # ------------------------------------------------------------------------------------------ launch shapes
 def _state(qa, sim):
     out = {w: getattr(sim, w) for w in ("fld", "diag", "bsrc")}
