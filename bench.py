@@ -3,9 +3,12 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2] [--no-cpu]
 
 A step is one quasi-static time step of the workload (quarz_amd/configs.py): beam deposit, the plasma sweep
-over every slice (one persistent kernel), the beam push.  W untimed steps, then exactly K timed steps between
-device synchronisations.  Prints one JSON line:
+over every slice (one persistent kernel), the beam push.  W untimed steps one at a time (at least one: the
+first starts the leapfrog), then exactly K timed steps between device synchronisations.  The timed steps run
+pipelined (Simulation.run_pipelined: up to --depth steps in flight, each split into --chunks slice ranges,
+bit-identical to stepping one at a time); --depth 0 steps them one at a time.  Prints one JSON line:
   value          plasma particle-slices per second of the whole timed steps (wall clock)
+  plain_value    the same run stepped one step at a time (--plain-steps steps), for comparison
   beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
   roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops
                  679 per particle-slice + 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP
@@ -27,6 +30,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# one hardware queue per step in flight: the pipelined run keeps up to 32 steps going, HIP's default is 4
+# queues per process (read when HIP starts, so before torch is imported); never above 32
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 32:
+    os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 METRIC = "plasma particle-slices/s & beam pushes/s (FP64, 1 GPU); batched vs plain"
 FP64_PEAK = 78.6e12
@@ -142,12 +149,15 @@ def bench_batched(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20, help="timed steps (c2: the workload's 20 time steps)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--runs", type=int, default=256, help="c5: number of independent runs")
     ap.add_argument("--plain-steps", type=int, default=1, help="c5: steps of the one-launch-per-run comparison")
+    ap.add_argument("--depth", type=int, default=-1,
+                    help="time steps in flight (-1: min(steps, 32); 0: one step at a time, no pipeline)")
+    ap.add_argument("--chunks", type=int, default=128, help="slice ranges per step of the pipelined run")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
@@ -162,27 +172,55 @@ def main():
     import quarz_amd
     from quarz_amd import configs
     deck = configs.deck(args.config)
-    total = args.warmup + args.steps + 1
+    pipelined = args.depth != 0
+    warmup = max(args.warmup, 1) if pipelined else args.warmup   # the pipeline continues a started run
+    depth = min(args.steps, 32) if args.depth < 0 else args.depth
+    total = warmup + args.steps + 1
     sim = quarz_amd.Simulation(deck, {"time.steps": total})
     modes = sim.m1
     shapes = sim.autotune(cache=args.tune_cache)   # launch shapes of the beam kernels for this run
-    for _ in range(args.warmup):
+    # warm-up steps, one at a time (the first starts the leapfrog); the last one's HIP-event kernel times
+    # give the per-launch figures (sweep, beam deposit, beam push)
+    eager = None
+    for _ in range(warmup):
         sim.step()
+        eager = sim.stats()
     torch.cuda.synchronize()
     psl = pushes = dep_particles = 0.0
     sweep_ms = beam_ms = dep_ms = 0.0
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        sim.step()
-        st = sim.stats()
-        psl += st["particle_slices"]
-        pushes += st["beam_pushed"]
-        dep_particles += st["beam_deposited"]
-        sweep_ms += st["sweep_ms"]
-        beam_ms += st["beam_push_ms"]
-        dep_ms += st["beam_dep_ms"]
+    if pipelined:
+        psl, pushes = sim.run_pipelined(args.steps, chunks=args.chunks, depth=depth)
+        dep_particles = sim.stats()["beam_deposited"] * args.steps
+    else:
+        for _ in range(args.steps):
+            sim.step()
+            st = sim.stats()
+            psl += st["particle_slices"]
+            pushes += st["beam_pushed"]
+            dep_particles += st["beam_deposited"]
+            sweep_ms += st["sweep_ms"]
+            beam_ms += st["beam_push_ms"]
+            dep_ms += st["beam_dep_ms"]
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    if pipelined:   # per-launch kernel times from the last eager step
+        sweep_ms = eager["sweep_ms"] * args.steps
+        beam_ms = eager["beam_push_ms"] * args.steps
+        dep_ms = eager["beam_dep_ms"] * args.steps
+    # the same run stepped one step at a time, for comparison
+    plain_value = None
+    if pipelined and args.plain_steps > 0:
+        ps = quarz_amd.Simulation(deck, {"time.steps": args.plain_steps + 2})
+        ps.step()
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        ppsl = 0.0
+        for _ in range(args.plain_steps):
+            ps.step()
+            ppsl += ps.stats()["particle_slices"]
+        torch.cuda.synchronize()
+        plain_value = ppsl / (time.perf_counter() - tp)
     fp, fn = FLOPS[modes]
     flops_per_sweep = fp * psl / args.steps + fn * sim.M * sim.K
     t_kernel = sweep_ms / args.steps / 1e3
@@ -193,7 +231,7 @@ def main():
         "unit": "particle-slices/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
@@ -202,7 +240,10 @@ def main():
         "data": "synthetic: beams sampled from the deck's seeds, plasma loaded at the box head",
         "config": {"workload": f"{args.config}: {configs.DESCRIPTION[args.config]}", "M": sim.M, "slices": sim.K,
                    "plasma_particles_per_slice": int(round(psl / args.steps / max(sim.K, 1))),
-                   "beam_particles": int(round(pushes / args.steps)), "parallelism": "one run per GPU"},
+                   "beam_particles": int(round(pushes / args.steps)),
+                   "parallelism": (f"one run per GPU; {depth} time steps in flight, {args.chunks} slice ranges per step"
+                                   if pipelined else "one run per GPU, one step at a time")},
+        "plain_value": plain_value,
         "beam_pushes_per_s": pushes / wall,
         "sweep_ms": sweep_ms / args.steps,
         "beam_deposit_ms": dep_ms / args.steps,
@@ -213,7 +254,12 @@ def main():
                      "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
                      "kernel": "sweep_kernel",
                      "flops_per_launch": flops_per_sweep,
-                     "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
+                     "aggregate_achieved": flops_per_sweep * args.steps / wall / 1e12,
+                     "aggregate_frac": flops_per_sweep * args.steps / wall / FP64_PEAK,
+                     "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s. "
+                             "achieved: one sweep launch (HIP events on the run's stream, last warm-up step); "
+                             "aggregate_achieved: every sweep of the timed steps over their wall time (pipelined "
+                             "steps overlap on several CUs)"},
     }
     nb_dep, nb_push = dep_particles / args.steps, pushes / args.steps
     ncf, ncb = (18, 15) if modes else (6, 5)
