@@ -315,6 +315,22 @@ def test_pipelined_steps_equal_plain(qa, decks, deck, ov, steps, chunks, depth):
     assert b.t == a.t
 
 
+def test_pipelined_betatron_m1_equals_oracle(qa, oracle, decks):
+    """Pipelined steps against the serial oracle directly: the betatron deck (m = 1, offset witness), the
+    start-up step, then 10 steps with 5 in flight over 6 slice ranges, ending on the last sweep."""
+    ov = {"modes": 1, "witness.x0": 0.05, "witness.nsym": 4, "time.steps": 10}
+    g = qa.Simulation(f"{decks}/betatron.in", ov)
+    o = oracle.Sim(f"{decks}/betatron.in", ov)
+    g.step()
+    for _ in range(11):
+        o.step()
+    g.run_pipelined(10, chunks=6, depth=5)
+    _compare_sims(g, o)
+    for k in qa.BEAM_ARRAYS:
+        a, b = g.get(f"beam0.{k}"), o.get(f"beam0.{k}")
+        assert _same(a, b), _report(f"witness {k}", a, b)
+
+
 # ------------------------------------------------------------------------------------------ launch shapes
 def _state(qa, sim):
     out = {w: getattr(sim, w) for w in ("fld", "diag", "bsrc")}
