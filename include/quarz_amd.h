@@ -147,7 +147,9 @@ int qz_beam_sums_device(int np, const qz_beam_soa* p, double gref, double xiref,
 
 /* One quasi-static sweep over all slices of one run: per slice the mode deposit, the psi / E_z / B_z and
  * B+ solves, the field store and the Adams-Bashforth push, in one persistent kernel
- * (Simulation::step_fields slice loop, src/Simulation.cpp:727-783).  args points to host memory. */
+ * (Simulation::step_fields slice loop, src/Simulation.cpp:727-783).  args points to host memory; the
+ * argument blocks are staged in a device buffer kept per (device, stream), allocated on a stream's first
+ * call (so that call is not capturable in a graph). */
 size_t qz_sweep_workspace_doubles(int M);
 int qz_sweep(const qz_sweep_args* args, void* stream);
 

@@ -4,6 +4,7 @@
 #include <cstdio>
 
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -116,6 +117,19 @@ int qz_validate_sweep(const qz_sweep_args& a) {
     return QZ_OK;
 }
 
+// The kernel reads its argument blocks from device memory for its whole duration.  They are staged in one
+// buffer per (device, stream), reused only by later calls on the same stream, which the stream orders after
+// the kernels that read it.  (A stream-ordered allocation per call could be handed to another stream while a
+// kernel still reads it.)  The lock keeps the copy and the launch of one call together.
+namespace {
+struct ArgsBuf {
+    qz_sweep_args* p = nullptr;
+    int cap = 0;
+};
+std::mutex g_args_mu;
+std::map<std::pair<int, hipStream_t>, ArgsBuf> g_args;
+}  // namespace
+
 static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
     if (!runs || n < 1) return qz_fail(QZ_EINVAL, "sweep: no runs");
     for (int i = 0; i < n; ++i) {
@@ -123,12 +137,22 @@ static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
         if (rc) return rc;
     }
     hipStream_t st = static_cast<hipStream_t>(stream);
-    qz_sweep_args* d = nullptr;
-    QZ_HIP(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(qz_sweep_args) * n, st));
-    QZ_HIP(hipMemcpyAsync(d, runs, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st));
-    const int rc = qz_sweep_launch(d, runs, n, st);
-    QZ_HIP(hipFreeAsync(d, st));
-    return rc;
+    int dev = 0;
+    QZ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_args_mu);
+    ArgsBuf& b = g_args[{dev, st}];
+    if (b.cap < n) {   // grow: the old buffer may still be read by this stream's kernels
+        if (b.p) {
+            QZ_HIP(hipStreamSynchronize(st));
+            QZ_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        QZ_HIP(hipMalloc(reinterpret_cast<void**>(&b.p), sizeof(qz_sweep_args) * n));
+        b.cap = n;
+    }
+    QZ_HIP(hipMemcpyAsync(b.p, runs, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st));
+    return qz_sweep_launch(b.p, runs, n, st);
 }
 
 extern "C" int qz_sweep(const qz_sweep_args* args, void* stream) { return sweep_many(args, 1, stream); }
