@@ -829,7 +829,9 @@ void Simulation::after_sweep() {
 //  - the deposit of a row range is the full deposit restricted to those rows (node sums in particle-index
 //    order), and step n overwrites rows of fld only after step n-1's pushes have read them.
 // Every step in flight has its own stream, plasma arrays, workspace, bsrc and counters (a context); beam
-// particles, fld and diag are shared.  Events order the ranges: deposit(n, c) waits for push(n-1, c+1).
+// particles, fld and diag are shared.  Events order the ranges: deposit(n, c) waits for push(n-1, c+1), which
+// pushes the particles that can land in rows up to cb[c+1] and follows the sweep of range c+1 of step n-1
+// (so step n runs two ranges behind step n-1, and step n-1 has read the rows step n overwrites).
 // The sweep launches read their argument blocks from one device array made before the first launch: a
 // stream-ordered allocation per launch (as qz_sweep makes) could be handed from one stream to another while
 // a kernel still reads it.
@@ -937,8 +939,10 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             qz_check(qz_zero_async(C.sp[s][6].p, sizeof(Real) * C.sp[s][6].n, st));
         }
         unsigned long long* cnt = cntv.p + static_cast<size_t>(i) * 24;
+        // push range c: gather slices [cb[c] - 1, cb[c+1] - 1) (from 0 for the first, to K for the last), whose
+        // stencil k, k+1 lies in slice ranges <= c; the range's last slice goes with the next push
         auto push_range = [&](int c) {
-            const int lo = cb[c], hi = c == nchunks - 1 ? K : cb[c + 1];
+            const int lo = c == 0 ? 0 : cb[c] - 1, hi = c == nchunks - 1 ? K : cb[c + 1] - 1;
             for (auto& b : beams) {
                 if (b->rigid) continue;
                 const qz_beam_soa so = b->soa();
@@ -969,9 +973,8 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             }
             const size_t ai = static_cast<size_t>(i) * nchunks + c;
             qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
-            if (!last && c > 0) push_range(c - 1);   // its last slice's gather reads row c0, just stored
+            if (!last) push_range(c);
         }
-        if (!last) push_range(nchunks - 1);
     }
     for (auto& cp : ctx) hip_check(hipStreamSynchronize(cp->st), "sync");
 
