@@ -944,7 +944,7 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         auto push_range = [&](int c) {
             const int lo = c == 0 ? 0 : cb[c] - 1, hi = c == nchunks - 1 ? K : cb[c + 1] - 1;
             for (auto& b : beams) {
-                if (b->rigid) continue;
+                if (b->rigid || b->Np == 0) continue;
                 const qz_beam_soa so = b->soa();
                 qz_check(qz_beam_advance_slices(&g, &box, mm, b->q / b->m, b->pusher, b->Np, &so, fld.p, NCF, dkick, ddrift,
                                                 1, lo, hi, marker, b->pstep.p, st));
