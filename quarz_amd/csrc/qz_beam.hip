@@ -129,18 +129,19 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     key[i] = -1;
+    // the slice window first: a deposit restricted to a few rows reads only xi of the other particles
+    const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
+    const int k0 = bg.k0, nloc = bg.nloc;
+    const bool ngp = bg.ngp != 0;
+    const Real sx = (P.xi[i] - xmin) * idxi;
+    const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
+    if (k < k0 - 1 || k >= k0 + nloc) return;
     const Real wi = P.w[i];
     if (wi == Real(0)) return;
     const Real X = P.x[i], Y = P.y[i];
     const Real r = dsqrt(X * X + Y * Y);
     if (r > g.R) return;
-    const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
-    const int k0 = bg.k0, nloc = bg.nloc;
-    const bool ngp = bg.ngp != 0;
     const Real c = r > Real(0) ? X / r : Real(1), s = r > Real(0) ? Y / r : Real(0);
-    const Real sx = (P.xi[i] - xmin) * idxi;
-    const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
-    if (k < k0 - 1 || k >= k0 + nloc) return;
     const Real tk = ngp ? Real(0) : sx - k;
     const Real px = P.px[i], py = P.py[i], pz = P.pz[i];
     const Real p2 = px * px + py * py;
