@@ -181,6 +181,14 @@ int qz_sim_run_pipelined(qz_sim* s, int nrun, int nchunks, int depth, double* ou
 long qz_sim_get(qz_sim* s, const char* what, double* host_out, long cap);
 int qz_sim_beam_diag(qz_sim* s, int b, double* out15);
 int qz_sim_beam_sums(qz_sim* s, int b, double* out17, double* refs2);
+/* No counterpart in the reference: the differentiable workload (a witness whose current follows a
+ * piecewise-linear profile, deck key <beam>.pwl_weights on a flat-top beam).  set_beam_profile gives beam b
+ * new profile values (nseg of them, positive): every macro-particle's weight becomes the flat-top weight
+ * times the profile at the xi it was sampled at.  snapshot keeps the beams and the step state as they are;
+ * rewind returns to them, so one step can be evaluated again with other profile values. */
+int qz_sim_set_beam_profile(qz_sim* s, int b, const double* values, int nseg);
+int qz_sim_snapshot(qz_sim* s);
+int qz_sim_rewind(qz_sim* s);
 /* times of the last sweep and beam push (ms, HIP events on the simulation's stream) */
 int qz_sim_timing(const qz_sim* s, double* out2);
 /* work of the last step: particle-slices of the sweep, beam particles pushed, beam particles deposited,

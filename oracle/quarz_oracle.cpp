@@ -1273,6 +1273,22 @@ struct Beam {
             wtot = n0 * 2 * PI * sr * sr * Pr * (hi - lo);
         }
         const double wmac = wtot / Np;
+        // build-defined piecewise-linear weight profile of a flat-top beam (the differentiable workload's
+        // witness; not in the reference): weight = wmac * profile(user xi), the profile's values at the
+        // centres of equal segments of the flat top, tail first, linear between centres, flat beyond them
+        std::vector<double> pwl;
+        for (const auto& tok : cfg.get_list(p + ".pwl_weights")) pwl.push_back(cfg.number(tok, p + ".pwl_weights"));
+        if (!pwl.empty() && (lprof != "flattop" || nsym != 1)) throw std::runtime_error(p + ".pwl_weights: flattop, nsym 1");
+        const double pu0 = xi_to_user(xi0 + 0.5 * len), pu1 = xi_to_user(xi0 - 0.5 * len);
+        auto profile = [&](double u) {
+            const int S = static_cast<int>(pwl.size());
+            const double tt = (u - pu0) / (pu1 - pu0) * S - 0.5;
+            if (!(tt > 0)) return pwl[0];
+            if (tt >= S - 1) return pwl[S - 1];
+            const int sg = static_cast<int>(tt);
+            const double f = tt - sg;
+            return pwl[sg] + (pwl[sg + 1] - pwl[sg]) * f;
+        };
         std::mt19937_64 rng(seed);
         std::normal_distribution<double> N01(0.0, 1.0);
         std::uniform_real_distribution<double> U01(0.0, 1.0);
@@ -1302,7 +1318,7 @@ struct Beam {
                 const double ph = 2 * PI * lr / nrot, cp = std::cos(ph), sp = std::sin(ph);
                 const double qv[7] = {cp * xx - sp * yy + cx0 + cxs * (xv - xi0), sgn * (sp * xx + cp * yy) + cy0 + cys * (xv - xi0),
                                       cp * tpx - sp * tpy + gam * xp0, sgn * (sp * tpx + cp * tpy) + gam * yp0,
-                                      std::sqrt(pz2), xv, wmac};
+                                      std::sqrt(pz2), xv, pwl.empty() ? wmac : wmac * profile(xi_to_user(xv))};
                 // Beam::accumulate (src/Beam.cpp:580-588)
                 if (qv[6] != 0) {
                     const double gg = std::sqrt(1 + qv[2] * qv[2] + qv[3] * qv[3] + qv[4] * qv[4]);

@@ -91,6 +91,9 @@ def lib():
         L.qz_sim_time.restype = C.c_double
         L.qz_sim_time.argtypes = [C.c_void_p]
         L.qz_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
+        L.qz_sim_set_beam_profile.argtypes = [C.c_void_p, C.c_int, DP, C.c_int]
+        L.qz_sim_snapshot.argtypes = [C.c_void_p]
+        L.qz_sim_rewind.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -253,6 +256,20 @@ class Simulation:
 
     def write_fields(self, path):
         check(lib().qz_sim_write_fields(self.h, path.encode()))
+
+    def set_beam_profile(self, b, values):
+        """New values of beam b's piecewise-linear current profile (deck key <beam>.pwl_weights): every
+        macro-particle's weight becomes the flat-top weight times the profile at its sampled xi."""
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().qz_sim_set_beam_profile(self.h, b, v.ctypes.data_as(DP), v.size))
+
+    def snapshot(self):
+        """Keep the beams and the step state as they are now (see rewind)."""
+        check(lib().qz_sim_snapshot(self.h))
+
+    def rewind(self):
+        """Return the beams and the step state to the last snapshot."""
+        check(lib().qz_sim_rewind(self.h))
 
 
 def step_batched(sims):

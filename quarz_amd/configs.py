@@ -5,6 +5,9 @@ c1  m = 0 blowout, uniform 256 cells to R = 8, ppc 4, xi in [-16, 0] step 0.01 (
 c2  m = 0 + 1, stretched 512 cells to R = 10 (5e-4 at the axis, 0.05 at the wall), 8 azimuths per ring,
     xi in [-20, 0] step 0.005 (4001 slices), offset driver (1e6 particles) and witness (5e5), Picard 3,
     dt 10, 20 steps.
+c4  m = 0, the c2 node list (512 stretched cells to R = 10), xi in [-19.99, 0] step 0.01 (2000 slices), the c1
+    driver with 5e5 particles, and a flat-top witness over xi in [-10, -6] (2e5 particles, linear deposit)
+    whose current follows a 16-value piecewise-linear profile, the parameters of the differentiable step.
 Driver parameters a workload leaves open (gamma, xi0, energy spread, emittance) are the c1 values.
 """
 import os
@@ -107,6 +110,57 @@ witness.cut = 3
 witness.nparticles = 500000
 """
 
+# c4: the differentiable step.  The witness's radial size, density scale and emittance (n0 1 at profile value
+# 1, sigma_r 0.1, emit 1e-3, no energy spread) are this workload's choices; two time steps so that the first
+# step pushes the beams.
+_C4 = """
+xi.convention = z-t
+modes = 0
+grid.type = file
+grid.file = {grid_file}
+xi.min = -19.99
+xi.max = 0
+xi.step = 0.01
+time.dt = 20
+time.steps = 2
+pusher.beam = vay
+pusher.ab_order = 3
+plasma.species = electrons
+electrons.ppc = 4
+beams = driver witness
+driver.n0 = 2
+driver.sigma_r = 0.3
+driver.sigma_xi = 1.0
+driver.xi0 = -2
+driver.gamma = 2000
+driver.espread = 0.01
+driver.emit_n = 0.002
+driver.cut = 3
+driver.nparticles = {nd}
+driver.seed = 12345
+witness.profile = flattop
+witness.length = 4
+witness.xi0 = -8
+witness.n0 = 1
+witness.sigma_r = 0.1
+witness.gamma = 2000
+witness.emit_n = 0.001
+witness.cut = 3
+witness.nparticles = {nw}
+witness.seed = 777
+witness.pwl_weights = {pwl}
+"""
+
+C4_SEGMENTS = 16
+
+
+def c4_initial_profile(seed=2026, nseg=C4_SEGMENTS):
+    """The witness profile's starting values: uniform in [0.5, 1.5] from `seed` (Python's Mersenne Twister)."""
+    import random
+    rng = random.Random(seed)
+    return [rng.uniform(0.5, 1.5) for _ in range(nseg)]
+
+
 # c5: many independent m = 0 runs for the batched launch.  Per run (seeded): N from {128, 256, 384, 512, 768,
 # 1024} stretched cells from 1e-3 at the axis to 0.05 at the wall (R = 8, or the largest R a growth ratio of
 # 1.05 reaches with N cells, 3.37 for N = 128), K uniform in [1000, 8000] slices at dxi = 0.01, driver n_b
@@ -184,6 +238,8 @@ DESCRIPTION = {
     "c1": "m0 blowout, uniform 256 cells R=8, ppc4, 1601 slices, driver 1e5, vay, dt20",
     "c2": "m0+1 stretched 512 cells (5e-4..0.05) R=10, 8 azimuths, 4001 slices, driver 1e6 + witness 5e5, "
           "picard 3, vay, dt10",
+    "c4": "m0 stretched 512 cells (5e-4..0.05) R=10, ppc4, 2000 slices, driver 5e5 + flat-top witness 2e5 "
+          "with a 16-value piecewise-linear current profile, vay, dt20",
     "c5": "256 independent m0 runs, N in {128..1024} stretched (1e-3..0.05), K in [1000, 8000], "
           "driver n_b in [0.5, 4] + witness, 5e4..5e5 beam particles, ppc 4, one batched launch per step",
 }
@@ -201,4 +257,18 @@ def deck(name, workdir=None):
             f.write("# 512 cells, 5e-4 at the axis, 0.05 at the wall, R = 10\n")
             f.writelines(f"{x:.17g}\n" for x in nodes)
         return _C2.format(grid_file=path)
+    if name == "c4":
+        return c4_deck(workdir=workdir)
     raise KeyError(name)
+
+
+def c4_deck(profile=None, nd=500000, nw=200000, workdir=None):
+    """Deck of the differentiable step; `profile` defaults to c4_initial_profile(); smaller beams for tests."""
+    workdir = workdir or tempfile.mkdtemp(prefix="quarz_amd_")
+    nodes, _ = stretched_nodes(512, 10.0, 5e-4, 0.05)
+    path = os.path.join(workdir, "grid_c2.txt")
+    with open(path, "w") as f:
+        f.write("# 512 cells, 5e-4 at the axis, 0.05 at the wall, R = 10\n")
+        f.writelines(f"{x:.17g}\n" for x in nodes)
+    prof = c4_initial_profile() if profile is None else list(profile)
+    return _C4.format(grid_file=path, nd=nd, nw=nw, pwl=" ".join(f"{v:.17g}" for v in prof))

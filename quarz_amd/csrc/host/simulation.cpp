@@ -419,6 +419,19 @@ void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_
         wtot = n0 * 2 * PI * sr * sr * Pr * (hi - lo);
     }
     const double wmac = wtot / Ntot;
+    // a piecewise-linear current profile over a flat-top beam (the differentiable workload's witness): the
+    // weight of a macro-particle at xi is wmac times the profile there
+    pwl.clear();
+    xi_init.clear();
+    g_init.clear();
+    for (const auto& tok : cfg.get_list(p + ".pwl_weights")) pwl.push_back(cfg.number(tok, p + ".pwl_weights"));
+    if (!pwl.empty()) {
+        if (lprof != "flattop") throw std::runtime_error(p + ".pwl_weights needs profile = flattop");
+        if (nsym != 1) throw std::runtime_error(p + ".pwl_weights needs nsym = 1 and no mirror_y");
+        pwl_u0 = xi_to_user(xi0 + 0.5 * len);
+        pwl_u1 = xi_to_user(xi0 - 0.5 * len);
+        pwl_wmac = wmac;
+    }
     std::mt19937_64 rng(sd);
     std::normal_distribution<double> N01(0.0, 1.0);
     std::uniform_real_distribution<double> U01(0.0, 1.0);
@@ -451,9 +464,10 @@ void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_
             const double ph = 2 * PI * lr / nrot, cp = std::cos(ph), sp = std::sin(ph);
             const double rec[7] = {cp * xx - sp * yy + cx0 + cxs * (xv - xi0), sgn * (sp * xx + cp * yy) + cy0 + cys * (xv - xi0),
                                    cp * tpx - sp * tpy + gam * xp0, sgn * (sp * tpx + cp * tpy) + gam * yp0,
-                                   std::sqrt(pz2), xv, wmac};
-            if (rec[6] != 0) {   // reference values of the moments (Beam::accumulate, :580-588)
+                                   std::sqrt(pz2), xv, pwl.empty() ? wmac : wmac * pwl_factor(pwl, pwl_u0, pwl_u1, xi_to_user(xv))};
+            if (rec[6] != 0 || !pwl.empty()) {   // reference values of the moments (Beam::accumulate, :580-588)
                 const double gg = std::sqrt(1 + rec[2] * rec[2] + rec[3] * rec[3] + rec[4] * rec[4]);
+                if (!pwl.empty()) { xi_init.push_back(xv); g_init.push_back(gg); }
                 aw += rec[6];
                 awg += rec[6] * gg;
                 awxi += rec[6] * rec[5];
@@ -464,6 +478,40 @@ void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_
     set_particles(all, st);
     gref = 0; xiref = 0;
     if (aw > 0) { gref = awg / aw; xiref = awxi / aw; }
+}
+
+// The profile's values sit at the centres of nseg equal segments of [u0, u1] (user xi, tail first); the
+// profile is linear between neighbouring centres and constant beyond the first and the last centre.
+double pwl_factor(const std::vector<double>& v, double u0, double u1, double u) {
+    const int S = static_cast<int>(v.size());
+    const double t = (u - u0) / (u1 - u0) * S - 0.5;
+    if (!(t > 0)) return v[0];
+    if (t >= S - 1) return v[S - 1];
+    const int sg = static_cast<int>(t);
+    const double f = t - sg;
+    return v[sg] + (v[sg + 1] - v[sg]) * f;
+}
+
+void Beam::set_profile(const std::vector<double>& v, hipStream_t st) {
+    if (pwl.empty()) throw std::runtime_error(name + ": no piecewise-linear profile (pwl_weights)");
+    if (v.size() != pwl.size()) throw std::runtime_error(name + ": profile needs " + std::to_string(pwl.size()) + " values");
+    for (double x : v)
+        if (!(x > 0) || !std::isfinite(x)) throw std::runtime_error(name + ": profile values must be positive and finite");
+    pwl = v;
+    std::vector<double> w(xi_init.size());
+    double aw = 0, awg = 0, awxi = 0;   // the moments' reference values, as init makes them
+    for (size_t i = 0; i < w.size(); ++i) {
+        w[i] = pwl_wmac * pwl_factor(pwl, pwl_u0, pwl_u1, xi_to_user(xi_init[i]));
+        if (w[i] != 0) {
+            aw += w[i];
+            awg += w[i] * g_init[i];
+            awxi += w[i] * xi_init[i];
+        }
+    }
+    gref = 0; xiref = 0;
+    if (aw > 0) { gref = awg / aw; xiref = awxi / aw; }
+    a[6].upload(w, st);
+    hip_check(hipStreamSynchronize(st), "sync");
 }
 
 void Beam::set_particles(const std::vector<double>& p7, hipStream_t st) {   // :1323-1335
@@ -1090,6 +1138,44 @@ void Simulation::write_fields(const std::string& fn) {
     std::fclose(fp);
 }
 
+void Simulation::snapshot() {
+    auto sn = std::make_unique<Snap>();
+    sn->beams.resize(beams.size());
+    for (size_t b = 0; b < beams.size(); ++b) {
+        for (int c = 0; c < 7; ++c) {
+            const DevBuf<Real>& a = beams[b]->a[c];
+            sn->beams[b][c].alloc(a.n);
+            if (a.n) hip_check(hipMemcpyAsync(sn->beams[b][c].p, a.p, sizeof(Real) * a.n, hipMemcpyDeviceToDevice, stream), "snapshot");
+        }
+        sn->started.push_back(beams[b]->started ? 1 : 0);
+        sn->refs.push_back({beams[b]->gref, beams[b]->xiref});
+        sn->pwl.push_back(beams[b]->pwl);
+    }
+    sn->t = t; sn->step_n = step_n; sn->done = done; sn->have_prev = have_prev; sn->swept = swept; sn->dt_prev = dt_prev;
+    hip_check(hipStreamSynchronize(stream), "sync");
+    snap = std::move(sn);
+}
+
+void Simulation::rewind() {
+    if (!snap) throw std::runtime_error("rewind: no snapshot");
+    for (size_t b = 0; b < beams.size(); ++b) {
+        Beam& B = *beams[b];
+        for (int c = 0; c < 7; ++c) {
+            const DevBuf<Real>& a = snap->beams[b][c];
+            if (a.n != B.a[c].n) throw std::runtime_error("rewind: beam arrays changed size");
+            if (a.n) hip_check(hipMemcpyAsync(B.a[c].p, a.p, sizeof(Real) * a.n, hipMemcpyDeviceToDevice, stream), "rewind");
+        }
+        B.started = snap->started[b] != 0;
+        B.gref = snap->refs[b][0];
+        B.xiref = snap->refs[b][1];
+        B.pwl = snap->pwl[b];
+        B.pstep.release();   // the pipelined step markers restart with the step count
+    }
+    t = snap->t; step_n = snap->step_n; done = snap->done; have_prev = snap->have_prev; swept = snap->swept;
+    dt_prev = snap->dt_prev;
+    hip_check(hipStreamSynchronize(stream), "sync");
+}
+
 }  // namespace qz
 
 // ============================================================================ qz_sim_* C-ABI
@@ -1425,6 +1511,22 @@ extern "C" long qz_sim_get(qz_sim* h, const char* what, double* out, long cap) {
         qz_fail(QZ_EINVAL, e.what());
         return -1;
     }
+}
+extern "C" int qz_sim_set_beam_profile(qz_sim* h, int b, const double* values, int nseg) {
+    if (!h || !values || nseg < 1) return qz_fail(QZ_EINVAL, "qz_sim_set_beam_profile: bad argument");
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
+        s.beams[b]->set_profile(std::vector<double>(values, values + nseg), s.stream);
+    })
+}
+extern "C" int qz_sim_snapshot(qz_sim* h) {
+    if (!h) return qz_fail(QZ_EINVAL, "qz_sim_snapshot: null run");
+    QZ_SIM_TRY(h->s->snapshot())
+}
+extern "C" int qz_sim_rewind(qz_sim* h) {
+    if (!h) return qz_fail(QZ_EINVAL, "qz_sim_rewind: null run");
+    QZ_SIM_TRY(h->s->rewind())
 }
 extern "C" int qz_sim_beam_sums(qz_sim* h, int b, double* out17, double* refs) {
     QZ_SIM_TRY({

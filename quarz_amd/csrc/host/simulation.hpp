@@ -93,6 +93,8 @@ struct PlasmaSpecies {
 };
 
 // ---------------------------------------------------------------------------- Beam (src/Beam.*)
+double pwl_factor(const std::vector<double>& v, double u0, double u1, double u);
+
 struct BeamDiag {
     double t = 0, npart = 0, charge = 0, gamma_mean = 0, gamma_rms = 0, xi_mean = 0, xi_rms = 0;
     double r_rms = 0, emit_nx = 0, emit_ny = 0, x_mean = 0, y_mean = 0, sigma_x = 0, sigma_y = 0;
@@ -110,7 +112,12 @@ struct Beam {
     int Np = 0;
     DevBuf<Real> a[7];   // x y px py pz xi w
     DevBuf<int> pstep;   // step that last pushed each particle (pipelined runs)
+    // piecewise-linear weight profile of a flat-top beam (<name>.pwl_weights): values, user-xi range, the
+    // flat-top macro-particle weight, and the sampled xi and gamma of every particle
+    std::vector<double> pwl, xi_init, g_init;
+    double pwl_u0 = 0, pwl_u1 = 0, pwl_wmac = 0;
     void init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st);
+    void set_profile(const std::vector<double>& values, hipStream_t st);   // new weights from the profile
     void set_particles(const std::vector<double>& p7, hipStream_t st);
     qz_beam_soa soa() const { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a[6].p}; }
     std::array<double, 17> sums(hipStream_t st) const;
@@ -138,6 +145,9 @@ public:
     // pushed of all nrun steps.
     void run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total);
     void write_fields(const std::string& path);
+    // the beams and the step state as they are now, and back to them (repeated evaluations of one step)
+    void snapshot();
+    void rewind();
 
     bool m1 = false;
     int picard = 1;
@@ -168,6 +178,17 @@ public:
     hipEvent_t ev[6] = {};
     std::string outdir;
     int out_every = 0;
+    struct Snap {
+        std::vector<std::array<DevBuf<Real>, 7>> beams;
+        std::vector<char> started;
+        std::vector<std::array<double, 2>> refs;     // gref, xiref
+        std::vector<std::vector<double>> pwl;
+        Real t = 0;
+        int step_n = 0;
+        bool done = false, have_prev = false, swept = false;
+        double dt_prev = 0;
+    };
+    std::unique_ptr<Snap> snap;
 };
 
 }  // namespace qz
