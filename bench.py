@@ -229,7 +229,8 @@ def bench_gradient(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20, help="timed steps (c2: the workload's 20 time steps)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (c2: the workload's 20 time steps; c4: 50 Adam iterations; c5: 20)")
     ap.add_argument("--warmup", type=int, default=2)   # the second step is a regular one: its kernel times are reported
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
@@ -241,6 +242,8 @@ def main():
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 50 if args.config == "c4" else 20
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
               f"see DESIGN.md for how the path shards)", file=sys.stderr)

@@ -73,7 +73,13 @@ class SpreadObjective:
         return J[0], (J[1::2] - J[2::2]) / (2 * eps)
 
 
+_FN = None
+
+
 def _torch_function():
+    global _FN
+    if _FN is not None:
+        return _FN
     import torch
 
     class SpreadFunction(torch.autograd.Function):
@@ -90,7 +96,8 @@ def _torch_function():
             (g,) = ctx.saved_tensors
             return gout * g, None, None
 
-    return SpreadFunction
+    _FN = SpreadFunction
+    return _FN
 
 
 def spread(v, objective, eps=1e-6):
