@@ -1,4 +1,5 @@
 #include "simulation.hpp"
+#include "../qz_device.hpp"   // component order of fld / diag / bsrc
 
 #include <algorithm>
 #include <cmath>
@@ -1066,53 +1067,35 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
 }
 
 // fields_NNNNNN.bin version 3 (src/Simulation.cpp:873-993), all components, one rank
+// fields_*.bin, format v3 (src/Simulation.cpp:873-993): header, r[M], xi[K] increasing (head last), then per
+// component its name[16] and [K][M] doubles; with m = 1 the _c / _s parts are 2 Re / -2 Im of the mode (vectors:
+// e_r and e_theta from the +1/-1 helical parts, :902-928).  Streamed: each component is formed in HBM from fld,
+// diag or bsrc and copied to the host alone, so the host holds K x M doubles instead of the whole field store.
 void Simulation::write_fields(const std::string& fn) {
-    std::vector<Real> F, D, B;
-    fld.download(F, stream);
-    diag.download(D, stream);
-    bsrc.download(B, stream);
-    auto f = [&](int k, int j, int c) { return F[(static_cast<size_t>(k) * M + j) * NCF + c]; };
-    auto d = [&](int k, int j, int c) { return D[(static_cast<size_t>(k) * M + j) * NCD + c]; };
-    auto b = [&](int k, int j, int c) { return B[(static_cast<size_t>(k) * M + j) * NCB + c]; };
-    std::vector<std::pair<std::string, std::function<double(int, int)>>> comps;
-    auto add = [&](const std::string& n, std::function<double(int, int)> fn2) { comps.emplace_back(n, fn2); };
-    add("psi", [&](int k, int j) { return d(k, j, 0); });
-    add("ez", [&](int k, int j) { return f(k, j, 4); });
-    add("er", [&](int k, int j) { return f(k, j, 0); });
-    add("eth", [&](int k, int j) { return f(k, j, 1); });
-    add("br", [&](int k, int j) { return f(k, j, 2); });
-    add("bth", [&](int k, int j) { return f(k, j, 3); });
-    add("bz", [&](int k, int j) { return f(k, j, 5); });
-    add("ne", [&](int k, int j) { return d(k, j, 1); });
-    add("ni", [&](int k, int j) { return d(k, j, 2); });
-    add("rhob", [&](int k, int j) { return b(k, j, 4); });
+    struct Comp { const char* name; int src; int c0; double s0; int c1; double s1; };   // src: 0 fld, 1 diag, 2 bsrc
+    std::vector<Comp> comps = {
+        {"psi", 1, D_PSI0, 1, 0, 0}, {"ez", 0, F_EZ0, 1, 0, 0}, {"er", 0, F_EP1R, 1, 0, 0}, {"eth", 0, F_EP1I, 1, 0, 0},
+        {"br", 0, F_BP1R, 1, 0, 0}, {"bth", 0, F_BP1I, 1, 0, 0}, {"bz", 0, F_BZ0, 1, 0, 0}, {"ne", 1, D_NE0, 1, 0, 0},
+        {"ni", 1, D_NI0, 1, 0, 0}, {"rhob", 2, B_RHO0, 1, 0, 0},
+    };
     if (m1) {
-        auto scalD = [&](const std::string& n, int cr, int ci) {
-            add(n + "_c", [&, cr](int k, int j) { return 2.0 * d(k, j, cr); });
-            add(n + "_s", [&, ci](int k, int j) { return -2.0 * d(k, j, ci); });
+        const std::vector<Comp> more = {
+            {"psi_c", 1, D_PSI1R, 2, 0, 0}, {"psi_s", 1, D_PSI1I, -2, 0, 0},
+            {"ez_c", 0, F_EZ1R, 2, 0, 0}, {"ez_s", 0, F_EZ1I, -2, 0, 0},
+            {"bz_c", 0, F_BZ1R, 2, 0, 0}, {"bz_s", 0, F_BZ1I, -2, 0, 0},
+            {"ne_c", 1, D_NE1R, 2, 0, 0}, {"ne_s", 1, D_NE1I, -2, 0, 0},
+            {"ni_c", 1, D_NI1R, 2, 0, 0}, {"ni_s", 1, D_NI1I, -2, 0, 0},
+            {"rhob_c", 2, B_RHO1R, 2, 0, 0}, {"rhob_s", 2, B_RHO1I, -2, 0, 0},
+            {"er_c", 0, F_EP0R, 1, F_EP2R, 1}, {"er_s", 0, F_EP0I, 1, F_EP2I, -1},
+            {"eth_c", 0, F_EP0I, 1, F_EP2I, 1}, {"eth_s", 0, F_EP2R, 1, F_EP0R, -1},
+            {"br_c", 0, F_BP0R, 1, F_BP2R, 1}, {"br_s", 0, F_BP0I, 1, F_BP2I, -1},
+            {"bth_c", 0, F_BP0I, 1, F_BP2I, 1}, {"bth_s", 0, F_BP2R, 1, F_BP0R, -1},
         };
-        auto scalF = [&](const std::string& n, int cr, int ci) {
-            add(n + "_c", [&, cr](int k, int j) { return 2.0 * f(k, j, cr); });
-            add(n + "_s", [&, ci](int k, int j) { return -2.0 * f(k, j, ci); });
-        };
-        scalD("psi", 3, 4);
-        scalF("ez", 14, 15);
-        scalF("bz", 16, 17);
-        scalD("ne", 5, 6);
-        scalD("ni", 7, 8);
-        add("rhob_c", [&](int k, int j) { return 2.0 * b(k, j, 13); });
-        add("rhob_s", [&](int k, int j) { return -2.0 * b(k, j, 14); });
-        auto vec = [&](const std::string& nr, const std::string& nt, int c0r, int c0i, int c2r, int c2i) {
-            add(nr + "_c", [&, c0r, c2r](int k, int j) { return f(k, j, c0r) + f(k, j, c2r); });
-            add(nr + "_s", [&, c0i, c2i](int k, int j) { return f(k, j, c0i) - f(k, j, c2i); });
-            add(nt + "_c", [&, c0i, c2i](int k, int j) { return f(k, j, c0i) + f(k, j, c2i); });
-            add(nt + "_s", [&, c0r, c2r](int k, int j) { return f(k, j, c2r) - f(k, j, c0r); });
-        };
-        vec("er", "eth", 6, 7, 8, 9);
-        vec("br", "bth", 10, 11, 12, 13);
+        comps.insert(comps.end(), more.begin(), more.end());
     }
     std::FILE* fp = std::fopen(fn.c_str(), "wb");
     if (!fp) throw std::runtime_error("cannot write " + fn);
+    struct Closer { std::FILE* f; ~Closer() { std::fclose(f); } } closer{fp};
     const int32_t hdr[4] = {3, M, K, m1 ? 1 : 0};
     std::fwrite(hdr, 4, 4, fp);
     std::fwrite(&t, 8, 1, fp);
@@ -1123,19 +1106,19 @@ void Simulation::write_fields(const std::string& fn) {
     }
     const int32_t nc = static_cast<int32_t>(comps.size());
     std::fwrite(&nc, 4, 1, fp);
-    std::vector<double> row(static_cast<size_t>(M));
-    for (auto& c : comps) {
+    DevBuf<Real> dcomp(static_cast<size_t>(K) * M);
+    std::vector<Real> hcomp;
+    const Real* srcs[3] = {fld.p, diag.p, bsrc.p};
+    const int ncs[3] = {NCF, NCD, NCB};
+    for (const Comp& c : comps) {
         char name[16];
         std::memset(name, 0, sizeof(name));
-        std::strncpy(name, c.first.c_str(), 15);
+        std::strncpy(name, c.name, 15);
         std::fwrite(name, 1, 16, fp);
-        for (int r = 0; r < K; ++r) {
-            const int k = K - 1 - r;
-            for (int j = 0; j < M; ++j) row[j] = c.second(k, j);
-            std::fwrite(row.data(), 8, M, fp);
-        }
+        qz_check(qz_field_component(srcs[c.src], ncs[c.src], K, M, c.c0, c.s0, c.c1, c.s1, dcomp.p, stream));
+        dcomp.download(hcomp, stream);
+        std::fwrite(hcomp.data(), 8, hcomp.size(), fp);
     }
-    std::fclose(fp);
 }
 
 void Simulation::snapshot() {

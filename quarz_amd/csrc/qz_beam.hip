@@ -359,6 +359,32 @@ int qz_zero_async(void* p, size_t bytes, hipStream_t st) {
     return QZ_OK;
 }
 
+// One output component of the fields_*.bin writer: out[r][j] = s0 a[k][j][c0] + s1 a[k][j][c1] with k = K-1-r
+// (the file lists xi increasing, the head last).  s0, s1 are 0, +-1 or +-2, so every product is exact and the
+// value equals the host expression (one term, 2 x, -2 x, x + y or x - y) bit for bit.
+__global__ void field_component_kernel(const Real* __restrict__ a, int nc, int K, int M, int c0, Real s0, int c1,
+                                       Real s1, Real* __restrict__ out) {
+    const long n = static_cast<long>(K) * M;
+    for (long e = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += static_cast<long>(gridDim.x) * blockDim.x) {
+        const int r = static_cast<int>(e / M), j = static_cast<int>(e % M);
+        const Real* A = a + (static_cast<size_t>(K - 1 - r) * M + j) * nc;
+        Real v = s0 * A[c0];
+        if (s1 != Real(0)) v = v + s1 * A[c1];
+        out[e] = v;
+    }
+}
+
+int qz_field_component(const double* a, int nc, int K, int M, int c0, double s0, int c1, double s1, double* out,
+                       hipStream_t st) {
+    if (!a || !out || nc < 1 || K < 1 || M < 1 || c0 < 0 || c0 >= nc || c1 < 0 || c1 >= nc)
+        return qz_fail(QZ_EINVAL, "field component: bad argument");
+    const long n = static_cast<long>(K) * M;
+    const unsigned nb = static_cast<unsigned>(std::min<long>((n + 255) / 256, 4096));
+    hipLaunchKernelGGL(field_component_kernel, dim3(nb), dim3(256), 0, st, a, nc, K, M, c0, s0, c1, s1, out);
+    QZ_HIP(hipGetLastError());
+    return QZ_OK;
+}
+
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st) {
     if (np <= 0) return QZ_OK;
     const int nb = std::min((np + 255) / 256, 512);

@@ -23,3 +23,6 @@ int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_O
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);
 // zero `bytes` (a multiple of 4) at p with a kernel: unlike a memset node it is safe inside a captured graph
 int qz_zero_async(void* p, size_t bytes, hipStream_t st);
+// out[r][j] = s0 a[K-1-r][j][c0] + s1 a[K-1-r][j][c1] (s1 = 0: one term), a of K x M x nc doubles, on the device
+int qz_field_component(const double* a, int nc, int K, int M, int c0, double s0, int c1, double s1, double* out,
+                       hipStream_t st);
