@@ -297,6 +297,32 @@ __global__ void scan_finish(const int* __restrict__ in, int n, const int* __rest
     if (i < n) out[i] = bsum[blockIdx.x] + sh[threadIdx.x] - v;
 }
 
+// ---- the same exclusive scan in one block when n <= 1024 * kScanOne (a deposit restricted to a few rows): each
+// thread sums a contiguous chunk, the block scans the chunk sums, each thread writes its chunk's prefixes
+constexpr int kScanOne = 64;
+__global__ void __launch_bounds__(1024) scan_one_block(const int* __restrict__ in, int n, int* __restrict__ out) {
+    __shared__ int sh[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(b + per, n);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    sh[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int run = sh[t] - s;
+    for (int i = b; i < e; ++i) {
+        const int v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
 }  // namespace qz
 
 using namespace qz;
@@ -313,9 +339,9 @@ DepWork dep_layout(int np, int nloc, int N) {
     size_t o = 0;
     d.key = o; o = al(o + 4 * static_cast<size_t>(np));
     d.rec = o; o = al(o + 8 * static_cast<size_t>(R_N) * np);
-    d.counts = o; o = al(o + 4 * (nc + 1));
-    d.start = o; o = al(o + 4 * (nc + 1));
+    d.counts = o; o = al(o + 4 * (nc + 1));   // counts and fill adjacent: one fill kernel zeroes both
     d.fill = o; o = al(o + 4 * (nc + 1));
+    d.start = o; o = al(o + 4 * (nc + 1));
     d.list = o; o = al(o + 4 * static_cast<size_t>(np));
     d.bsum = o; o = al(o + 4 * nb);
     d.total = o; o = al(o + 4);
@@ -416,18 +442,20 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
     int* total = reinterpret_cast<int*>(w + L.total);
     const int ncells = (box->nloc + 1) * grid->N;
     {
-        const int rc = qz_zero_async(counts, 4 * (static_cast<size_t>(ncells) + 1), st);
+        const int rc = qz_zero_async(counts, L.fill - L.counts + 4 * (static_cast<size_t>(ncells) + 1), st);
         if (rc) return rc;
-        const int rc2 = qz_zero_async(fill, 4 * (static_cast<size_t>(ncells) + 1), st);
-        if (rc2) return rc2;
     }
     const int tb = g_block[QZ_K_BEAM_PARTICLES], tsort = g_block[QZ_K_BEAM_SORT], tnode = g_block[QZ_K_BEAM_NODES];
     hipLaunchKernelGGL(beam_key_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q, np, *p, key, rec, counts);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
     const int nb = (n1 + 1023) / 1024;
-    hipLaunchKernelGGL(scan_block_sums, dim3(nb), dim3(1024), 0, st, counts, n1, bsum);
-    hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
-    hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
+    if (nb <= kScanOne) {
+        hipLaunchKernelGGL(scan_one_block, dim3(1), dim3(1024), 0, st, counts, n1, start);
+    } else {
+        hipLaunchKernelGGL(scan_block_sums, dim3(nb), dim3(1024), 0, st, counts, n1, bsum);
+        hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+        hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
+    }
     hipLaunchKernelGGL(beam_scatter_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, key, start, fill, list);
     hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tsort - 1) / tsort), dim3(tsort), 0, st, ncells, start, list);
     const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
