@@ -118,6 +118,11 @@ int qz_tridiag_thomas_batched(const double* a, const double* b, const double* c,
 size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N);
 int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
                     const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream);
+/* The deposits of nbeams beams (charges q[b], np[b] particles, arrays p[b]) in one pass: the same bsrc as
+ * qz_beam_deposit of beam 0, then beam 1, ... (Simulation::step_fields' loop over beams,
+ * src/Simulation.cpp:642-643), bit for bit.  work: qz_beam_deposit_workspace_bytes(sum of np, nloc, N) bytes. */
+int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
+                          const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream);
 
 /* Beam push with the stored fields fld (nloc, N+1, nc): gather, pusher (0 vay, 1 boris, 2 hc, 3 imp),
  * drift; move = 0 is the explicit leapfrog start-up kick (Beam::advance, src/Beam.cpp:948-1131). */

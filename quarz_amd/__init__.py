@@ -51,6 +51,8 @@ def lib():
         L.qz_beam_deposit_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
         L.qz_beam_deposit.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.qz_beam_deposit_beams.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_int, DP, IP,
+                                            C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.qz_beam_advance.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_int, C.c_void_p]
@@ -359,6 +361,16 @@ def beam_deposit(grid, box, modes, q, p, bsrc, work, stream=None):
     nc = bsrc.shape[-1]
     check(lib().qz_beam_deposit(C.byref(grid), C.byref(box), modes, q, p["x"].numel(), C.byref(soa), _ptr(bsrc), nc,
                                 _ptr(work), stream))
+
+
+def beam_deposit_beams(grid, box, modes, qs, ps, bsrc, work, stream=None):
+    """Deposits of several beams (charges qs, particle dicts ps) in one pass, as one deposit after another."""
+    n = len(ps)
+    soas = (BeamSoAC * n)(*[beam_soa(p) for p in ps])
+    q = (C.c_double * n)(*qs)
+    npa = (C.c_int * n)(*[p["x"].numel() for p in ps])
+    check(lib().qz_beam_deposit_beams(C.byref(grid), C.byref(box), modes, n, q, npa, soas, _ptr(bsrc), bsrc.shape[-1],
+                                      _ptr(work), stream))
 
 
 def beam_deposit_workspace_bytes(np_, nloc, N):

@@ -917,6 +917,21 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     std::vector<PlasmaSpecies*> mob;
     for (auto& sp : species)
         if (sp->mobile) mob.push_back(sp.get());
+    // the beams' windowed deposits go in one pass (qz_beam_deposit_beams); the window is at most one range + 1 rows
+    std::vector<double> dq;
+    std::vector<int> dnp;
+    std::vector<qz_beam_soa> dsoa;
+    int dtot = 0;
+    for (auto& b : beams)
+        if (b->Np > 0) {
+            dq.push_back(b->q);
+            dnp.push_back(b->Np);
+            dsoa.push_back(b->soa());
+            dtot += b->Np;
+        }
+    int wmax = 1;
+    for (int c = 0; c < nchunks; ++c) wmax = std::max(wmax, cb[c + 1] - cb[c] + 1);
+    const size_t dep_bytes = qz_beam_deposit_workspace_bytes(std::max(dtot, 1), std::min(wmax, K), grid.N);
     std::vector<std::unique_ptr<StepCtx>> ctx(depth);
     for (auto& cp : ctx) {
         cp.reset(new StepCtx());
@@ -928,7 +943,7 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         }
         cp->ws.alloc(ws.n);
         cp->bsrc.alloc(bsrc.n);
-        cp->work.alloc(dep_work.n);
+        cp->work.alloc(dep_bytes);
         cp->pushed.resize(nchunks);
         for (auto& e : cp->pushed) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
     }
@@ -1015,10 +1030,9 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
                 qz_beam_grid bx = box;
                 bx.k0 = box.k0 + d0;
                 bx.nloc = d1 - d0 + 1;
-                for (auto& b : beams) {
-                    const qz_beam_soa so = b->soa();
-                    qz_check(qz_beam_deposit(&g, &bx, mm, b->q, b->Np, &so, C.bsrc.p + d0 * row, NCB, C.work.p, st));
-                }
+                if (!dq.empty())
+                    qz_check(qz_beam_deposit_beams(&g, &bx, mm, static_cast<int>(dq.size()), dq.data(), dnp.data(),
+                                                   dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, st));
             }
             const size_t ai = static_cast<size_t>(i) * nchunks + c;
             qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));

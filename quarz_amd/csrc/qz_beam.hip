@@ -124,11 +124,13 @@ __global__ void __launch_bounds__(256) beam_count_live_kernel(int np, const Real
 }
 
 // deposit step 1: cell key, per-particle record and bucket counts
-__global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int* __restrict__ key,
-                                Real* __restrict__ rec, int* __restrict__ counts) {
+// i0: the particle's index in the deposit's common index space (beams after one another)
+__global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int i0,
+                                int* __restrict__ key, Real* __restrict__ rec, int* __restrict__ counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    key[i] = -1;
+    const int gi = i0 + i;
+    key[gi] = -1;
     // the slice window first: a deposit restricted to a few rows reads only xi of the other particles
     const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
     const int k0 = bg.k0, nloc = bg.nloc;
@@ -153,8 +155,8 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     grid_weights(g.r, g.inv_h, g.N, g.R, r, j, t);
     const int kl = k - k0;   // local slice, -1 .. nloc-1
     const int kk = (kl + 1) * g.N + j;
-    key[i] = kk;
-    Real* rr = rec + static_cast<size_t>(i) * R_N;
+    key[gi] = kk;
+    Real* rr = rec + static_cast<size_t>(gi) * R_N;
     rr[R_WQ] = wi * q; rr[R_TK] = tk; rr[R_T] = t; rr[R_OMVZ] = omvz; rr[R_VZ] = vz; rr[R_VR] = vr; rr[R_VT] = vt;
     rr[R_C] = c; rr[R_S] = s; rr[R_VX] = vx; rr[R_VY] = vy;
     atomicAdd(&counts[kk], 1);
@@ -423,14 +425,22 @@ extern "C" size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N) {
     return dep_layout(np, nloc, N).bytes;
 }
 
-extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
-                               const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream) {
-    if (!grid || !box || !p || !bsrc || (np > 0 && !work)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
-    if (np < 0 || grid->N < 2 || box->nloc < 1) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
+// The deposit of nbeams beams into one bsrc, their particles indexed one beam after another: every node sums
+// the first beam's contributions in particle order, then the next beam's, as nbeams deposits one after another do.
+static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
+                         const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work, hipStream_t st) {
+    long ntot = 0;
+    for (int b = 0; b < nbeams; ++b) {
+        if (np[b] < 0) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
+        ntot += np[b];
+    }
+    if (ntot > 0x7fffffffL) return qz_fail(QZ_ERANGE, "qz_beam_deposit: more than 2^31 - 1 particles");
+    if (!grid || !box || !bsrc || (ntot > 0 && !work)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
+    if (grid->N < 2 || box->nloc < 1) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
     if (nc != (modes == 1 ? B_NC1 : B_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: nc must be 5 (m = 0) or 15 (m = 1)");
-    if (np == 0) return QZ_OK;
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const DepWork L = dep_layout(np, box->nloc, grid->N);
+    if (ntot == 0) return QZ_OK;
+    const int npt = static_cast<int>(ntot);
+    const DepWork L = dep_layout(npt, box->nloc, grid->N);
     char* w = static_cast<char*>(work);
     int* key = reinterpret_cast<int*>(w + L.key);
     Real* rec = reinterpret_cast<Real*>(w + L.rec);
@@ -446,7 +456,10 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
         if (rc) return rc;
     }
     const int tb = g_block[QZ_K_BEAM_PARTICLES], tsort = g_block[QZ_K_BEAM_SORT], tnode = g_block[QZ_K_BEAM_NODES];
-    hipLaunchKernelGGL(beam_key_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q, np, *p, key, rec, counts);
+    for (int b = 0, i0 = 0; b < nbeams; i0 += np[b], ++b)
+        if (np[b] > 0)
+            hipLaunchKernelGGL(beam_key_kernel, dim3((np[b] + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q[b], np[b], p[b],
+                               i0, key, rec, counts);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
     const int nb = (n1 + 1023) / 1024;
     if (nb <= kScanOne) {
@@ -456,13 +469,26 @@ extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* b
         hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
         hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
     }
-    hipLaunchKernelGGL(beam_scatter_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, st, np, key, start, fill, list);
+    hipLaunchKernelGGL(beam_scatter_kernel, dim3((npt + tb - 1) / tb), dim3(tb), 0, st, npt, key, start, fill, list);
     hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tsort - 1) / tsort), dim3(tsort), 0, st, ncells, start, list);
     const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
     hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tnode - 1) / tnode)), dim3(tnode), 0, st, *grid, *box,
                        modes == 1 ? 1 : 0, start, list, rec, bsrc, nc);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
+}
+
+extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
+                               const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream) {
+    if (!p) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
+    return deposit_beams(grid, box, modes, 1, &q, &np, p, bsrc, nc, work, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams,
+                                     const double* q, const int* np, const qz_beam_soa* p, double* bsrc, int nc,
+                                     void* work, void* stream) {
+    if (nbeams < 1 || !q || !np || !p) return qz_fail(QZ_EINVAL, "qz_beam_deposit_beams: null argument or no beam");
+    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,

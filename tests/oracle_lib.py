@@ -210,8 +210,9 @@ def mt19937_64(seed, n):
 BEAM_ARRAYS = ("x", "y", "px", "py", "pz", "xi", "w")
 
 
-def beam_deposit(solver, xi_min, dxi, nxi, ngp, m1, q, parts, nc):
-    src = np.zeros(nxi * solver.M * nc)
+def beam_deposit(solver, xi_min, dxi, nxi, ngp, m1, q, parts, nc, into=None):
+    """Beam::deposit of one beam, added to `into` (a (nxi, M, nc) array) or to zeros."""
+    src = np.zeros(nxi * solver.M * nc) if into is None else np.ascontiguousarray(into, dtype=np.float64).reshape(-1).copy()
     p = [np.ascontiguousarray(parts[k], dtype=np.float64) for k in BEAM_ARRAYS]
     check(lib().or_beam_deposit(solver.h, xi_min, dxi, nxi, int(ngp), int(m1), q, len(p[0]), *[_p(a) for a in p],
                                 _p(src), nc))

@@ -104,6 +104,38 @@ def test_beam_deposit_bit_exact(torch_dev, qa, oracle, m1, ngp):
 
 
 @pytest.mark.parametrize("m1", [False, True])
+def test_beam_deposit_beams_equals_one_after_another(torch_dev, qa, oracle, m1):
+    """Two beams of different charge in one deposit pass: bit-identical to the oracle depositing the first
+    beam and then the second into the same array (a row window of one pass included)."""
+    torch = torch_dev
+    rng = np.random.default_rng(31 + m1)
+    nodes = oracle.make_regions(4.0, [(0.3, 0.005), (4.0, 0.05)], 1.05)
+    s = oracle.Solver(nodes)
+    nxi, dxi, xi_min = 100, 0.02, 0.0
+    A = _beam_parts(rng, 40000, 3.0, -0.1, nxi * dxi + 0.1, m1)
+    B = _beam_parts(rng, 15000, 1.0, 0.5, 1.5, m1)
+    nc = 15 if m1 else 5
+    ref = oracle.beam_deposit(s, xi_min, dxi, nxi, False, m1, -1.0, A, nc)
+    ref = oracle.beam_deposit(s, xi_min, dxi, nxi, False, m1, 2.0, B, nc, into=ref)
+    r, ih, iV = _grid_dev(torch, s)
+    g = qa.radial_grid(r, ih, iV)
+    pa = {k: torch.tensor(A[k], dtype=torch.float64, device="cuda") for k in qa.BEAM_ARRAYS}
+    pb = {k: torch.tensor(B[k], dtype=torch.float64, device="cuda") for k in qa.BEAM_ARRAYS}
+    work = torch.empty(qa.beam_deposit_workspace_bytes(55000, nxi, s.M - 1), dtype=torch.uint8, device="cuda")
+    bsrc = torch.zeros((nxi, s.M, nc), dtype=torch.float64, device="cuda")
+    qa.beam_deposit_beams(g, qa.BeamGridC(xi_min, dxi, nxi, 0, nxi, 0), 1 if m1 else 0, [-1.0, 2.0], [pa, pb], bsrc, work)
+    torch.cuda.synchronize()
+    got = bsrc.cpu().numpy()
+    assert _same(got, ref), _report("bsrc", got, ref)
+    # rows 40..59 only, into a window array: the same rows
+    win = torch.zeros((20, s.M, nc), dtype=torch.float64, device="cuda")
+    qa.beam_deposit_beams(g, qa.BeamGridC(xi_min, dxi, nxi, 40, 20, 0), 1 if m1 else 0, [-1.0, 2.0], [pa, pb], win, work)
+    torch.cuda.synchronize()
+    got = win.cpu().numpy()
+    assert _same(got, ref[40:60]), _report("window", got, ref[40:60])
+
+
+@pytest.mark.parametrize("m1", [False, True])
 @pytest.mark.parametrize("pusher", [0, 1, 2, 3])
 @pytest.mark.parametrize("move", [False, True])
 def test_beam_advance_bit_exact(torch_dev, qa, oracle, m1, pusher, move):
