@@ -30,10 +30,6 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-# one hardware queue per step in flight: the pipelined run keeps up to 32 steps going, HIP's default is 4
-# queues per process (read when HIP starts, so before torch is imported); never above 32
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 32:
-    os.environ["GPU_MAX_HW_QUEUES"] = "32"
 
 METRIC = "plasma particle-slices/s & beam pushes/s (FP64, 1 GPU); batched vs plain"
 FP64_PEAK = 78.6e12
@@ -244,6 +240,13 @@ def main():
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 50 if args.config == "c4" else 20
+    # The pipelined run wants one hardware queue per step in flight; HIP's default is 4 per process, read when
+    # HIP starts (so set before torch is imported), never above 32.  The batched launches (c4, c5) keep the
+    # default: with 32 queues their long persistent launch ran about 1.5x slower (c4: 412 against 276 ms).
+    if args.config not in ("c4", "c5") and args.depth != 0:
+        want = min(32, max(4, (min(args.steps, 32) if args.depth < 0 else args.depth) + 4))
+        if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
+            os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
               f"see DESIGN.md for how the path shards)", file=sys.stderr)

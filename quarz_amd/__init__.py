@@ -142,7 +142,10 @@ class Simulation:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:   # interpreter shutdown: the library may already be gone
+            pass
 
     def step(self):
         check(lib().qz_sim_step(self.h))
