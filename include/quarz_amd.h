@@ -178,8 +178,8 @@ int qz_sim_step_batched(qz_sim** sims, int n, double* batch_ms);
 int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
 /* nrun steps of Simulation::run with up to `depth` steps in flight, each step split into nchunks slice ranges:
  * step n sweeps a range as soon as step n-1 has pushed the beam particles that can reach it.  Bit-identical
- * to nrun calls of qz_sim_step.  Needs a started run (after the first step), the same dt and no field output
- * in those steps.  out2 (optional): particle-slices and beam particles pushed over the nrun steps.  Not in
+ * to nrun calls of qz_sim_step.  Needs a started run (after the first step), the same dt and no output
+ * directory (output.dir: the beam log and field files are written between steps).  out2 (optional): particle-slices and beam particles pushed over the nrun steps.  Not in
  * the reference: its MPI ranks pipeline slice blocks instead, with different arithmetic at the block edges. */
 int qz_sim_run_pipelined(qz_sim* s, int nrun, int nchunks, int depth, double* out2);
 /* what: fld diag bsrc nodes, beam<b>.<x|y|px|py|pz|xi|w>, sp<s>.<x|y|px|py|dl|w>; returns the count */

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <fstream>
 #include <functional>
+#include <iomanip>
 #include <random>
 #include <sstream>
 #include <stdexcept>
@@ -837,10 +838,10 @@ void Simulation::enqueue_push(bool timed) {
 }
 
 // the device work of the next step equals that of the step before: background and plasma template made,
-// every beam started (no leapfrog start-up), the same dt, a beam push to do and no field output
+// every beam started (no leapfrog start-up), the same dt, a beam push to do and no output directory
 bool Simulation::capturable() const {
     if (done || !bg_ready || !have_prev || dt_prev != dt || step_n >= nsteps) return false;
-    if (!outdir.empty() && out_every > 0 && step_n % out_every == 0) return false;
+    if (!outdir.empty()) return false;   // the beam log (and field output) of every step is host work
     for (const auto& s : species)
         if (s->mobile && !s->have_template) return false;
     for (const auto& b : beams)
@@ -856,6 +857,7 @@ void Simulation::step() {
 
 // the rest of one iteration of Simulation::run after the sweep: output, beam push, step counter
 void Simulation::after_sweep() {
+    if (!outdir.empty()) write_beam_output();
     if (!outdir.empty() && out_every > 0 && step_n % out_every == 0) {
         char nm[32];
         std::snprintf(nm, sizeof(nm), "/fields_%06d.bin", step_n);
@@ -865,6 +867,25 @@ void Simulation::after_sweep() {
     push_beams();
     if (last) done = true;
     ++step_n;
+}
+
+// Simulation::write_beam_output for one rank (src/Simulation.cpp:1066-1078, header :259-261): every step, after
+// the sweep and before the beam push, one line per beam with the moments of Beam::from_sums.
+void Simulation::write_beam_output() {
+    if (!beamlog.is_open()) {
+        beamlog.open(outdir + "/beams.txt", std::ios::out | std::ios::trunc);
+        if (!beamlog) throw std::runtime_error("cannot write " + outdir + "/beams.txt");
+        beamlog << "# beam step t alive npart charge gamma_mean gamma_rms xi_mean xi_rms r_rms emit_nx"
+                   " x_mean y_mean emit_ny sigma_x sigma_y\n";
+    }
+    for (const auto& b : beams) {
+        const BeamDiag d = b->from_sums(b->sums(stream), t);
+        beamlog << b->name << " " << step_n << " " << std::setprecision(10) << d.t << " " << d.alive << " " << d.npart
+                << " " << d.charge << " " << d.gamma_mean << " " << d.gamma_rms << " " << d.xi_mean << " " << d.xi_rms
+                << " " << d.r_rms << " " << d.emit_nx << " " << d.x_mean << " " << d.y_mean << " " << d.emit_ny << " "
+                << d.sigma_x << " " << d.sigma_y << "\n";
+    }
+    beamlog.flush();
 }
 
 // ---------------------------------------------------------------------------- pipelined steps
@@ -903,12 +924,9 @@ struct StepCtx {
 void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total) {
     if (nrun < 1) throw std::runtime_error("run_pipelined: nrun < 1");
     if (!capturable())
-        throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no field "
-                                 "output, steps left)");
+        throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no output "
+                                 "directory, steps left)");
     if (step_n + nrun - 1 > nsteps) throw std::runtime_error("run_pipelined: more steps than time.steps leaves");
-    if (!outdir.empty() && out_every > 0)
-        for (int n = step_n; n < step_n + nrun; ++n)
-            if (n % out_every == 0) throw std::runtime_error("run_pipelined: a field output falls in the steps");
     nchunks = std::max(1, std::min(nchunks, K));
     depth = std::max(1, std::min(depth, nrun));
     std::vector<int> cb(nchunks + 1);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <fstream>
 #include <memory>
 #include <string>
 #include <vector>
@@ -145,6 +146,7 @@ public:
     // pushed of all nrun steps.
     void run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total);
     void write_fields(const std::string& path);
+    void write_beam_output();   // one beams.txt line per beam: the beam log of the current step
     // the beams and the step state as they are now, and back to them (repeated evaluations of one step)
     void snapshot();
     void rewind();
@@ -178,6 +180,7 @@ public:
     hipEvent_t ev[6] = {};
     std::string outdir;
     int out_every = 0;
+    std::ofstream beamlog;   // outdir/beams.txt
     struct Snap {
         std::vector<std::array<DevBuf<Real>, 7>> beams;
         std::vector<char> started;

@@ -249,6 +249,34 @@ def test_fields_file_equals_oracle(qa, oracle, decks, tmp_path, deck, ov, steps)
     assert len(a) == len(b) and a == b, (len(a), len(b))
 
 
+def test_beam_log_equals_oracle(qa, oracle, decks, tmp_path):
+    """beams.txt (Simulation::write_beam_output, one line per beam and step, after the sweep and before the
+    push): the header, and every value equal to the oracle's beam log of the same beam state at 10 digits."""
+    ov = {"time.steps": 3, "driver.nparticles": 20000, "xi.min": -8}
+    g = qa.Simulation(f"{decks}/gain.in", {**ov, "output.dir": str(tmp_path)})
+    o = oracle.Sim(f"{decks}/gain.in", ov)
+    want = []
+    for n in range(3):
+        # the oracle's beams as the GPU run logs them: after step n - 1's push, at the time of step n
+        for b in range(o.nbeams):
+            d = o.beam_diag(b)
+            want.append((n, b, d))
+        g.step()
+        o.step()
+    lines = (tmp_path / "beams.txt").read_text().splitlines()
+    assert lines[0] == ("# beam step t alive npart charge gamma_mean gamma_rms xi_mean xi_rms r_rms emit_nx"
+                        " x_mean y_mean emit_ny sigma_x sigma_y")
+    rows = [l.split() for l in lines[1:]]
+    assert len(rows) == len(want)
+    keys = ["t", "alive", "npart", "charge", "gamma_mean", "gamma_rms", "xi_mean", "xi_rms", "r_rms", "emit_nx",
+            "x_mean", "y_mean", "emit_ny", "sigma_x", "sigma_y"]
+    for r, (n, b, d) in zip(rows, want):
+        assert int(r[1]) == n, r
+        for k, v in zip(keys, r[2:]):
+            ref = int(d[k]) if k == "alive" else d[k]
+            assert v == (str(ref) if k == "alive" else f"{ref:.10g}"), (n, b, k, v, ref)
+
+
 # ------------------------------------------------------------------------------------------ batched runs
 def _batched_vs_plain(qa, deck, overrides, steps):
     plain = [qa.Simulation(deck, ov) for ov in overrides]
