@@ -1865,6 +1865,42 @@ static void write_fields(const Sim& s, const std::string& fn) {
     std::fclose(fp);
 }
 
+// src/Simulation.cpp:995-1029 (one rank): fixed-width text, a 256-byte header of two padded lines, then one line
+// per slice from the tail to the head: xi, Ez(0), psi(0), ne(0), rho_beam(0), (Er - Bth)/r at the first node
+// off the axis, and with m = 1 the axis values of W = E_perp + z x B_perp.
+static void write_axis(const Sim& s, const std::string& fn) {
+    const int M = s.M, K = s.K;
+    auto F = [&](int k, int j, int c) { return s.fld[(static_cast<size_t>(k) * M + j) * s.NCF + c]; };
+    auto D = [&](int k, int j, int c) { return s.diag[(static_cast<size_t>(k) * M + j) * s.NCD + c]; };
+    auto B = [&](int k, int j, int c) { return s.bsrc[(static_cast<size_t>(k) * M + j) * s.NCB + c]; };
+    const double r1 = s.grid.r[1];
+    const int ncol = s.m1 ? 8 : 6;
+    std::string h = "# t = " + std::to_string(s.t);
+    h.resize(127, ' ');
+    h += '\n';
+    std::string h2 = "# xi  Ez(0)  psi(0)  ne(0)  rho_beam(0)  (Er-Bth)/r|r1";
+    if (s.m1) h2 += "  Wx(0)  Wy(0)  [W = E_perp + z x B_perp on the axis, from m=1]";
+    h2.resize(127, ' ');
+    h2 += '\n';
+    h += h2;
+    std::string buf;
+    char cell[32];
+    for (int k = K - 1; k >= 0; --k) {
+        double v[8] = {xi_to_user(s.box.xi_min + k * s.box.dxi), F(k, 0, F_EZ0), D(k, 0, D_PSI0), D(k, 0, D_NE0),
+                       B(k, 0, B_RHO0), (F(k, 1, F_EP1R) - F(k, 1, F_BP1I)) / r1, 0, 0};
+        if (s.m1) { v[6] = F(k, 0, F_EP0R) - F(k, 0, F_BP0I); v[7] = F(k, 0, F_EP0I) + F(k, 0, F_BP0R); }
+        for (int c = 0; c < ncol; ++c) {
+            std::snprintf(cell, sizeof(cell), "%+.10e%c", v[c], c == ncol - 1 ? '\n' : ' ');
+            buf += cell;
+        }
+    }
+    std::FILE* fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot write " + fn);
+    std::fwrite(h.data(), 1, h.size(), fp);
+    std::fwrite(buf.data(), 1, buf.size(), fp);
+    std::fclose(fp);
+}
+
 }  // namespace orc
 
 // ============================================================================ C API (test infrastructure)
@@ -2132,5 +2168,6 @@ int or_sim_beam_diag(void* h, int b, double* out15) {
     return 0;
 }
 int or_sim_write_fields(void* h, const char* path) { OR_TRY(write_fields(*static_cast<Sim*>(h), path)) }
+int or_sim_write_axis(void* h, const char* path) { OR_TRY(write_axis(*static_cast<Sim*>(h), path)) }
 
 }  // extern "C"

@@ -653,6 +653,7 @@ Simulation::Simulation(const std::string& deck, const std::string& overrides) { 
     }
     outdir = cfg.get_string("output.dir", "");
     out_every = cfg.get_int("output.every", 0);
+    out_axis = cfg.get_bool("output.axis", true);
     (void)cfg.get_int("output.beam_every", 0);
     M = grid.N + 1;
     K = box.nloc;
@@ -862,6 +863,10 @@ void Simulation::after_sweep() {
         char nm[32];
         std::snprintf(nm, sizeof(nm), "/fields_%06d.bin", step_n);
         write_fields(outdir + nm);
+        if (out_axis) {
+            std::snprintf(nm, sizeof(nm), "/axis_%06d.txt", step_n);
+            write_axis(outdir + nm);
+        }
     }
     const bool last = (step_n >= nsteps);
     push_beams();
@@ -1151,6 +1156,51 @@ void Simulation::write_fields(const std::string& fn) {
         dcomp.download(hcomp, stream);
         std::fwrite(hcomp.data(), 8, hcomp.size(), fp);
     }
+}
+
+// axis_*.txt (src/Simulation.cpp:995-1029, one rank): a 256-byte header of two padded lines, then one fixed-width
+// line per slice from the tail to the head: xi, Ez(0), psi(0), ne(0), rho_beam(0), (Er - Bth)/r at the first
+// node off the axis, and with m = 1 the axis values of W = E_perp + z x B_perp.  Only the first two nodes of
+// every slice are copied from HBM (strided copies), not the field store.
+void Simulation::write_axis(const std::string& fn) {
+    std::vector<Real> F(static_cast<size_t>(K) * 2 * NCF), D(static_cast<size_t>(K) * NCD), B(static_cast<size_t>(K) * NCB);
+    hip_check(hipMemcpy2DAsync(F.data(), 16 * NCF, fld.p, 8 * static_cast<size_t>(M) * NCF, 16 * NCF, K,
+                               hipMemcpyDeviceToHost, stream), "axis copy");
+    hip_check(hipMemcpy2DAsync(D.data(), 8 * NCD, diag.p, 8 * static_cast<size_t>(M) * NCD, 8 * NCD, K,
+                               hipMemcpyDeviceToHost, stream), "axis copy");
+    hip_check(hipMemcpy2DAsync(B.data(), 8 * NCB, bsrc.p, 8 * static_cast<size_t>(M) * NCB, 8 * NCB, K,
+                               hipMemcpyDeviceToHost, stream), "axis copy");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    auto f = [&](int k, int j, int c) { return F[(static_cast<size_t>(k) * 2 + j) * NCF + c]; };
+    auto d = [&](int k, int c) { return D[static_cast<size_t>(k) * NCD + c]; };
+    auto b = [&](int k, int c) { return B[static_cast<size_t>(k) * NCB + c]; };
+    const double r1 = grid.r[1];
+    const int ncol = m1 ? 8 : 6;
+    std::string h = "# t = " + std::to_string(t);
+    h.resize(127, ' ');
+    h += '\n';
+    std::string h2 = "# xi  Ez(0)  psi(0)  ne(0)  rho_beam(0)  (Er-Bth)/r|r1";
+    if (m1) h2 += "  Wx(0)  Wy(0)  [W = E_perp + z x B_perp on the axis, from m=1]";
+    h2.resize(127, ' ');
+    h2 += '\n';
+    h += h2;
+    std::string buf;
+    buf.reserve(static_cast<size_t>(18) * ncol * K);
+    char cell[32];
+    for (int k = K - 1; k >= 0; --k) {   // increasing user xi: the head of the box is the last line
+        double v[8] = {xi_to_user(box.xi_min + k * box.dxi), f(k, 0, F_EZ0), d(k, D_PSI0), d(k, D_NE0), b(k, B_RHO0),
+                       (f(k, 1, F_EP1R) - f(k, 1, F_BP1I)) / r1, 0, 0};
+        if (m1) { v[6] = f(k, 0, F_EP0R) - f(k, 0, F_BP0I); v[7] = f(k, 0, F_EP0I) + f(k, 0, F_BP0R); }
+        for (int c = 0; c < ncol; ++c) {
+            std::snprintf(cell, sizeof(cell), "%+.10e%c", v[c], c == ncol - 1 ? '\n' : ' ');
+            buf += cell;
+        }
+    }
+    std::FILE* fp = std::fopen(fn.c_str(), "wb");
+    if (!fp) throw std::runtime_error("cannot write " + fn);
+    std::fwrite(h.data(), 1, h.size(), fp);
+    std::fwrite(buf.data(), 1, buf.size(), fp);
+    std::fclose(fp);
 }
 
 void Simulation::snapshot() {

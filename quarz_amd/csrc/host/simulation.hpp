@@ -147,6 +147,7 @@ public:
     void run_pipelined(int nrun, int nchunks, int depth, double& psl_total, double& pushed_total);
     void write_fields(const std::string& path);
     void write_beam_output();   // one beams.txt line per beam: the beam log of the current step
+    void write_axis(const std::string& path);   // axis_*.txt: on-axis fields of every slice
     // the beams and the step state as they are now, and back to them (repeated evaluations of one step)
     void snapshot();
     void rewind();
@@ -180,6 +181,7 @@ public:
     hipEvent_t ev[6] = {};
     std::string outdir;
     int out_every = 0;
+    bool out_axis = true;    // output.axis: axis_*.txt with every field output
     std::ofstream beamlog;   // outdir/beams.txt
     struct Snap {
         std::vector<std::array<DevBuf<Real>, 7>> beams;

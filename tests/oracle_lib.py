@@ -78,6 +78,7 @@ def lib():
         L.or_sim_beam_sums.argtypes = [C.c_void_p, C.c_int, DP, DP]
         L.or_sim_beam_diag.argtypes = [C.c_void_p, C.c_int, DP]
         L.or_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
+        L.or_sim_write_axis.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -310,3 +311,6 @@ class Sim:
 
     def write_fields(self, path):
         check(lib().or_sim_write_fields(self.h, path.encode()))
+
+    def write_axis(self, path):
+        check(lib().or_sim_write_axis(self.h, path.encode()))

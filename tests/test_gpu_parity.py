@@ -249,6 +249,34 @@ def test_fields_file_equals_oracle(qa, oracle, decks, tmp_path, deck, ov, steps)
     assert len(a) == len(b) and a == b, (len(a), len(b))
 
 
+@pytest.mark.parametrize("deck,ov", [
+    ("gain.in", {"time.steps": 2, "driver.nparticles": 20000, "xi.min": -8}),
+    ("betatron.in", {"modes": 1, "witness.x0": 0.05, "witness.nsym": 4, "time.steps": 2}),
+])
+def test_axis_file_equals_oracle(qa, oracle, decks, tmp_path, deck, ov):
+    """axis_*.txt, written with every field output from the first two nodes of each slice: the oracle's text
+    byte for byte, after the time line of the header (the run writes it before the step's beam push, at t of
+    the step; the oracle's file is made after the whole step)."""
+    g = qa.Simulation(f"{decks}/{deck}", {**ov, "output.dir": str(tmp_path), "output.every": 1})
+    o = oracle.Sim(f"{decks}/{deck}", ov)
+    dt = None
+    for n in range(3):
+        g.step()
+        o.step()
+        ref = tmp_path / f"oracle_{n}.txt"
+        o.write_axis(str(ref))
+        a = (tmp_path / f"axis_{n:06d}.txt").read_bytes()
+        b = ref.read_bytes()
+        assert len(a) == len(b) and a[128:] == b[128:], (n, len(a), len(b))
+        line = a[:128].decode()
+        assert line.startswith("# t = ") and len(line) == 128 and line.endswith("\n"), line
+        t = float(line[6:])
+        if n == 1:
+            dt = t
+        if n == 2:
+            assert t == 2 * dt, (t, dt)
+
+
 def test_beam_log_equals_oracle(qa, oracle, decks, tmp_path):
     """beams.txt (Simulation::write_beam_output, one line per beam and step, after the sweep and before the
     push): the header, and every value equal to the oracle's beam log of the same beam state at 10 digits."""
