@@ -10,7 +10,8 @@ bit-identical to stepping one at a time); --depth 0 steps them one at a time.  P
   value          plasma particle-slices per second of the whole timed steps (wall clock)
   plain_value    the same run stepped one step at a time (--plain-steps steps), for comparison
   beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
-  roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops
+  roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops; valu_frac: the FP64 VALU
+                 lane operations rocprof counts per launch (profiles/sweep_fp64_valu.md) over the same time
                  679 per particle-slice + 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP
                  events on the simulation's stream
   beam_roofline  the beam deposit and push against MI355X HBM bandwidth (8 TB/s), algorithmic bytes per step:
@@ -38,6 +39,9 @@ FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-s
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
 # same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
 MEASURED_TRAFFIC = {"c2": 841.9e6}
+# FP64 VALU lane operations per sweep launch, 64 x SQ_INSTS_VALU_FLOPS_FP64 (profiles/sweep_fp64_valu.md):
+# c2 one run, c4 the 33-run batched launch
+MEASURED_VALU = {"c2": 2.33e10, "c4": 1.33e11}
 
 
 def cpu_baseline(deck, nthreads):
@@ -216,6 +220,8 @@ def bench_gradient(args):
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel (batched)",
                      "flops_per_launch": flops,
+                     "valu_fp64_lane_ops": MEASURED_VALU["c4"],
+                     "valu_frac": MEASURED_VALU["c4"] / (batch_ms / iters / 1e3) / FP64_PEAK,
                      "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
     }
     obj.close()
@@ -343,6 +349,9 @@ def main():
                      "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
                      "kernel": "sweep_kernel",
                      "flops_per_launch": flops_per_sweep,
+                     "valu_fp64_lane_ops": MEASURED_VALU.get(args.config),
+                     "valu_frac": (MEASURED_VALU[args.config] / t_kernel / FP64_PEAK
+                                   if args.config in MEASURED_VALU and t_kernel > 0 else None),
                      "aggregate_achieved": flops_per_sweep * args.steps / wall / 1e12,
                      "aggregate_frac": flops_per_sweep * args.steps / wall / FP64_PEAK,
                      "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s. "
