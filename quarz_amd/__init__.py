@@ -43,6 +43,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise QuarzError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        # torch ships its own HIP runtime.  Loading it first makes this library bind to the same runtime;
+        # if this library's runtime starts first, torch's own cannot open the device (torch.cuda.is_available()
+        # turns False) and the process has two runtimes.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         L.qz_last_error.restype = C.c_char_p
         L.qz_device_info.argtypes = [C.c_char_p, C.c_int, IP]

@@ -2,15 +2,18 @@
 #include "../qz_device.hpp"   // component order of fld / diag / bsrc
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <fstream>
 #include <functional>
 #include <iomanip>
 #include <random>
 #include <sstream>
 #include <stdexcept>
+#include <thread>
 
 namespace qz {
 
@@ -810,7 +813,8 @@ void Simulation::push_beams() {
     hip_check(hipEventElapsedTime(&ms_beam, ev[2], ev[3]), "elapsed");
     {
         unsigned long long v = 0;
-        hip_check(hipMemcpy(&v, counters.p + 21, sizeof(v), hipMemcpyDeviceToHost), "counter");
+        hip_check(hipMemcpyAsync(&v, counters.p + 21, sizeof(v), hipMemcpyDeviceToHost, stream), "counter");
+        hip_check(hipStreamSynchronize(stream), "sync");
         beam_pushed = static_cast<double>(v);
     }
     t += dt;
@@ -1317,11 +1321,28 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
         qz::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
         if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
         if (batch_ms) *batch_ms = ms;
-        for (int i = 0; i < n; ++i) {
-            qz::Simulation& s = *hs[i]->s;
-            s.finish_sweep(ms);
-            s.after_sweep();
-        }
+        // the rest of every run's step (counters, output, beam push) touches only that run's stream and
+        // buffers: a few host threads do the runs side by side
+        std::vector<std::exception_ptr> errs(static_cast<size_t>(n));
+        std::atomic<int> next{0};
+        auto work = [&]() {
+            for (int i = next++; i < n; i = next++) {
+                try {
+                    qz::Simulation& s = *hs[i]->s;
+                    s.finish_sweep(ms);
+                    s.after_sweep();
+                } catch (...) {
+                    errs[static_cast<size_t>(i)] = std::current_exception();
+                }
+            }
+        };
+        const int nth = std::min(n, 8);
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nth; ++t) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
+        for (auto& e : errs)
+            if (e) std::rethrow_exception(e);
     })
 }
 

@@ -17,6 +17,8 @@ with the 2S + 1 evaluations run as one batched step (one persistent sweep launch
 The hand-written adjoint sweep is not built; `SpreadFunction` puts this finite-difference gradient behind
 torch.autograd, and `optimise` runs Adam on the profile with it.
 """
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import Simulation, step_batched
@@ -38,8 +40,12 @@ class SpreadObjective:
             s.snapshot()
         self.evaluations = 0
         self.last_batch_ms = 0.0
+        # the per-run host work (profile weights, beam sums) of different runs touches different streams and
+        # buffers; the library calls release the GIL, so a few threads run them side by side
+        self._pool = ThreadPoolExecutor(max_workers=min(8, len(self.sims)))
 
     def close(self):
+        self._pool.shutdown()
         for s in self.sims:
             s.close()
         self.sims = []
@@ -50,14 +56,18 @@ class SpreadObjective:
         if not 1 <= len(profiles) <= len(self.sims):
             raise ValueError(f"1..{len(self.sims)} profiles per evaluation")
         sims = self.sims[:len(profiles)]
-        for s, v in zip(sims, profiles):
+        for v in profiles:
             if v.shape != (self.nseg,):
                 raise ValueError(f"a profile has {self.nseg} values")
-            s.rewind()
-            s.set_beam_profile(self.beam, v)
+
+        def prepare(sv):
+            sv[0].rewind()
+            sv[0].set_beam_profile(self.beam, sv[1])
+
+        list(self._pool.map(prepare, zip(sims, profiles)))
         self.last_batch_ms = step_batched(sims)
         self.evaluations += len(sims)
-        return np.array([relative_spread(s.beam_diag(self.beam)) for s in sims])
+        return np.array(list(self._pool.map(lambda s: relative_spread(s.beam_diag(self.beam)), sims)))
 
     def value_and_grad(self, v, eps=1e-6):
         """J(v) and its central finite-difference gradient, in one batched step of 2S + 1 runs."""

@@ -305,6 +305,25 @@ def test_beam_log_equals_oracle(qa, oracle, decks, tmp_path):
             assert v == (str(ref) if k == "alive" else f"{ref:.10g}"), (n, b, k, v, ref)
 
 
+def test_torch_keeps_the_device_after_the_library(qa):
+    """In a fresh process the package loads its library and steps a run before torch touches the GPU;
+    torch must still see the device and compute (one HIP runtime in the process)."""
+    import subprocess
+    import sys
+    import os
+    code = ("import quarz_amd, sys\n"
+            "from quarz_amd import configs\n"
+            "s = quarz_amd.Simulation(configs.c4_deck(nd=2000, nw=1000), {'xi.min': -11.99})\n"
+            "s.step()\n"
+            "import torch\n"
+            "ok = torch.cuda.is_available() and float(torch.ones(3, device='cuda').sum()) == 3.0\n"
+            "print('torch ok' if ok else 'torch lost the device', flush=True)\n"
+            "sys.exit(0 if ok else 1)\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr[-2000:]
+
+
 # ------------------------------------------------------------------------------------------ batched runs
 def _batched_vs_plain(qa, deck, overrides, steps):
     plain = [qa.Simulation(deck, ov) for ov in overrides]
