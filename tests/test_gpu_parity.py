@@ -324,6 +324,41 @@ def test_torch_keeps_the_device_after_the_library(qa):
     assert r.returncode == 0, r.stdout + r.stderr[-2000:]
 
 
+def test_c1_workload_bit_exact(qa, oracle):
+    """The c1 workload at full size (m = 0, 256 uniform cells, 1601 slices, 1e5-particle driver), three steps:
+    fields, diagnostics, beam sources and the driver's phase space equal the oracle's bit for bit."""
+    from quarz_amd import configs
+    deck = configs.deck("c1")
+    g = qa.Simulation(deck, {"time.steps": 3})
+    o = oracle.Sim(deck, {"time.steps": 3})
+    for _ in range(3):
+        g.step()
+        o.step()
+    _compare_sims(g, o)
+    for k in qa.BEAM_ARRAYS:
+        a, b = g.get(f"beam0.{k}"), o.get(f"beam0.{k}")
+        assert _same(a, b), _report(f"driver {k}", a, b)
+
+
+def test_c2_workload_pipelined_bit_exact(qa, oracle, tmp_path):
+    """The c2 workload at full size (m = 0 + 1, 512 stretched cells, 4001 slices, driver 1e6 + witness 5e5)
+    through the bench's path: the start-up step, then two steps in flight over 128 slice ranges.  Fields,
+    diagnostics, beam sources and both beams equal the oracle's three serial steps bit for bit."""
+    from quarz_amd import configs
+    deck = configs.deck("c2", workdir=str(tmp_path))
+    g = qa.Simulation(deck, {"time.steps": 4})
+    o = oracle.Sim(deck, {"time.steps": 4})
+    g.step()
+    g.run_pipelined(2, chunks=128, depth=2)
+    for _ in range(3):
+        o.step()
+    _compare_sims(g, o)
+    for bi in range(2):
+        for k in qa.BEAM_ARRAYS:
+            a, b = g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")
+            assert _same(a, b), _report(f"beam{bi} {k}", a, b)
+
+
 # ------------------------------------------------------------------------------------------ batched runs
 def _batched_vs_plain(qa, deck, overrides, steps):
     plain = [qa.Simulation(deck, ov) for ov in overrides]
