@@ -359,6 +359,31 @@ def test_c2_workload_pipelined_bit_exact(qa, oracle, tmp_path):
             assert _same(a, b), _report(f"beam{bi} {k}", a, b)
 
 
+def test_c5_runs_batched_bit_exact(qa, oracle, tmp_path):
+    """Two runs of the c5 scan (their own grids, slice counts, beams), stepped as a Batch (eager start-up
+    step, eager step that captures, one graph replay): each run equals the oracle's serial run bit for bit."""
+    from quarz_amd import configs
+    runs = configs.c5_runs(nruns=2, steps=3, workdir=str(tmp_path))
+    sims = [qa.Simulation(deck) for deck, _ in runs]
+    batch = qa.Batch(sims, graph=True)
+    for _ in range(3):
+        batch.step()
+    replays = batch.replays
+    batch.close()
+    assert replays >= 1
+    for i, ((deck, info), g) in enumerate(zip(runs, sims)):
+        o = oracle.Sim(deck)
+        for _ in range(3):
+            o.step()
+        for w in ("fld", "diag", "bsrc"):
+            a, b = getattr(g, w), getattr(o, w)
+            assert _same(a, b), _report(f"run {i} ({info}) {w}", a, b)
+        for bi in range(2):
+            for k in qa.BEAM_ARRAYS:
+                a, b = g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")
+                assert _same(a, b), _report(f"run {i} beam{bi} {k}", a, b)
+
+
 # ------------------------------------------------------------------------------------------ batched runs
 def _batched_vs_plain(qa, deck, overrides, steps):
     plain = [qa.Simulation(deck, ov) for ov in overrides]
