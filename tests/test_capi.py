@@ -49,3 +49,44 @@ def test_set_block_validates_and_round_trips():
         quarz_amd.set_block(k, 128)
         assert quarz_amd.get_block(k) == 128
         quarz_amd.set_block(k, before)
+
+
+def test_entry_points_reject_bad_arguments_before_any_gpu_call():
+    """Argument checks of the kernel entry points return QZ_EINVAL with a message before any HIP call (the
+    pointers below are never read)."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    dummy = C.c_void_p(256)
+    g = quarz_amd.RadialGridC()          # N = 0
+    box = quarz_amd.BeamGridC(0.0, 0.01, 10, 0, 10, 0)
+    soa = quarz_amd.BeamSoAC()
+    q = (C.c_double * 2)(-1.0, 1.0)
+    n = (C.c_int * 2)(5, 7)
+
+    def dep(nbeams, nc):
+        return L.qz_beam_deposit_beams(C.byref(g), C.byref(box), 0, nbeams, q, n, C.byref(soa), dummy, nc, dummy,
+                                       None)
+
+    assert dep(0, 5) == -1 and "no beam" in quarz_amd.last_error()
+    assert dep(2, 5) == -1 and "bad extent" in quarz_amd.last_error()       # N < 2
+    g.N = 8
+    assert dep(2, 7) == -1 and "nc must be" in quarz_amd.last_error()       # m = 0 needs 5 components
+    n[1] = -3
+    assert dep(2, 5) == -1 and "bad extent" in quarz_amd.last_error()       # negative particle count
+    assert L.qz_beam_deposit(C.byref(g), C.byref(box), 0, -1.0, 5, C.byref(soa), dummy, 15, dummy, None) == -1
+    assert L.qz_beam_advance(C.byref(g), C.byref(box), 0, -1.0, 0, 0, 5, C.byref(soa), dummy, 18, 1.0, 1.0, 1,
+                             None) == -1
+    assert "nc must be" in quarz_amd.last_error()
+    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 0, 1, None) == -1
+    assert "bad extent" in quarz_amd.last_error()
+
+
+def test_sweep_rejects_unsupported_sizes_before_any_gpu_call():
+    """qz_sweep checks every run before it touches the device: a zero-filled argument block has N = 0."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    blob = (C.c_char * 4096)()           # larger than qz_sweep_args, all zero: modes 0, N 0
+    assert L.qz_sweep(blob, None) == -2 and "radial cells" in quarz_amd.last_error()
+    assert L.qz_sweep_batched(blob, 0, None) == -1 and "no runs" in quarz_amd.last_error()
