@@ -120,9 +120,14 @@ int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
                     const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream);
 /* The deposits of nbeams beams (charges q[b], np[b] particles, arrays p[b]) in one pass: the same bsrc as
  * qz_beam_deposit of beam 0, then beam 1, ... (Simulation::step_fields' loop over beams,
- * src/Simulation.cpp:642-643), bit for bit.  work: qz_beam_deposit_workspace_bytes(sum of np, nloc, N) bytes. */
+ * src/Simulation.cpp:642-643), bit for bit.  work: qz_beam_deposit_workspace_bytes(sum of np, nloc, N) bytes.
+ * pstep (optional, a host array of nbeams device pointers, as qz_beam_advance_slices marks them): particles
+ * whose marker is `marker` or later are left out.  A step run range by range deposits a range after it has pushed
+ * earlier ranges; the particles it pushed there were deposited before their push and must not be seen again
+ * at their new places (a slow particle can slip into a later range). */
 int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
-                          const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream);
+                          const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
+                          const int* const* pstep, int marker, void* stream);
 
 /* Beam push with the stored fields fld (nloc, N+1, nc): gather, pusher (0 vay, 1 boris, 2 hc, 3 imp),
  * drift; move = 0 is the explicit leapfrog start-up kick (Beam::advance, src/Beam.cpp:948-1131). */
@@ -130,9 +135,10 @@ int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
                     int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt, int move,
                     void* stream);
 /* The same push restricted to the particles whose gather slice (after clamping to the box) is in
- * [s_lo, s_hi) and whose marker pstep[i] (np ints, caller-owned) differs from `step`; pushed particles get
- * pstep[i] = step.  A step pushed range by range, in increasing slice order, then pushes every live
- * particle exactly once, as Beam::advance does (particles only slip to later slices).  Not in the reference:
+ * [s_lo, s_hi) and whose marker pstep[i] (np ints, caller-owned) is below `step`; pushed particles get
+ * pstep[i] = step.  Markers grow with the step, so a particle a later step has already pushed is not pushed
+ * again.  A step pushed range by range, in increasing slice order, then pushes every live particle exactly
+ * once, as Beam::advance does (particles only slip to later slices).  Not in the reference:
  * it serves the pipelined run, where slice ranges are pushed as soon as their fields exist. */
 int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,
                            int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt,

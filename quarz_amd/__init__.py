@@ -56,7 +56,8 @@ def lib():
         L.qz_beam_deposit.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.qz_beam_deposit_beams.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_int, DP, IP,
-                                            C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+                                            C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_int, C.c_void_p]
         L.qz_beam_advance.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_int, C.c_void_p]
@@ -377,7 +378,7 @@ def beam_deposit_beams(grid, box, modes, qs, ps, bsrc, work, stream=None):
     q = (C.c_double * n)(*qs)
     npa = (C.c_int * n)(*[p["x"].numel() for p in ps])
     check(lib().qz_beam_deposit_beams(C.byref(grid), C.byref(box), modes, n, q, npa, soas, _ptr(bsrc), bsrc.shape[-1],
-                                      _ptr(work), stream))
+                                      _ptr(work), None, 0, stream))
 
 
 def beam_deposit_workspace_bytes(np_, nloc, N):

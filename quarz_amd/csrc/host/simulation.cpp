@@ -936,7 +936,9 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no output "
                                  "directory, steps left)");
     if (step_n + nrun - 1 > nsteps) throw std::runtime_error("run_pipelined: more steps than time.steps leaves");
-    nchunks = std::max(1, std::min(nchunks, K));
+    // ranges of at least two slices: a range's deposit then needs only the previous step's pushes up to the
+    // next range (a particle in the deposit window has its gather slice below the range after next)
+    nchunks = std::max(1, std::min(nchunks, K / 2));
     depth = std::max(1, std::min(depth, nrun));
     std::vector<int> cb(nchunks + 1);
     for (int c = 0; c <= nchunks; ++c) cb[c] = static_cast<int>((static_cast<long long>(K) * c) / nchunks);
@@ -948,12 +950,14 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     std::vector<double> dq;
     std::vector<int> dnp;
     std::vector<qz_beam_soa> dsoa;
+    std::vector<Beam*> dbeam;
     int dtot = 0;
     for (auto& b : beams)
         if (b->Np > 0) {
             dq.push_back(b->q);
             dnp.push_back(b->Np);
             dsoa.push_back(b->soa());
+            dbeam.push_back(b.get());
             dtot += b->Np;
         }
     int wmax = 1;
@@ -984,6 +988,8 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             qz_check(qz_zero_async(b->pstep.p, sizeof(int) * b->pstep.n, stream));
         }
     hip_check(hipStreamSynchronize(stream), "sync");   // the run's own stream is done before the contexts start
+    std::vector<const int*> dpstep;   // the deposits leave out what their own step has pushed
+    for (Beam* b : dbeam) dpstep.push_back(b->pstep.p);
 
     // the sweep arguments of every (step, range): the run's, with the step context's own arrays
     std::vector<qz_sweep_args> hargs(static_cast<size_t>(nrun) * nchunks);
@@ -1059,7 +1065,8 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
                 bx.nloc = d1 - d0 + 1;
                 if (!dq.empty())
                     qz_check(qz_beam_deposit_beams(&g, &bx, mm, static_cast<int>(dq.size()), dq.data(), dnp.data(),
-                                                   dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, st));
+                                                   dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, dpstep.data(),
+                                                   marker, st));
             }
             const size_t ai = static_cast<size_t>(i) * nchunks + c;
             qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));

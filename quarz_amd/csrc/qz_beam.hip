@@ -51,7 +51,9 @@ __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, R
         tk = s - k;
     }
     if (pstep) {
-        if (k < s_lo || k >= s_hi || pstep[i] == step) return;
+        // markers grow with the step: a particle a later step has already pushed (a slow one can slip past
+        // the ranges this step has still to push) is not pushed again
+        if (k < s_lo || k >= s_hi || pstep[i] >= step) return;
         pstep[i] = step;
     }
     const int k1 = ngp ? k : k + 1;
@@ -124,9 +126,12 @@ __global__ void __launch_bounds__(256) beam_count_live_kernel(int np, const Real
 }
 
 // deposit step 1: cell key, per-particle record and bucket counts
-// i0: the particle's index in the deposit's common index space (beams after one another)
+// i0: the particle's index in the deposit's common index space (beams after one another).  pstep (optional):
+// particles whose marker is `marker` or later are left out (a pipelined step's deposit skips the particles
+// that step, or a later one, has already pushed)
 __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int i0,
-                                int* __restrict__ key, Real* __restrict__ rec, int* __restrict__ counts) {
+                                const int* __restrict__ pstep, int marker, int* __restrict__ key,
+                                Real* __restrict__ rec, int* __restrict__ counts) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int gi = i0 + i;
@@ -138,6 +143,7 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     const Real sx = (P.xi[i] - xmin) * idxi;
     const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
     if (k < k0 - 1 || k >= k0 + nloc) return;
+    if (pstep && pstep[i] >= marker) return;
     const Real wi = P.w[i];
     if (wi == Real(0)) return;
     const Real X = P.x[i], Y = P.y[i];
@@ -428,7 +434,8 @@ extern "C" size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N) {
 // The deposit of nbeams beams into one bsrc, their particles indexed one beam after another: every node sums
 // the first beam's contributions in particle order, then the next beam's, as nbeams deposits one after another do.
 static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
-                         const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work, hipStream_t st) {
+                         const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
+                         const int* const* pstep, int marker, hipStream_t st) {
     long ntot = 0;
     for (int b = 0; b < nbeams; ++b) {
         if (np[b] < 0) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
@@ -459,7 +466,7 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
     for (int b = 0, i0 = 0; b < nbeams; i0 += np[b], ++b)
         if (np[b] > 0)
             hipLaunchKernelGGL(beam_key_kernel, dim3((np[b] + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q[b], np[b], p[b],
-                               i0, key, rec, counts);
+                               i0, pstep ? pstep[b] : nullptr, marker, key, rec, counts);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
     const int nb = (n1 + 1023) / 1024;
     if (nb <= kScanOne) {
@@ -481,14 +488,15 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
 extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
                                const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream) {
     if (!p) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
-    return deposit_beams(grid, box, modes, 1, &q, &np, p, bsrc, nc, work, static_cast<hipStream_t>(stream));
+    return deposit_beams(grid, box, modes, 1, &q, &np, p, bsrc, nc, work, nullptr, 0, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams,
                                      const double* q, const int* np, const qz_beam_soa* p, double* bsrc, int nc,
-                                     void* work, void* stream) {
+                                     void* work, const int* const* pstep, int marker, void* stream) {
     if (nbeams < 1 || !q || !np || !p) return qz_fail(QZ_EINVAL, "qz_beam_deposit_beams: null argument or no beam");
-    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, static_cast<hipStream_t>(stream));
+    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, pstep, marker,
+                         static_cast<hipStream_t>(stream));
 }
 
 extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,

@@ -66,7 +66,7 @@ def test_entry_points_reject_bad_arguments_before_any_gpu_call():
 
     def dep(nbeams, nc):
         return L.qz_beam_deposit_beams(C.byref(g), C.byref(box), 0, nbeams, q, n, C.byref(soa), dummy, nc, dummy,
-                                       None)
+                                       None, 0, None)
 
     assert dep(0, 5) == -1 and "no beam" in quarz_amd.last_error()
     assert dep(2, 5) == -1 and "bad extent" in quarz_amd.last_error()       # N < 2

@@ -477,6 +477,10 @@ def test_graph_batch_m1_equals_plain(qa, decks):
     ("linear.in", {"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
                    "driver.nparticles": 20000, "driver.sigma_r": 0.5, "electrons.ppc": 1, "time.steps": 4,
                    "time.dt": 5}, 4, 4, 4),
+    # a slow witness (gamma 3) slips about 100 slices per step, across several ranges: a range's deposit must
+    # not see the particles its own step has already pushed from earlier ranges
+    ("gain.in", {"time.steps": 4, "driver.nparticles": 20000, "xi.min": -8, "witness.gamma": 3,
+                 "witness.sigma_xi": 1.0, "witness.xi0": -4, "witness.n0": 0.001}, 3, 32, 3),
 ])
 def test_pipelined_steps_equal_plain(qa, decks, deck, ov, steps, chunks, depth):
     """Steps overlapped in time (step n sweeps a slice range once step n-1 has pushed the beam particles that
