@@ -505,6 +505,20 @@ def test_pipelined_steps_equal_plain(qa, decks, deck, ov, steps, chunks, depth):
     assert b.t == a.t
 
 
+def test_pipelined_with_many_hardware_queues(qa):
+    """The bench's setting: 24 hardware queues, so that the steps in flight really overlap (in this process
+    HIP has started with its default of 4).  A child process compares pipelined and plain steps for a witness
+    that slips several ranges per step and for the c2 workload (four steps in flight)."""
+    import os
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="24")
+    r = subprocess.run([sys.executable, os.path.join(here, "pipelined_check.py")], capture_output=True, text=True,
+                       timeout=280, env=env)
+    assert r.returncode == 0, r.stdout + r.stderr[-2000:]
+
+
 def test_pipelined_betatron_m1_equals_oracle(qa, oracle, decks):
     """Pipelined steps against the serial oracle directly: the betatron deck (m = 1, offset witness), the
     start-up step, then 10 steps with 5 in flight over 6 slice ranges, ending on the last sweep."""
