@@ -254,8 +254,8 @@ def main():
         if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < want:
             os.environ["GPU_MAX_HW_QUEUES"] = str(want)
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (one run per GPU, "
-              f"see DESIGN.md for how the path shards)", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (independent runs would "
+              f"shard across GPUs by run, one large run by slice blocks)", file=sys.stderr)
         sys.exit(2)
     if args.config == "c5":
         bench_batched(args)
