@@ -232,7 +232,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (c2: the workload's 20 time steps; c4: 50 Adam iterations; c5: 20)")
+                    help="timed steps (c2: the workload's 20 time steps; c4: 50 Adam iterations; c5: 3, after the "
+                         "2 warm-up steps: the workload's 5)")
     ap.add_argument("--warmup", type=int, default=2)   # the second step is a regular one: its kernel times are reported
     ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
@@ -245,7 +246,7 @@ def main():
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = 50 if args.config == "c4" else 20
+        args.steps = {"c4": 50, "c5": 3}.get(args.config, 20)
     # The pipelined run wants one hardware queue per step in flight; HIP's default is 4 per process, read when
     # HIP starts (so set before torch is imported), never above 32.  The batched launches (c4, c5) keep the
     # default: with 32 queues their long persistent launch ran about 1.5x slower (c4: 412 against 276 ms).
