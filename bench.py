@@ -10,10 +10,10 @@ bit-identical to stepping one at a time); --depth 0 steps them one at a time.  P
   value          plasma particle-slices per second of the whole timed steps (wall clock)
   plain_value    the same run stepped one step at a time (--plain-steps steps), for comparison
   beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
-  roofline       the sweep kernel against MI355X FP64 vector peak, algorithmic flops; valu_frac: the FP64 VALU
-                 lane operations rocprof counts per launch (profiles/sweep_fp64_valu.md) over the same time
-                 679 per particle-slice + 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP
-                 events on the simulation's stream
+  roofline       the sweep kernel against MI355X FP64 vector peak: algorithmic flops, 679 per particle-slice +
+                 508 per node-slice (m = 1; 361 / 84 for m = 0), kernel time from HIP events on the simulation's
+                 stream; valu_frac: the FP64 VALU lane operations rocprof counts per launch
+                 (profiles/sweep_fp64_valu.md) over the same time
   beam_roofline  the beam deposit and push against MI355X HBM bandwidth (8 TB/s), algorithmic bytes per step:
                  deposit 56 B read per live beam particle + bsrc (K M NC_B doubles) zeroed and written once;
                  push 56 B read + 48 B written per particle + fld (K M NC_F doubles) read once; times from HIP
