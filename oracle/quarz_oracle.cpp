@@ -25,8 +25,9 @@
 // (tests/test_solvers.cpp:34-56), manufactured-solution orders of L0/L1F/L1D/L2D (:58-124), the uniform
 // deposit (:126-143), the AB coefficients (tests/test_ab.cpp:17-43), the pusher identities and the printed
 // numbers of validation/pusher_test_output.txt, the std::mt19937_64 known answer, hat volumes in closed
-// form, and the blowout / linear-wake validation numbers of validation/reference_full.txt:1-8.
-// Functions without such a check are listed as "parity unpinned" in tests/test_oracle_pins.py's header.
+// form, the blowout / linear-wake validation numbers of validation/reference_full.txt:1-8, the m = 1 shift
+// validation with a rigid analytic driver (reference_full.txt:20-24), the 90-degree rotation covariance of the
+// m = 1 equations, and the mobile-ion validation of reference_quick.txt:12-16.
 
 #include <algorithm>
 #include <array>
@@ -1132,6 +1133,74 @@ static void beam_deposit(const Grid& g, const BeamGrid& bg, bool m1, Real q, int
     }
 }
 
+// e^{-z} I0(z) and e^{-z} I1(z) for z >= 0 by the polynomial approximations of Abramowitz & Stegun,
+// Handbook of Mathematical Functions, 9.8.1-9.8.4 (|relative error| < 2e-7).  The reference uses the same
+// published approximation for its analytic beam (src/Beam.cpp:217-233); the coefficients are A&S's table.
+static void scaled_bessel_i0_i1(Real z, Real& i0, Real& i1) {
+    if (z < Real(3.75)) {
+        const Real u = (z / Real(3.75)) * (z / Real(3.75));
+        // A&S 9.8.1 and 9.8.3 (polynomials in (z/3.75)^2), Horner order
+        const Real p0 = 1 + u * (3.5156229 + u * (3.0899424 + u * (1.2067492 + u * (0.2659732 + u * (0.0360768 + u * 0.0045813)))));
+        const Real p1 = z * (0.5 + u * (0.87890594 + u * (0.51498869 + u * (0.15084934 + u * (0.02658733 + u * (0.00301532 + u * 0.00032411))))));
+        const Real e = std::exp(-z);
+        i0 = p0 * e;
+        i1 = p1 * e;
+        return;
+    }
+    // A&S 9.8.2 and 9.8.4 (asymptotic polynomials in 3.75/z), already scaled by e^{-z}
+    const Real v = Real(3.75) / z;
+    const Real rs = Real(1) / std::sqrt(z);
+    i0 = rs * (0.39894228 + v * (0.01328592 + v * (0.00225319 + v * (-0.00157565 + v * (0.00916281 + v * (-0.02057706 + v * (0.02635537 + v * (-0.01647633 + v * 0.00392377))))))));
+    i1 = rs * (0.39894228 + v * (-0.03988024 + v * (-0.00362018 + v * (0.00163801 + v * (-0.01031555 + v * (0.02282967 + v * (-0.02895312 + v * (0.01787654 - v * 0.00420059))))))));
+}
+
+// Rigid analytic beam (deck key <beam>.analytic = 1), Beam::deposit_analytic (src/Beam.cpp:235-281): a Gaussian
+// of peak density n0 and rms radius sr whose centroid c(xi) = (x0, y0) + (xs, ys)(xi - xi0) sits at distance d
+// from the axis.  Its azimuthal modes at radius r follow from the generating function of I_n:
+// exp(-(r^2 + d^2 - 2 r d cos(th - phi)) / 2 sr^2) = e^{-(r-d)^2/2sr^2} [e^{-z} I0(z) + 2 e^{-z} I1(z) cos(th - phi) + ...],
+// z = r d / sr^2; mode 1 carries the phase e^{-i phi}.  The source is added at every node of every slice (no
+// particles).  Oracle-only: it drives the m = 1 validation runs of validation/run_validation.sh section 6.
+struct AnalyticBeam {
+    Real n0 = 0, sr = 0, xi0 = 0, sxi = 0, len = 0, gamma = 0, cut = -1e300, x0 = 0, y0 = 0, xs = 0, ys = 0;
+    bool flat = false;
+};
+static void beam_deposit_analytic(const Grid& g, const BeamGrid& bg, bool m1, Real q, const AnalyticBeam& a, Real* src,
+                                  int nc) {
+    const int M = g.N + 1;
+    const Real s2 = a.sr * a.sr;
+    const Real vz = std::sqrt(Real(1) - Real(1) / (a.gamma * a.gamma));
+    const Real omvz = Real(1) / (a.gamma * a.gamma * (Real(1) + vz));
+    for (int k = 0; k < bg.nloc; ++k) {
+        const Real xi = bg.xi_min + (bg.k0 + k) * bg.dxi;
+        if (xi < a.cut) continue;
+        const Real fxi = a.flat ? ((std::fabs(xi - a.xi0) <= Real(0.5) * a.len) ? Real(1) : Real(0))
+                                : std::exp(-Real(0.5) * (xi - a.xi0) * (xi - a.xi0) / (a.sxi * a.sxi));
+        if (fxi == Real(0)) continue;
+        const Real cx = a.x0 + a.xs * (xi - a.xi0), cy = a.y0 + a.ys * (xi - a.xi0);
+        const Real d = std::sqrt(cx * cx + cy * cy);
+        for (int j = 0; j < M; ++j) {
+            Real* S = src + (static_cast<size_t>(k) * M + j) * nc;
+            const Real r = g.r[j];
+            Real i0, i1;
+            scaled_bessel_i0_i1(r * d / s2, i0, i1);
+            const Real env = q * a.n0 * fxi * std::exp(-Real(0.5) * (r - d) * (r - d) / s2);
+            const Real m0 = env * i0;
+            S[B_RHO0] += m0;
+            S[B_RT0] += m0 * omvz;
+            S[B_JZ0] += m0 * vz;
+            if (m1 && d > Real(0)) {
+                const Real f1r = env * i1 * (cx / d), f1i = -env * i1 * (cy / d);
+                S[B_RHO1R] += f1r;
+                S[B_RHO1I] += f1i;
+                S[B_RT1R] += f1r * omvz;
+                S[B_RT1I] += f1i * omvz;
+                S[B_JZ1R] += f1r * vz;
+                S[B_JZ1I] += f1i * vz;
+            }
+        }
+    }
+}
+
 // Beam::advance without laser, spin, radiation reaction or scattering (src/Beam.cpp:948-1131)
 template <bool PAR>
 static void beam_advance(const Grid& g, const BeamGrid& bg, bool m1, Real qm, int mode, int i0, int Np, Real* x,
@@ -1217,6 +1286,8 @@ struct Beam {
     bool m1 = false;
     Real q = -1, m = 1;
     bool rigid = false;
+    bool analytic = false;
+    AnalyticBeam an;
     int mode = 0;
     bool started = false;
     double gref = 0, xiref = 0;
@@ -1238,6 +1309,27 @@ struct Beam {
         const std::string p = nm;
         const std::string lprof = cfg.get_string(p + ".profile", "gaussian");
         if (lprof != "gaussian" && lprof != "flattop") throw std::runtime_error("oracle: beam profile gaussian|flattop");
+        analytic = cfg.get_bool(p + ".analytic", false);
+        if (analytic) {   // rigid, no macro-particles (src/Beam.cpp:183-200)
+            rigid = true;
+            an.n0 = cfg.get_double(p + ".n0");
+            an.sr = cfg.get_double(p + ".sigma_r");
+            an.xi0 = xi_from_user(cfg.get_double(p + ".xi0"));
+            an.gamma = cfg.get_double(p + ".gamma");
+            an.flat = (lprof == "flattop");
+            if (an.flat) an.len = cfg.get_double(p + ".length");
+            else an.sxi = cfg.get_double(p + ".sigma_xi");
+            if (cfg.has(p + ".xi_cut_head")) an.cut = xi_from_user(cfg.get_double(p + ".xi_cut_head"));
+            an.x0 = cfg.get_double(p + ".x0", 0.0);
+            an.y0 = cfg.get_double(p + ".y0", 0.0);
+            an.xs = -cfg.get_double(p + ".x_slope", 0.0);
+            an.ys = -cfg.get_double(p + ".y_slope", 0.0);
+            if (!m1 && (an.x0 != 0 || an.y0 != 0 || an.xs != 0 || an.ys != 0))
+                throw std::runtime_error(p + ": transverse offsets/tilts need azimuthal mode 1 (modes = 1)");
+            set_particles({});
+            gref = 0; xiref = 0;
+            return;
+        }
         const double n0 = cfg.get_double(p + ".n0");
         const double sr = cfg.get_double(p + ".sigma_r");
         const double xi0 = xi_from_user(cfg.get_double(p + ".xi0"));
@@ -1704,9 +1796,11 @@ struct Sim {
     // the beam part of step_fields (src/Simulation.cpp:642-643)
     template <bool PAR> void deposit_beams() {
         std::fill(bsrc.begin(), bsrc.end(), 0.0);
-        for (auto& b : beams)
-            beam_deposit<PAR>(grid, box, m1, b.q, b.Np, b.x.data(), b.y.data(), b.px.data(), b.py.data(), b.pz.data(),
-                              b.xi.data(), b.w.data(), bsrc.data(), NCB);
+        for (auto& b : beams) {
+            if (b.analytic) beam_deposit_analytic(grid, box, m1, b.q, b.an, bsrc.data(), NCB);
+            else beam_deposit<PAR>(grid, box, m1, b.q, b.Np, b.x.data(), b.y.data(), b.px.data(), b.py.data(),
+                                   b.pz.data(), b.xi.data(), b.w.data(), bsrc.data(), NCB);
+        }
     }
 
     // plasma load, background and the B+ warm start of step_fields (src/Simulation.cpp:653-708)

@@ -314,3 +314,20 @@ class Sim:
 
     def write_axis(self, path):
         check(lib().or_sim_write_axis(self.h, path.encode()))
+
+
+def read_fields(path):
+    """A fields_*.bin file of format version 3 (src/Simulation.cpp:873-993) as a dict: "r" (M nodes), "xi" (K user
+    xi values, increasing: the head is the last row) and one (K, M) array per named component."""
+    with open(path, "rb") as f:
+        ver, M, K, _m1 = (int(v) for v in np.frombuffer(f.read(16), np.int32))
+        if ver != 3:
+            raise ValueError(f"{path}: format version {ver}")
+        f.read(8)   # time
+        out = {"r": np.frombuffer(f.read(8 * M), np.float64).copy(),
+               "xi": np.frombuffer(f.read(8 * K), np.float64).copy()}
+        nc = int(np.frombuffer(f.read(4), np.int32)[0])
+        for _ in range(nc):
+            name = f.read(16).split(b"\0")[0].decode()
+            out[name] = np.frombuffer(f.read(8 * K * M), np.float64).reshape(K, M).copy()
+    return out
