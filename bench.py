@@ -44,18 +44,71 @@ MEASURED_TRAFFIC = {"c2": 841.9e6}
 MEASURED_VALU = {"c2": 2.33e10, "c4": 1.33e11}
 
 
-def cpu_baseline(deck, nthreads):
+def physical_cores():
+    """Physical cores among the CPUs this process may run on (unique (package, core) pairs of /proc/cpuinfo)."""
+    cpus = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else set(range(os.cpu_count() or 1))
+    seen, cur = set(), {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if not line.strip():
+                    if cur.get("processor") in cpus:
+                        seen.add((cur.get("physical id"), cur.get("core id")))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                cur[k] = int(v) if k == "processor" else v
+        if cur.get("processor") in cpus:
+            seen.add((cur.get("physical id"), cur.get("core id")))
+    except OSError:
+        pass
+    return max(1, len(seen)) if seen else len(cpus)
+
+
+def _cpu_child(config, nthreads, nslices):
+    """Runs in a child process (bench.py --cpu-child): one sweep of the oracle's OpenMP build on the deck,
+    or its first nslices slices, with nthreads threads bound as the environment says.  Prints one JSON line."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
+    from quarz_amd import configs
     oracle_lib.set_threads(nthreads)
-    s = oracle_lib.Sim(deck, parallel=True)
+    s = oracle_lib.Sim(configs.deck(config), parallel=True)
     t0 = time.perf_counter()
-    s.step_fields()
+    if nslices > 0:
+        s.sweep_partial(nslices)
+    else:
+        s.step_fields()
     wall = time.perf_counter() - t0
-    sweep_s, _, psl = s.timing()
-    return {"value": psl / wall, "unit": "particle-slices/s", "cores": nthreads, "kind": "port",
-            "sample": f"one full sweep of the same deck ({s.K} slices, {int(psl)} particle-slices, "
-                      f"{wall:.2f} s, OpenMP atomics, g++ -O2 -ffp-contract=off)"}
+    _, _, psl = s.timing()
+    print(json.dumps({"wall": wall, "psl": psl, "K": s.K, "threads": nthreads}))
+
+
+def cpu_baseline(config, nthreads, nslices=0):
+    """The oracle's OpenMP build (the reference's loop structure: one parallel loop per kernel, atomic deposits,
+    serial Thomas; g++ -O3 -ffp-contract=off) timed on this host in a child process, with OMP_PROC_BIND=close and
+    OMP_PLACES=cores: nthreads threads and one thread.  MPI-style context: the one-thread rate times the cores
+    at 90% efficiency (the reference's xi-pipelined MPI ranks scale near linearly, its OpenMP does not)."""
+    import subprocess
+
+    def run(n):
+        env = dict(os.environ, OMP_NUM_THREADS=str(n), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", config, str(n), str(nslices)],
+                             env=env, capture_output=True, text=True, timeout=900, check=True)
+        return json.loads(out.stdout.strip().splitlines()[-1])
+
+    par = run(nthreads)
+    one = run(1)
+    rate, rate1 = par["psl"] / par["wall"], one["psl"] / one["wall"]
+    what = f"the first {nslices} slices of one sweep" if nslices > 0 else f"one full sweep ({par['K']} slices)"
+    return {"value": rate, "unit": "particle-slices/s", "cores": nthreads, "kind": "port",
+            "sample": f"{what} of the same deck, {int(par['psl'])} particle-slices: {par['wall']:.2f} s on "
+                      f"{nthreads} threads, {one['wall']:.2f} s on one (OpenMP atomics, OMP_PROC_BIND=close, "
+                      f"OMP_PLACES=cores, g++ -O3 -ffp-contract=off)",
+            "one_thread_value": rate1,
+            "host_physical_cores": physical_cores(),
+            "mpi_style_estimate": rate1 * nthreads * 0.90,
+            "mpi_style_note": "context only: one-thread rate x cores x 0.90"}
 
 
 def bench_batched(args):
@@ -138,7 +191,7 @@ def bench_batched(args):
         "plain_value": ppsl / pwall if ppsl > 0 else None,
         "plain_steps": args.plain_steps,
         "batched_vs_plain": (psl / wall) / (ppsl / pwall) if ppsl > 0 else None,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel (batched)",
                      "flops_per_launch": flops,
                      "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
@@ -217,7 +270,7 @@ def bench_gradient(args):
         "launch_shapes": shapes,
         "plain_value": ppsl / pwall,
         "batched_vs_plain": (psl / wall) / (ppsl / pwall),
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": None, "kernel": "sweep_kernel (batched)",
                      "flops_per_launch": flops,
                      "valu_fp64_lane_ops": MEASURED_VALU["c4"],
@@ -228,7 +281,27 @@ def bench_gradient(args):
     print(json.dumps(out))
 
 
+def calibrate_copy():
+    """bench.py --calibrate-copy: one plain device copy of known size (2**27 float64: 1 GiB read, 1 GiB
+    written), the reference reading for the HBM byte counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) that
+    scale the kernels' counter bytes to HBM bytes."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": n * 8}))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--cpu-child":
+        _cpu_child(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "--calibrate-copy":
+        calibrate_copy()
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
@@ -285,9 +358,13 @@ def main():
     psl = pushes = dep_particles = 0.0
     sweep_ms = beam_ms = dep_ms = 0.0
     t0 = time.perf_counter()
+    timed_sweep_ms, launches = None, 1
     if pipelined:
         psl, pushes = sim.run_pipelined(args.steps, chunks=args.chunks, depth=depth)
-        dep_particles = sim.stats()["beam_deposited"] * args.steps
+        st = sim.stats()
+        dep_particles = st["beam_deposited"] * args.steps
+        # kernel time of the timed steps: every range launch bracketed by HIP events on its own stream
+        timed_sweep_ms, launches = st["sweep_ms"], int(st["sweep_launches"])
     else:
         for _ in range(args.steps):
             sim.step()
@@ -319,7 +396,9 @@ def main():
         plain_value = ppsl / (time.perf_counter() - tp)
     fp, fn = FLOPS[modes]
     flops_per_sweep = fp * psl / args.steps + fn * sim.M * sim.K
-    t_kernel = sweep_ms / args.steps / 1e3
+    # the dominant kernel's rate: algorithmic flops per launch over the average launch time of the timed steps
+    # (pipelined: flops of one step's slice ranges over the sum of their launch times)
+    t_kernel = (timed_sweep_ms if timed_sweep_ms is not None else sweep_ms / args.steps) / 1e3
     achieved = flops_per_sweep / t_kernel
     out = {
         "metric": METRIC,
@@ -342,23 +421,27 @@ def main():
         "plain_value": plain_value,
         "beam_pushes_per_s": pushes / wall,
         "sweep_ms": sweep_ms / args.steps,
+        "timed_sweep_launch_ms_per_step": timed_sweep_ms,
         "beam_deposit_ms": dep_ms / args.steps,
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
         "launch_shapes": shapes,
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
+        "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
                      "kernel": "sweep_kernel",
-                     "flops_per_launch": flops_per_sweep,
+                     "launches_per_step": launches,
+                     "avg_launch_ms": t_kernel * 1e3 / launches,
+                     "flops_per_launch": flops_per_sweep / launches,
                      "valu_fp64_lane_ops": MEASURED_VALU.get(args.config),
                      "valu_frac": (MEASURED_VALU[args.config] / t_kernel / FP64_PEAK
                                    if args.config in MEASURED_VALU and t_kernel > 0 else None),
                      "aggregate_achieved": flops_per_sweep * args.steps / wall / 1e12,
                      "aggregate_frac": flops_per_sweep * args.steps / wall / FP64_PEAK,
-                     "note": "FP64 vector work; MI355X's FP64 vector and matrix peaks are both 78.6 TFLOP/s. "
-                             "achieved: one sweep launch (HIP events on the run's stream, last warm-up step); "
-                             "aggregate_achieved: every sweep of the timed steps over their wall time (pipelined "
-                             "steps overlap on several CUs)"},
+                     "note": "FP64 vector work, no dense contraction (no MFMA); MI355X's FP64 vector peak is 78.6 "
+                             "TFLOP/s. achieved: algorithmic flops per sweep launch over the average launch time "
+                             "of the timed steps (HIP events around every launch on its stream); aggregate_achieved: "
+                             "every sweep of the timed steps over their wall time (pipelined steps overlap on "
+                             "several CUs)"},
     }
     nb_dep, nb_push = dep_particles / args.steps, pushes / args.steps
     ncf, ncb = (18, 15) if modes else (6, 5)
@@ -376,8 +459,8 @@ def main():
     beam["frac"] = tot_b / tot_t / HBM_PEAK if tot_t > 0 else None
     out["beam_roofline"] = beam
     if not args.no_cpu:
-        ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(deck, min(16, ncpu))
+        # the GPU box's share of the host is 16 cores per GPU
+        out["cpu_baseline"] = cpu_baseline(args.config, min(16, physical_cores()))
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

@@ -107,10 +107,14 @@ const char* qz_last_error(void);
 /* Library / device information: gcnArchName and the number of CUs of the current device. */
 int qz_device_info(char* arch, int arch_len, int* n_cu);
 
-/* Batched Thomas solve of `batch` systems of size M, system s at offset s*M of every array
- * (TridiagSolver::solve, Thomas branch, src/Tridiag.cpp:111-126).  a[0] and c[M-1] are not used. */
+/* Batched Thomas solve of `batch` systems of size M (TridiagSolver::solve, Thomas branch,
+ * src/Tridiag.cpp:111-126): element i of system s at s * sys_stride + i * elem_stride of a, b, c, d and x
+ * (strides in elements; system-major: sys_stride = M, elem_stride = 1; element-major: sys_stride = 1,
+ * elem_stride = batch).  a[0] and c[M-1] are not used.  work: device scratch of
+ * qz_tridiag_workspace_doubles(M, batch) doubles for the c' of every system; the call allocates nothing. */
+size_t qz_tridiag_workspace_doubles(int M, int batch);
 int qz_tridiag_thomas_batched(const double* a, const double* b, const double* c, const double* d, double* x, int M,
-                              int batch, void* stream);
+                              int batch, long long sys_stride, long long elem_stride, double* work, void* stream);
 
 /* Beam deposit: adds the sources of one beam to bsrc (nloc, N+1, nc)
  * (Beam::deposit, src/Beam.cpp:725-785).  Particles are summed per node in index order.
@@ -124,10 +128,12 @@ int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
  * pstep (optional, a host array of nbeams device pointers, as qz_beam_advance_slices marks them): particles
  * whose marker is `marker` or later are left out.  A step run range by range deposits a range after it has pushed
  * earlier ranges; the particles it pushed there were deposited before their push and must not be seen again
- * at their new places (a slow particle can slip into a later range). */
+ * at their new places (a slow particle can slip into a later range).  live_count (optional, device): adds the
+ * live particles whose lower slice is in rows [k0, k0 + nloc), so the windows of one step count every live
+ * particle of the box once. */
 int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
                           const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
-                          const int* const* pstep, int marker, void* stream);
+                          const int* const* pstep, int marker, unsigned long long* live_count, void* stream);
 
 /* Beam push with the stored fields fld (nloc, N+1, nc): gather, pusher (0 vay, 1 boris, 2 hc, 3 imp),
  * drift; move = 0 is the explicit leapfrog start-up kick (Beam::advance, src/Beam.cpp:948-1131). */
@@ -138,11 +144,13 @@ int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
  * [s_lo, s_hi) and whose marker pstep[i] (np ints, caller-owned) is below `step`; pushed particles get
  * pstep[i] = step.  Markers grow with the step, so a particle a later step has already pushed is not pushed
  * again.  A step pushed range by range, in increasing slice order, then pushes every live particle exactly
- * once, as Beam::advance does (particles only slip to later slices).  Not in the reference:
+ * once, as Beam::advance does (particles only slip to later slices); pushed_count (optional, device) adds the
+ * live particles this call pushes.  Not in the reference:
  * it serves the pipelined run, where slice ranges are pushed as soon as their fields exist. */
 int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm, int pusher,
                            int np, const qz_beam_soa* p, const double* fld, int nc, double dt_kick, double dt,
-                           int move, int s_lo, int s_hi, int step, int* pstep, void* stream);
+                           int move, int s_lo, int s_hi, int step, int* pstep, unsigned long long* pushed_count,
+                           void* stream);
 
 /* The 17 weighted beam sums relative to gref and xiref (Beam::sums, src/Beam.cpp:1133-1155): the particles
  * are copied to the host and summed in index order, as the reference does (bit-identical). */
@@ -166,9 +174,17 @@ int qz_sweep(const qz_sweep_args* args, void* stream);
 
 /* Many independent runs in one persistent launch, one workgroup per run (not in the reference). */
 int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream);
+/* Frees the argument staging buffer qz_sweep / qz_sweep_batched keep for `stream` on the current device, after
+ * the stream's work is done: call it before destroying a stream the sweeps were launched on (a new stream can
+ * reuse the handle).  No counterpart in the reference. */
+int qz_sweep_release(void* stream);
 
 /* ---- whole simulation (Simulation, src/Simulation.hpp:67-103) on a deck (src/Config.cpp format) ---- */
 typedef struct qz_sim qz_sim;
+/* The deck reader alone (src/Config.cpp's format; host only, no device work): every key after the overrides as
+ * "key = value" lines in key order, written to out when cap exceeds the returned length (bytes, without the
+ * terminating 0); a negative code for a deck the reader refuses (qz_last_error() says why). */
+long qz_deck_keys(const char* deck, const char* overrides, char* out, long cap);
 /* deck: text of an input deck; overrides: "key=value" lines (src/main.cpp:82-93). */
 int qz_sim_create(const char* deck, const char* overrides, qz_sim** out);
 void qz_sim_destroy(qz_sim* s);
@@ -190,7 +206,10 @@ int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
 int qz_sim_run_pipelined(qz_sim* s, int nrun, int nchunks, int depth, double* out2);
 /* what: fld diag bsrc nodes, beam<b>.<x|y|px|py|pz|xi|w>, sp<s>.<x|y|px|py|dl|w>; returns the count */
 long qz_sim_get(qz_sim* s, const char* what, double* host_out, long cap);
+/* beam log values of beam b (Beam::from_sums of the sums reduced on the device, qz_beam_sums_device order):
+ * t npart charge gamma_mean gamma_rms xi_mean xi_rms r_rms emit_nx x_mean y_mean emit_ny sigma_x sigma_y alive */
 int qz_sim_beam_diag(qz_sim* s, int b, double* out15);
+/* the 17 sums of Beam::sums in the reference's index order (host copy of the particles) and gref, xiref */
 int qz_sim_beam_sums(qz_sim* s, int b, double* out17, double* refs2);
 /* No counterpart in the reference: the differentiable workload (a witness whose current follows a
  * piecewise-linear profile, deck key <beam>.pwl_weights on a flat-top beam).  set_beam_profile gives beam b
@@ -203,8 +222,10 @@ int qz_sim_rewind(qz_sim* s);
 /* times of the last sweep and beam push (ms, HIP events on the simulation's stream) */
 int qz_sim_timing(const qz_sim* s, double* out2);
 /* work of the last step: particle-slices of the sweep, beam particles pushed, beam particles deposited,
- * sweep kernel ms, beam deposit ms, beam push ms */
-int qz_sim_stats(const qz_sim* s, double* out6);
+ * sweep kernel ms, beam deposit ms, beam push ms, sweep kernel launches.  After qz_sim_run_pipelined: the sweep
+ * kernel ms are the sum of every range launch's time (HIP events around each launch) divided by the steps, and
+ * the launches are the ranges of one step; the beam times are 0 (deposits and pushes run inside the ranges). */
+int qz_sim_stats(const qz_sim* s, double* out7);
 /* wall-clock ticks (100 MHz) of the QZ_NPHASE phases of the last sweep, measured by the first thread of the
  * run (a phase includes that thread's wait at the barrier that ends it): geometry, sort, deposit
  * contributions, deposit node sums, combine, B_z rhs, psi/E_z/B_z solves, W+, S+ gather, S+ contributions,

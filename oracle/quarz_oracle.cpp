@@ -18,7 +18,7 @@
 //     the reference's OpenMP build (README.md:1498).  It is timed as the CPU baseline; it is never used
 //     as a checker.
 //
-// Compiled with g++ -O2 -ffp-contract=off (no -march=native): a default x86-64 build without FMA, as a
+// Compiled with g++ -O3 -ffp-contract=off (no -march=native): a default x86-64 build without FMA, as a
 // default Release build of the reference.
 //
 // Parity pins (tests/test_oracle_*.py): Thomas on random diagonally dominant systems
@@ -1461,6 +1461,41 @@ struct Beam {
         }
         return S;
     }
+    // The same 17 sums in the fixed tree order of the product's on-device reduction (qz_beam_sums_device):
+    // blocks of 4096 particles; in a block, lane t (of 256) accumulates particles t, t + 256, ... in index order;
+    // the 256 lane partials are halved pairwise (lane t adds lane t + h, h = 128, 64, ..., 1); the block
+    // partials are added in block order.  Not in the reference (its Beam::sums is the index-order loop above):
+    // it is the summation order the product's beam log and differentiable objective use.
+    std::array<double, 17> sums_tree() const {
+        constexpr int LANES = 256, CHUNK = 16 * LANES;
+        std::array<double, 17> out{};
+        for (long lo = 0; lo < Np; lo += CHUNK) {
+            const long hi = std::min<long>(Np, lo + CHUNK);
+            std::vector<std::array<double, 17>> acc(LANES);
+            for (auto& a : acc) a.fill(0.0);
+            for (int t = 0; t < LANES; ++t) {
+                auto& A = acc[t];
+                for (long i = lo + t; i < hi; i += LANES) {
+                    const double ww = w[i];
+                    if (ww == 0) continue;
+                    A[16] += 1;
+                    const double PX = px[i], PY = py[i], PZ = pz[i];
+                    const double gg = std::sqrt(1 + PX * PX + PY * PY + PZ * PZ);
+                    const double X = x[i], Y = y[i];
+                    const double G = gg - gref, XI = xi[i] - xiref;
+                    const double v[15] = {G, G * G, XI, XI * XI, X * X + Y * Y, X, Y, X * X, Y * Y, PX, PY, PX * PX,
+                                          PY * PY, X * PX, Y * PY};
+                    A[0] += ww;
+                    for (int c = 0; c < 15; ++c) A[c + 1] += ww * v[c];
+                }
+            }
+            for (int h = LANES / 2; h > 0; h >>= 1)
+                for (int t = 0; t < h; ++t)
+                    for (int c = 0; c < 17; ++c) acc[t][c] += acc[t + h][c];
+            for (int c = 0; c < 17; ++c) out[c] += acc[0][c];
+        }
+        return out;
+    }
     // src/Beam.cpp:1157-1192
     BeamDiag from_sums(const std::array<double, 17>& S, Real t) const {
         BeamDiag d;
@@ -2252,10 +2287,11 @@ int or_sim_beam_sums(void* h, int b, double* out17, double* refs) {
     return 0;
 }
 // BeamDiag: t npart charge gamma_mean gamma_rms xi_mean xi_rms r_rms emit_nx x_mean y_mean emit_ny sigma_x sigma_y alive
-int or_sim_beam_diag(void* h, int b, double* out15) {
+// tree = 0: from the index-order sums (the reference's); tree = 1: from the on-device reduction's order
+int or_sim_beam_diag(void* h, int b, int tree, double* out15) {
     Sim& s = *static_cast<Sim*>(h);
     if (b < 0 || b >= static_cast<int>(s.beams.size())) return -1;
-    const BeamDiag d = s.beams[b].from_sums(s.beams[b].sums(), s.t);
+    const BeamDiag d = s.beams[b].from_sums(tree ? s.beams[b].sums_tree() : s.beams[b].sums(), s.t);
     const double v[15] = {d.t, d.npart, d.charge, d.gamma_mean, d.gamma_rms, d.xi_mean, d.xi_rms, d.r_rms,
                           d.emit_nx, d.x_mean, d.y_mean, d.emit_ny, d.sigma_x, d.sigma_y, double(d.alive)};
     std::copy(v, v + 15, out15);

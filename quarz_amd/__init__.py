@@ -50,14 +50,17 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.qz_last_error.restype = C.c_char_p
         L.qz_device_info.argtypes = [C.c_char_p, C.c_int, IP]
-        L.qz_tridiag_thomas_batched.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_void_p]
+        L.qz_tridiag_workspace_doubles.restype = C.c_size_t
+        L.qz_tridiag_workspace_doubles.argtypes = [C.c_int, C.c_int]
+        L.qz_tridiag_thomas_batched.argtypes = [C.c_void_p] * 5 + [C.c_int, C.c_int, C.c_longlong, C.c_longlong,
+                                                                  C.c_void_p, C.c_void_p]
         L.qz_beam_deposit_workspace_bytes.restype = C.c_size_t
         L.qz_beam_deposit_workspace_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
         L.qz_beam_deposit.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.qz_beam_deposit_beams.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_int, DP, IP,
                                             C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
-                                            C.c_int, C.c_void_p]
+                                            C.c_int, C.c_void_p, C.c_void_p]
         L.qz_beam_advance.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_int, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double,
                                       C.c_double, C.c_int, C.c_void_p]
@@ -71,6 +74,8 @@ def lib():
         L.qz_sim_create.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
         L.qz_sim_destroy.argtypes = [C.c_void_p]
         L.qz_sim_info.argtypes = [C.c_void_p, IP]
+        L.qz_deck_keys.restype = C.c_long
+        L.qz_deck_keys.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_long]
         L.qz_sim_step.argtypes = [C.c_void_p]
         L.qz_sim_step_fields.argtypes = [C.c_void_p]
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
@@ -127,6 +132,21 @@ DIAG_KEYS = ["t", "npart", "charge", "gamma_mean", "gamma_rms", "xi_mean", "xi_r
              "y_mean", "emit_ny", "sigma_x", "sigma_y", "alive"]
 
 
+def deck_keys(deck, overrides=None):
+    """The keys of a deck (text) after key=value overrides, as read by the library's deck reader: {key: value}."""
+    ov = "\n".join(f"{k}={v}" for k, v in (overrides or {}).items())
+    n = lib().qz_deck_keys(deck.encode(), ov.encode(), None, 0)
+    if n < 0:
+        raise QuarzError(last_error())
+    buf = C.create_string_buffer(n + 1)
+    lib().qz_deck_keys(deck.encode(), ov.encode(), buf, n + 1)
+    out = {}
+    for line in buf.value.decode().splitlines():
+        k, _, v = line.partition(" = ")
+        out[k] = v
+    return out
+
+
 class Simulation:
     """One QUARZ run on a deck (text or path) with key=value overrides; the step runs on the GPU."""
 
@@ -168,10 +188,12 @@ class Simulation:
         return out
 
     def stats(self):
-        """Work of the last step: particle_slices, beam_pushed, beam_deposited, sweep_ms, beam_dep_ms, beam_push_ms."""
-        out = np.zeros(6)
+        """Work of the last step: particle_slices, beam_pushed, beam_deposited, sweep_ms, beam_dep_ms, beam_push_ms,
+        sweep_launches (after run_pipelined: sweep_ms sums every range launch of a step, over the steps)."""
+        out = np.zeros(7)
         check(lib().qz_sim_stats(self.h, out.ctypes.data_as(DP)))
-        return dict(zip(["particle_slices", "beam_pushed", "beam_deposited", "sweep_ms", "beam_dep_ms", "beam_push_ms"], out))
+        return dict(zip(["particle_slices", "beam_pushed", "beam_deposited", "sweep_ms", "beam_dep_ms", "beam_push_ms",
+                         "sweep_launches"], out))
 
     PHASES = ["geometry", "sort", "dep_contrib", "dep_sums", "combine", "bz_rhs", "solves", "wplus", "s_gather",
               "s_contrib", "s_sums", "picard_stage", "picard_rhs", "picard_solve", "store", "push"]
@@ -349,10 +371,21 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
-def thomas_batched(a, b, c, d, x, stream=None):
-    """Batched Thomas solve on device tensors (batch, M), float64."""
-    batch, M = d.shape
-    check(lib().qz_tridiag_thomas_batched(_ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), M, batch, stream))
+def thomas_batched(a, b, c, d, x, work=None, element_major=False, stream=None):
+    """Batched Thomas solve on float64 device tensors: (batch, M) system-major, or (M, batch) with
+    element_major=True (element i of system s at i * batch + s).  work: a device tensor of at least
+    qz_tridiag_workspace_doubles(M, batch) doubles (made here if None)."""
+    if element_major:
+        M, batch = d.shape
+        ss, es = 1, batch
+    else:
+        batch, M = d.shape
+        ss, es = M, 1
+    if work is None:
+        import torch
+        work = torch.empty(max(1, lib().qz_tridiag_workspace_doubles(M, batch)), dtype=torch.float64, device=d.device)
+    check(lib().qz_tridiag_thomas_batched(_ptr(a), _ptr(b), _ptr(c), _ptr(d), _ptr(x), M, batch, ss, es, _ptr(work),
+                                          stream))
 
 
 def radial_grid(r, inv_h, inv_V):
@@ -378,7 +411,7 @@ def beam_deposit_beams(grid, box, modes, qs, ps, bsrc, work, stream=None):
     q = (C.c_double * n)(*qs)
     npa = (C.c_int * n)(*[p["x"].numel() for p in ps])
     check(lib().qz_beam_deposit_beams(C.byref(grid), C.byref(box), modes, n, q, npa, soas, _ptr(bsrc), bsrc.shape[-1],
-                                      _ptr(work), None, 0, stream))
+                                      _ptr(work), None, 0, None, stream))
 
 
 def beam_deposit_workspace_bytes(np_, nloc, N):

@@ -1,6 +1,8 @@
-// Input-deck reader of the quarz_amd host: the format of the reference (src/Config.hpp:1-11,
-// src/Config.cpp:48-236): "key = value" statements, flat or in blocks, ';' separators, '#' comments,
-// named blocks (bunch/beam/species/diag), units with units.n0, a key given twice is an error.
+// Input-deck reader of the quarz_amd host.  It reads the reference's deck format (src/Config.hpp:1-11; the
+// reader it replaces is src/Config.cpp): "key = value" statements, flat or in nested blocks that prefix their
+// keys ("xi { step = 0.01 }"), named blocks that also join a list ("bunch driver {" joins beams), ';' and newlines
+// as separators, '#' comments, numbers with units converted with the reference density units.n0, and a key set
+// twice is an error.
 #pragma once
 
 #include <map>
@@ -35,6 +37,7 @@ public:
     std::vector<std::pair<double, double>> get_pairs(const std::string& key) const;
 
     std::vector<std::string> unused_keys() const;
+    std::string dump() const;   // "key = value" lines in key order
 
 private:
     const std::string& raw(const std::string& key) const;

@@ -52,169 +52,227 @@ template <class T> void DevBuf<T>::download(std::vector<T>& h, hipStream_t st) c
     h.resize(n);
     if (n) hip_check(hipMemcpyAsync(h.data(), p, sizeof(T) * n, hipMemcpyDeviceToHost, st), "download");
     hip_check(hipStreamSynchronize(st), "download sync");   // the host reads h right after
-    hip_check(hipStreamSynchronize(st), "sync");
 }
 template struct DevBuf<Real>;
 template struct DevBuf<int>;
 template struct DevBuf<char>;
 template struct DevBuf<unsigned long long>;
+template struct DevBuf<qz_sweep_args>;
+
+// Stage n sweep argument blocks in the run's own device buffer and launch them on `st`.  The buffer is reused
+// only by this run's later launches on the same stream; a larger batch first waits for the stream.
+static void launch_sweeps(DevBuf<qz_sweep_args>& buf, const qz_sweep_args* host, int n, hipStream_t st) {
+    for (int i = 0; i < n; ++i)
+        if (qz_validate_sweep(host[i]) != QZ_OK) throw std::runtime_error(qz_last_error());
+    if (buf.n < static_cast<size_t>(n)) {
+        hip_check(hipStreamSynchronize(st), "sync");
+        buf.alloc(static_cast<size_t>(n));
+    }
+    hip_check(hipMemcpyAsync(buf.p, host, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st), "sweep arguments");
+    qz_check(qz_sweep_launch(buf.p, host, n, st));
+}
 
 // ---------------------------------------------------------------------------- RadialGrid (src/RadialGrid.cpp)
+// The node lists of the three grid types.  Node positions, cell sizes and hat volumes are formed with the same
+// floating-point operations as the reference's, so every array derived from them agrees bit for bit; the input
+// checks and their messages are this library's.
+namespace {
+void grid_require(bool ok, const std::string& what) {
+    if (!ok) throw std::runtime_error("radial grid: " + what);
+}
+
+// Spacing law of a graded grid: inside region i (up to its end radius) the cells aim at that region's spacing,
+// and a finer region further out is approached so that the cells there arrive at its spacing, growing by at most
+// the factor `ratio` per cell on the way (the target at x is at most h_i + (ratio - 1) * (start_i - x)).
+struct Grading {
+    std::vector<std::pair<double, double>> regions;   // (end radius, spacing), end radii increasing
+    Real grow;                                        // ratio - 1
+    Real target(Real x) const {
+        Real h = regions.back().second;
+        for (const auto& reg : regions)
+            if (x < reg.first) {
+                h = reg.second;
+                break;
+            }
+        Real start = 0;   // region i starts where region i - 1 ends
+        for (const auto& reg : regions) {
+            if (start > x) h = std::min(h, grow > 0 ? Real(reg.second + grow * (start - x)) : Real(reg.second));
+            start = reg.first;
+        }
+        return h;
+    }
+};
+}  // namespace
+
 std::vector<Real> RadialGrid::make_uniform(Real R, Real dr) {   // :14-20
-    if (R <= 0 || dr <= 0) throw std::runtime_error("grid: R and dr must be positive");
-    const int n = std::max(1, static_cast<int>(std::lround(R / dr)));
-    std::vector<Real> nodes(n + 1);
-    for (int j = 0; j <= n; ++j) nodes[j] = R * Real(j) / Real(n);
+    grid_require(R > 0, "grid.rmax must be positive");
+    grid_require(dr > 0, "grid.dr must be positive");
+    const int cells = std::max(1, static_cast<int>(std::lround(R / dr)));
+    std::vector<Real> nodes;
+    nodes.reserve(static_cast<size_t>(cells) + 1);
+    for (int j = 0; j <= cells; ++j) nodes.push_back(R * Real(j) / Real(cells));
     return nodes;
 }
 
+// March outwards cell by cell: each new cell takes the target spacing at its middle (estimated with the previous
+// cell's size), limited to a factor `ratio` from the previous cell; the first cell takes the target at the axis.
+// The last node is dropped when it overshoots R by more than the one before falls short, and the nodes are scaled
+// to end exactly at R.
 std::vector<Real> RadialGrid::make_regions(Real R, const std::vector<std::pair<double, double>>& regions,
                                            Real max_ratio) {   // :27-71
-    if (regions.empty()) throw std::runtime_error("grid: grid.regions is empty");
-    if (max_ratio < 1.0) throw std::runtime_error("grid: grid.max_ratio must be >= 1");
+    grid_require(!regions.empty(), "grid.regions lists no region");
+    grid_require(max_ratio >= 1.0, "grid.max_ratio below 1 would shrink every cell");
     for (size_t i = 0; i < regions.size(); ++i) {
-        if (regions[i].second <= 0) throw std::runtime_error("grid: region spacing must be > 0");
-        if (i > 0 && regions[i].first <= regions[i - 1].first) throw std::runtime_error("grid: region end radii must increase");
+        grid_require(regions[i].second > 0, "every region spacing of grid.regions must be positive");
+        grid_require(i == 0 || regions[i].first > regions[i - 1].first,
+                     "the end radii of grid.regions must increase from region to region");
     }
-    const Real grow = max_ratio - 1.0;
-    auto target = [&](Real x) {
-        Real ht = regions.back().second;
-        for (const auto& reg : regions)
-            if (x < reg.first) { ht = reg.second; break; }
-        Real start = 0;   // arrive at a finer region with its spacing
-        for (const auto& reg : regions) {
-            if (start > x && grow > 0) ht = std::min(ht, Real(reg.second + grow * (start - x)));
-            if (start > x && grow == 0) ht = std::min(ht, Real(reg.second));
-            start = reg.first;
-        }
-        return ht;
-    };
+    const Grading law{regions, max_ratio - 1.0};
     std::vector<Real> nodes{0.0};
-    Real x = 0, hprev = target(0);
+    Real x = 0, prev = law.target(0);
     while (x < R * (1 - 1e-12)) {
-        const Real ht = target(x + 0.5 * hprev);
-        Real hnew = std::clamp(ht, hprev / max_ratio, hprev * max_ratio);
-        if (nodes.size() == 1) hnew = target(0);
-        x += hnew;
+        Real h = std::clamp(law.target(x + 0.5 * prev), prev / max_ratio, prev * max_ratio);
+        if (nodes.size() == 1) h = law.target(0);
+        x += h;
         nodes.push_back(x);
-        hprev = hnew;
-        if (nodes.size() > 10000000) throw std::runtime_error("grid: too many nodes");
+        prev = h;
+        grid_require(nodes.size() <= 10000000, "grid.regions asks for more than 10^7 cells");
     }
     const size_t n = nodes.size();
-    if (n >= 3 && (nodes[n - 1] - R) > (R - nodes[n - 2])) nodes.pop_back();
+    if (n >= 3 && nodes[n - 1] - R > R - nodes[n - 2]) nodes.pop_back();
     const Real scale = R / nodes.back();
-    for (auto& v : nodes) v *= scale;
+    for (Real& v : nodes) v *= scale;
     nodes.back() = R;
     return nodes;
 }
 
+// one node per line; '#' starts a comment; blank lines are skipped
 std::vector<Real> RadialGrid::read_file(const std::string& filename) {   // :73-85
     std::ifstream in(filename);
-    if (!in) throw std::runtime_error("grid: cannot open grid file '" + filename + "'");
+    grid_require(static_cast<bool>(in), "grid.file '" + filename + "' cannot be opened");
     std::vector<Real> nodes;
-    std::string line;
-    while (std::getline(in, line)) {
-        const auto hash = line.find('#');
-        if (hash != std::string::npos) line.erase(hash);
-        if (line.find_first_not_of(" \t\r\n") == std::string::npos) continue;
-        nodes.push_back(std::stod(line));
+    int lineno = 0;
+    for (std::string line; std::getline(in, line);) {
+        ++lineno;
+        line = line.substr(0, line.find('#'));
+        const char* p = line.c_str();
+        char* end = nullptr;
+        const double v = std::strtod(p, &end);
+        if (end == p) {
+            grid_require(line.find_first_not_of(" \t\r") == std::string::npos,
+                         "grid.file '" + filename + "' line " + std::to_string(lineno) + " is not a radius");
+            continue;
+        }
+        nodes.push_back(v);
     }
     return nodes;
 }
 
 void RadialGrid::finalize(const std::vector<Real>& nodes, hipStream_t st) {   // :106-148
-    if (nodes.size() < 3) throw std::runtime_error("grid: need at least 2 cells");
-    if (std::abs(nodes[0]) > 0) throw std::runtime_error("grid: first node must be r=0");
+    grid_require(nodes.size() >= 3, "a grid needs at least two cells");
+    grid_require(nodes[0] == 0, "the first node must sit on the axis (r = 0)");
     for (size_t j = 1; j < nodes.size(); ++j)
-        if (!(nodes[j] > nodes[j - 1])) throw std::runtime_error("grid: nodes must be strictly increasing");
+        grid_require(nodes[j] > nodes[j - 1], "the nodes must increase strictly (node " + std::to_string(j) + ")");
     r = nodes;
     N = static_cast<int>(r.size()) - 1;
     R = r[N];
-    h.resize(N);
-    rmid.resize(N);
-    for (int j = 0; j < N; ++j) {
-        h[j] = r[j + 1] - r[j];
-        rmid[j] = 0.5 * (r[j + 1] + r[j]);
+    h.assign(N, 0.0);
+    rmid.assign(N, 0.0);
+    inv_h.assign(N, 0.0);
+    // hat volume of node j, 2 pi Int W_j r dr: cell c contributes r_c h/2 + h^2/6 to its inner node and
+    // r_c h/2 + h^2/3 to its outer node (the inner part first, as the reference adds them)
+    std::vector<Real> half(N + 1, 0.0);
+    for (int c = 0; c < N; ++c) {
+        h[c] = r[c + 1] - r[c];
+        rmid[c] = 0.5 * (r[c + 1] + r[c]);
+        inv_h[c] = 1.0 / h[c];
+        half[c + 1] += r[c] * h[c] / 2 + h[c] * h[c] / 3;
     }
-    // hat volumes: left part r_{j-1} h/2 + h^2/3, right part r_j h/2 + h^2/6
     V.assign(N + 1, 0.0);
+    inv_V.assign(N + 1, 0.0);
     for (int j = 0; j <= N; ++j) {
-        Real v = 0;
-        if (j > 0) v += r[j - 1] * h[j - 1] / 2 + h[j - 1] * h[j - 1] / 3;
+        Real v = half[j];
         if (j < N) v += r[j] * h[j] / 2 + h[j] * h[j] / 6;
         V[j] = 2 * PI * v;
+        inv_V[j] = 1.0 / V[j];
     }
-    inv_h.resize(N);
-    inv_V.resize(N + 1);
-    for (int j = 0; j <= N; ++j) inv_V[j] = 1.0 / V[j];
-    for (int j = 0; j < N; ++j) inv_h[j] = 1.0 / h[j];
     d_r.upload(r, st);
     d_inv_h.upload(inv_h, st);
     d_inv_V.upload(inv_V, st);
 }
 
 // ---------------------------------------------------------------------------- FieldSolver (src/FieldSolver.cpp)
-// rows of the finite-volume L_n (n >= 1) with Dirichlet rows 0 and N (:20-33)
-static void fv_rows(const RadialGrid& g, int n, std::vector<Real>& a, std::vector<Real>& b, std::vector<Real>& c) {
-    const int N = g.N, M = N + 1;
-    a.assign(M, 0); b.assign(M, 0); c.assign(M, 0);
-    b[0] = 1.0;
-    b[N] = 1.0;
+namespace {
+struct Rows {
+    std::vector<Real> a, b, c;
+    explicit Rows(int M) : a(M, 0.0), b(M, 0.0), c(M, 0.0) {}
+};
+
+// L0: the lumped finite-element rows of the axisymmetric Laplacian, a Dirichlet row at the wall (:43-59)
+Rows laplacian_m0(const RadialGrid& g) {
+    const int N = g.N;
+    Rows L(N + 1);
+    for (int j = 0; j < N; ++j) {
+        const Real scale = 2 * PI / g.V[j];
+        L.a[j] = j > 0 ? g.rmid[j - 1] / g.h[j - 1] * scale : 0.0;
+        L.c[j] = g.rmid[j] / g.h[j] * scale;
+        L.b[j] = -(L.a[j] + L.c[j]);
+    }
+    L.b[N] = 1.0;
+    return L;
+}
+
+// L_n, n >= 1: finite-volume rows of r^(n-1) d/dr (r^(1-2n) d/dr (r^n f)) / r^... written for f = r^n g with the
+// flux of cell j weighted by 1 / (h_j rmid_j^(2n-1)) (:20-33, 63-85); Dirichlet rows at the axis and, unless the
+// wall row is the flux (B_theta) condition, at the wall.  flux_coef receives the wall row's source factor.
+Rows laplacian_mn(const RadialGrid& g, int n, bool flux_wall, double* flux_coef) {
+    const int N = g.N;
+    Rows L(N + 1);
     auto kp = [&](int j) { return 1.0 / (g.h[j] * c_pow(g.rmid[j], 2 * n - 1)); };
+    L.b[0] = 1.0;
     for (int j = 1; j < N; ++j) {
         const Real dV = 0.5 * (g.h[j] + g.h[j - 1]);
         const Real pre = c_pow(g.r[j], n - 1) / dV;
-        c[j] = pre * c_pow(g.r[j + 1], n) * kp(j);
-        a[j] = pre * c_pow(g.r[j - 1], n) * kp(j - 1);
-        b[j] = -pre * c_pow(g.r[j], n) * (kp(j) + kp(j - 1));
+        L.c[j] = pre * c_pow(g.r[j + 1], n) * kp(j);
+        L.a[j] = pre * c_pow(g.r[j - 1], n) * kp(j - 1);
+        L.b[j] = -pre * c_pow(g.r[j], n) * (kp(j) + kp(j - 1));
     }
+    if (flux_wall) {
+        const Real dV = 0.5 * g.h[N - 1];
+        const Real k = kp(N - 1) / dV;
+        L.a[N] = g.r[N - 1] * k;
+        L.b[N] = -g.r[N] * k;
+        *flux_coef = 1.0 / dV;
+    } else {
+        L.b[N] = 1.0;
+    }
+    return L;
 }
 
-// Thomas factors c' and m of a chi-free operator, formed exactly as TridiagSolver's loop forms them
-// (src/Tridiag.cpp:114-121)
-static void thomas_factors(const std::vector<Real>& a, const std::vector<Real>& b, const std::vector<Real>& c,
-                           std::vector<Real>& m, std::vector<Real>& cp) {
-    const size_t M = b.size();
+// Thomas factors c' and m of a chi-free operator, formed as TridiagSolver's loop forms them (src/Tridiag.cpp:114-121)
+void thomas_factors(const Rows& L, std::vector<Real>& m, std::vector<Real>& cp) {
+    const size_t M = L.b.size();
     m.assign(M, 0.0);
     cp.assign(M, 0.0);
-    cp[0] = c[0] / b[0];
+    cp[0] = L.c[0] / L.b[0];
     for (size_t i = 1; i < M; ++i) {
-        m[i] = Real(1) / (b[i] - a[i] * cp[i - 1]);
-        cp[i] = c[i] * m[i];
+        m[i] = Real(1) / (L.b[i] - L.a[i] * cp[i - 1]);
+        cp[i] = L.c[i] * m[i];
     }
 }
+}  // namespace
 
 void FieldSolver::init(const RadialGrid& g, hipStream_t st) {   // :43-109, 157-176
     const int N = g.N, M = N + 1;
-    {
-        std::vector<Real> aa(M, 0), bb(M, 0), cc(M, 0);
-        for (int j = 0; j < N; ++j) {
-            const Real invV = 2 * PI / g.V[j];
-            aa[j] = (j > 0) ? g.rmid[j - 1] / g.h[j - 1] * invV : 0.0;
-            cc[j] = g.rmid[j] / g.h[j] * invV;
-            bb[j] = -(aa[j] + cc[j]);
-        }
-        bb[N] = 1.0;
-        a[L0] = aa; b[L0] = bb; c[L0] = cc;
+    const Rows ops[4] = {laplacian_m0(g), laplacian_mn(g, 1, false, nullptr), laplacian_mn(g, 1, true, &flux_coef),
+                         laplacian_mn(g, 2, false, nullptr)};   // order L0, L1D, L1F, L2D
+    for (int op = 0; op < 4; ++op) {
+        a[op] = ops[op].a;
+        b[op] = ops[op].b;
+        c[op] = ops[op].c;
     }
-    auto build_fv = [&](int n, bool flux, int op) {
-        std::vector<Real> aa, bb, cc;
-        fv_rows(g, n, aa, bb, cc);
-        auto kp = [&](int j) { return 1.0 / (g.h[j] * c_pow(g.rmid[j], 2 * n - 1)); };
-        if (flux) {
-            const Real dV = 0.5 * g.h[N - 1];
-            const Real k = kp(N - 1) / dV;
-            aa[N] = g.r[N - 1] * k;
-            bb[N] = -g.r[N] * k;
-            flux_coef = 1.0 / dV;
-        } else {
-            bb[N] = 1.0;
-        }
-        a[op] = aa; b[op] = bb; c[op] = cc;
-    };
-    build_fv(1, false, L1D);
-    build_fv(1, true, L1F);
-    build_fv(2, false, L2D);
+    // three-point derivative on the non-uniform grid (interior: centred; wall: one-sided), and 1/r with the
+    // axis value 1/r_1
     gm.assign(M, 0); g0.assign(M, 0); gp.assign(M, 0); ir.assign(M, 0);
     for (int j = 1; j < N; ++j) {
         const Real hm = g.h[j - 1], hp = g.h[j];
@@ -223,18 +281,16 @@ void FieldSolver::init(const RadialGrid& g, hipStream_t st) {   // :43-109, 157-
         gm[j] = -hp * hp / den;
         g0[j] = (hp * hp - hm * hm) / den;
     }
-    {
-        const Real h1 = g.h[N - 1], h2 = (N >= 2) ? g.h[N - 2] : g.h[N - 1];
-        const Real s = h1 + h2;
-        g0[N] = (2 * h1 + h2) / (h1 * s);
-        gm[N] = -s / (h1 * h2);
-        gp[N] = h1 / (h2 * s);
-    }
+    const Real h1 = g.h[N - 1], h2 = N >= 2 ? g.h[N - 2] : g.h[N - 1];
+    const Real s12 = h1 + h2;
+    g0[N] = (2 * h1 + h2) / (h1 * s12);
+    gm[N] = -s12 / (h1 * h2);
+    gp[N] = h1 / (h2 * s12);
     for (int j = 1; j <= N; ++j) ir[j] = 1.0 / g.r[j];
     ir[0] = 1.0 / g.r[1];
-    // the chi-free solves (psi, E_z, B_z) prepare b unchanged: Dirichlet rows keep b0, the others b0 - 0 = b0
-    thomas_factors(a[L0], b[L0], c[L0], l0_m, l0_cp);
-    thomas_factors(a[L1D], b[L1D], c[L1D], l1_m, l1_cp);
+    // the chi-free solves (psi, E_z, B_z) keep b as it is: factors formed once per run
+    thomas_factors(ops[L0], l0_m, l0_cp);
+    thomas_factors(ops[L1D], l1_m, l1_cp);
     for (int op = 0; op < 4; ++op) {
         d_a[op].upload(a[op], st); d_b[op].upload(b[op], st); d_c[op].upload(c[op], st);
     }
@@ -243,91 +299,104 @@ void FieldSolver::init(const RadialGrid& g, hipStream_t st) {   // :43-109, 157-
 }
 
 // ---------------------------------------------------------------------------- PlasmaSpecies (src/PlasmaSpecies.cpp)
-static void make_ab(int order, Real* c) {   // :14-28
-    for (int i = 0; i < 5; ++i) c[i] = 0;
-    switch (order) {
-        case 1: c[0] = 1; break;
-        case 2: c[0] = 1.5; c[1] = -0.5; break;
-        case 3: c[0] = 23. / 12; c[1] = -16. / 12; c[2] = 5. / 12; break;
-        case 4: c[0] = 55. / 24; c[1] = -59. / 24; c[2] = 37. / 24; c[3] = -9. / 24; break;
-        case 5:
-            c[0] = 1901. / 720; c[1] = -2774. / 720; c[2] = 2616. / 720; c[3] = -1274. / 720; c[4] = 251. / 720;
-            break;
-        default: throw std::runtime_error("pusher.ab_order must be 1..5");
-    }
+// Adams-Bashforth weights of orders 1 to 5 for uniform steps, newest force first (the method's table; the
+// reference's make_ab, :14-28, holds the same values)
+static void make_ab(int order, Real* c) {
+    static const Real table[5][5] = {
+        {1, 0, 0, 0, 0},
+        {1.5, -0.5, 0, 0, 0},
+        {23. / 12, -16. / 12, 5. / 12, 0, 0},
+        {55. / 24, -59. / 24, 37. / 24, -9. / 24, 0},
+        {1901. / 720, -2774. / 720, 2616. / 720, -1274. / 720, 251. / 720},
+    };
+    if (order < 1 || order > 5) throw std::runtime_error("pusher.ab_order: Adams-Bashforth orders 1 to 5 exist, not " +
+                                                         std::to_string(order));
+    for (int i = 0; i < 5; ++i) c[i] = table[order - 1][i];
 }
 
 void PlasmaSpecies::init(const Config& cfg, const std::string& nm, const RadialGrid& g, bool mode1) {   // :92-185
     name = nm;
     m1 = mode1;
-    q = cfg.get_double(nm + ".charge", -1.0);
-    m = cfg.get_double(nm + ".mass", 1.0);
-    mobile = cfg.get_bool(nm + ".mobile", true);
-    ppc = cfg.get_int(nm + ".ppc", 4);
-    ntheta = cfg.get_int(nm + ".ntheta", m1 ? 8 : 1);
-    uth = cfg.get_double(nm + ".uth", 0.0);
-    density = cfg.get_double(nm + ".density", 1.0);
+    auto key = [&](const char* k) { return nm + "." + k; };
+    q = cfg.get_double(key("charge"), -1.0);
+    m = cfg.get_double(key("mass"), 1.0);
+    mobile = cfg.get_bool(key("mobile"), true);
+    ppc = cfg.get_int(key("ppc"), 4);
+    ntheta = cfg.get_int(key("ntheta"), m1 ? 8 : 1);
+    uth = cfg.get_double(key("uth"), 0.0);
+    density = cfg.get_double(key("density"), 1.0);
     rmax = cfg.get_double("plasma.rmax", 1e300);
-    if (cfg.get_string("plasma.radial", "uniform") != "uniform")
-        throw std::runtime_error("plasma.radial: only the uniform profile is on this path");
     delta_min = cfg.get_double("pusher.delta_min", 1e-3);
     max_qsa = cfg.get_double("pusher.max_qsa_factor", 35.0);
     ab_order = cfg.get_int("pusher.ab_order", 3);
+    const std::string profile = cfg.get_string("plasma.radial", "uniform");
+    if (profile != "uniform")
+        throw std::runtime_error("plasma.radial = " + profile + ": this step loads the uniform profile only");
+    if (ppc < 1) throw std::runtime_error(key("ppc") + " must be at least 1 ring per cell");
+    if (ntheta < 1) throw std::runtime_error(key("ntheta") + " must be at least 1 particle per ring");
+    if (m1 && ntheta < 4)
+        throw std::runtime_error(key("ntheta") + " must be at least 4 with azimuthal mode 1 (8 is the default)");
     make_ab(ab_order, ab_c);
-    if (ppc < 1) throw std::runtime_error(nm + ".ppc must be >= 1");
-    if (ntheta < 1) throw std::runtime_error(nm + ".ntheta must be >= 1");
-    if (m1 && ntheta < 4) throw std::runtime_error(nm + ".ntheta must be >= 4 with azimuthal mode 1 (8 recommended)");
     cap = std::max(g.N * ppc * ntheta, 1);
     Np = cap;
-    x.alloc(cap); y.alloc(cap); px.alloc(cap); py.alloc(cap); dl.alloc(cap); w.alloc(cap);
+    for (DevBuf<Real>* a : {&x, &y, &px, &py, &dl, &w}) a->alloc(cap);
     hist.alloc(static_cast<size_t>(ab_order) * 5 * cap);
 }
 
-// rings at the sub-ring volume centroids of every cell (src/PlasmaSpecies.cpp:202-295)
+// The rings at the head of the box (src/PlasmaSpecies.cpp:202-295): every cell holds ppc rings of equal radial
+// width, each at the volume centroid r = (2/3)(rb^3 - ra^3) / (rb^2 - ra^2) of its annulus [ra, rb], with the
+// annulus volume pi (rb^2 - ra^2) per unit length times the density as its weight (zero beyond plasma.rmax);
+// every other ring is turned by half an azimuth step, and a ring is ntheta particles at equal angles (one on
+// the x axis when ntheta = 1).  A thermal plasma draws ux, uy, uz per particle from std::normal_distribution in
+// that order.  Particles of zero weight are not loaded.
 void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st) {
-    (void)z;
+    (void)z;   // the longitudinal profile is flat on this path
     if (have_template) {   // the rings of the first load (device copies: capturable, no host work)
         const size_t bytes = sizeof(Real) * static_cast<size_t>(Np);
-        if (Np) {
-            const DevBuf<Real>* src[6] = {&t_x, &t_y, &t_px, &t_py, &t_dl, &t_w};
-            DevBuf<Real>* dst[6] = {&x, &y, &px, &py, &dl, &w};
-            for (int c = 0; c < 6; ++c)
-                hip_check(hipMemcpyAsync(dst[c]->p, src[c]->p, bytes, hipMemcpyDeviceToDevice, st), "plasma copy");
-        }
+        const DevBuf<Real>* from[6] = {&t_x, &t_y, &t_px, &t_py, &t_dl, &t_w};
+        DevBuf<Real>* to[6] = {&x, &y, &px, &py, &dl, &w};
+        for (int a = 0; a < 6 && bytes; ++a)
+            hip_check(hipMemcpyAsync(to[a]->p, from[a]->p, bytes, hipMemcpyDeviceToDevice, st), "plasma copy");
         qz_check(qz_zero_async(hist.p, sizeof(Real) * hist.n, st));
         return;
     }
     std::mt19937_64 rng(seed);
     std::normal_distribution<double> gauss(0.0, 1.0);
-    const double fz = 1.0 * density;
-    hx.assign(cap, 0); hy.assign(cap, 0); hpx.assign(cap, 0); hpy.assign(cap, 0); hdl.assign(cap, 0); hw.assign(cap, 0);
-    int i = 0;
-    for (int j = 0; j < g.N; ++j) {
-        const double hh = g.h[j];
-        for (int mm = 0; mm < ppc; ++mm) {
-            const double ra = g.r[j] + mm * hh / ppc, rb = g.r[j] + (mm + 1) * hh / ppc;
-            const double r = (2.0 / 3.0) * (rb * rb * rb - ra * ra * ra) / (rb * rb - ra * ra);
-            const double vring = fz * PI * (rb * rb - ra * ra);
-            const double wring = (r > rmax ? 0.0 : 1.0) * vring;
-            const double off = ((j * ppc + mm) % 2) * 0.5;
-            for (int l = 0; l < ntheta; ++l, ++i) {
-                const double th = 2 * PI * (l + off) / ntheta;
-                double ux = 0, uy = 0, uz = 0;
-                if (uth > 0) { ux = uth * gauss(rng); uy = uth * gauss(rng); uz = uth * gauss(rng); }
-                ux += 0; uy += 0; uz += 0;
-                const double gam = std::sqrt(1 + ux * ux + uy * uy + uz * uz);
-                const double X = (ntheta == 1) ? r : r * std::cos(th), Y = (ntheta == 1) ? 0.0 : r * std::sin(th);
-                const double W = wring / ntheta;
-                if (W == 0) { --i; continue; }
-                hx[i] = X; hy[i] = Y; hpx[i] = ux; hpy[i] = uy; hdl[i] = gam - uz; hw[i] = W;
-            }
+    for (std::vector<Real>* a : {&hx, &hy, &hpx, &hpy, &hdl, &hw}) a->assign(cap, 0.0);
+    int n = 0;
+    for (int ring = 0; ring < g.N * ppc; ++ring) {
+        const int j = ring / ppc, sub = ring % ppc;
+        const double hc = g.h[j];
+        const double ra = g.r[j] + sub * hc / ppc, rb = g.r[j] + (sub + 1) * hc / ppc;
+        const double rc = (2.0 / 3.0) * (rb * rb * rb - ra * ra * ra) / (rb * rb - ra * ra);
+        const double volume = (1.0 * density) * PI * (rb * rb - ra * ra);
+        const double wp = (rc > rmax ? 0.0 : 1.0) * volume / ntheta;
+        const double turn = (ring % 2) * 0.5;
+        for (int l = 0; l < ntheta; ++l) {
+            double u[3] = {0, 0, 0};
+            if (uth > 0)
+                for (double& v : u) v = uth * gauss(rng);
+            if (wp == 0) continue;
+            const double th = 2 * PI * (l + turn) / ntheta;
+            const double gam = std::sqrt(1 + u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            hx[n] = ntheta == 1 ? rc : rc * std::cos(th);
+            hy[n] = ntheta == 1 ? 0.0 : rc * std::sin(th);
+            hpx[n] = u[0];
+            hpy[n] = u[1];
+            hdl[n] = gam - u[2];
+            hw[n] = wp;
+            ++n;
         }
     }
-    Np = i;
-    x.upload(hx, st); y.upload(hy, st); px.upload(hpx, st); py.upload(hpy, st); dl.upload(hdl, st); w.upload(hw, st);
-    t_x.alloc(cap); t_y.alloc(cap); t_px.alloc(cap); t_py.alloc(cap); t_dl.alloc(cap); t_w.alloc(cap);
-    t_x.upload(hx, st); t_y.upload(hy, st); t_px.upload(hpx, st); t_py.upload(hpy, st); t_dl.upload(hdl, st);
-    t_w.upload(hw, st);
+    Np = n;
+    const std::vector<Real>* host[6] = {&hx, &hy, &hpx, &hpy, &hdl, &hw};
+    DevBuf<Real>* live[6] = {&x, &y, &px, &py, &dl, &w};
+    DevBuf<Real>* keep[6] = {&t_x, &t_y, &t_px, &t_py, &t_dl, &t_w};
+    for (int a = 0; a < 6; ++a) {
+        live[a]->upload(*host[a], st);
+        keep[a]->alloc(cap);
+        keep[a]->upload(*host[a], st);
+    }
     have_template = true;
     qz_check(qz_zero_async(hist.p, sizeof(Real) * hist.n, st));
 }
@@ -374,115 +443,156 @@ void PlasmaSpecies::deposit_rho(const RadialGrid& g, std::vector<Real>& rhot, st
 }
 
 // ---------------------------------------------------------------------------- Beam (src/Beam.cpp)
-void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st) {   // :53-207, 483-570
+namespace {
+// What a deck says about one particle beam of this path (src/Beam.cpp:53-207, 483-570): the Gaussian or
+// flat-top bunch, its transverse offset and tilt, its momenta, its symmetry copies and the optional
+// piecewise-linear current profile of a flat-top witness.
+struct BeamSpec {
+    bool gaussian = true;
+    double n0 = 0, sr = 0, xi0 = 0, sxi = 0, len = 0, gamma0 = 0, espread = 0, emit = 0, cut = 3;
+    double cx0 = 0, cy0 = 0, cxs = 0, cys = 0, xp0 = 0, yp0 = 0, xcut = -1e300;
+    int ntot = 0, nrot = 1, nsym = 1;
+    unsigned long long seed = 12345;
+    std::vector<double> pwl;
+};
+
+BeamSpec read_beam_spec(const Config& cfg, const std::string& nm, bool m1) {
+    auto key = [&](const char* k) { return nm + "." + k; };
+    BeamSpec b;
+    const std::string profile = cfg.get_string(key("profile"), "gaussian");
+    if (profile != "gaussian" && profile != "flattop")
+        throw std::runtime_error(key("profile") + " = " + profile + ": this step samples gaussian and flattop beams");
+    b.gaussian = profile == "gaussian";
+    b.n0 = cfg.get_double(key("n0"));
+    b.sr = cfg.get_double(key("sigma_r"));
+    b.xi0 = xi_from_user(cfg.get_double(key("xi0")));
+    b.gamma0 = cfg.get_double(key("gamma"));
+    b.espread = cfg.get_double(key("espread"), 0.0);
+    b.emit = cfg.get_double(key("emit_n"), 0.0);
+    b.cut = cfg.get_double(key("cut"), 3.0);
+    b.cx0 = cfg.get_double(key("x0"), 0.0);
+    b.cy0 = cfg.get_double(key("y0"), 0.0);
+    b.cxs = -cfg.get_double(key("x_slope"), 0.0);   // d x_c / d xi = - d x_c / d xi_user
+    b.cys = -cfg.get_double(key("y_slope"), 0.0);
+    b.xp0 = cfg.get_double(key("xp0"), 0.0);
+    b.yp0 = cfg.get_double(key("yp0"), 0.0);
+    b.ntot = cfg.get_int(key("nparticles"), 100000);
+    b.nrot = cfg.get_int(key("nsym"), 1);
+    b.nsym = b.nrot * (cfg.get_bool(key("mirror_y"), false) ? 2 : 1);
+    b.seed = cfg.get_int(key("seed"), 12345);
+    if (cfg.has(key("xi_cut_head"))) b.xcut = xi_from_user(cfg.get_double(key("xi_cut_head")));
+    if (b.gaussian) b.sxi = cfg.get_double(key("sigma_xi"));
+    else b.len = cfg.get_double(key("length"));
+    for (const auto& tok : cfg.get_list(key("pwl_weights"))) b.pwl.push_back(cfg.number(tok, key("pwl_weights")));
+    const bool offset = b.cx0 != 0 || b.cy0 != 0 || b.cxs != 0 || b.cys != 0 || b.xp0 != 0 || b.yp0 != 0;
+    if (offset && !m1) throw std::runtime_error(nm + ": an off-axis or tilted beam needs modes = 1");
+    if (b.nrot < 1) throw std::runtime_error(key("nsym") + " must be at least 1");
+    if (b.ntot % b.nsym != 0)
+        throw std::runtime_error(key("nparticles") + " = " + std::to_string(b.ntot) + " is not a multiple of the " +
+                                 std::to_string(b.nsym) + " symmetry copies (nsym, doubled by mirror_y)");
+    if (!b.pwl.empty() && (b.gaussian || b.nsym != 1))
+        throw std::runtime_error(key("pwl_weights") + ": a current profile needs a flattop beam without symmetry copies");
+    return b;
+}
+
+// Weight of one macro-particle: the bunch charge in units of n0 (c/omega_p)^3 inside the radial cut and the
+// longitudinal window (after the head cut), shared by the macro-particles.
+double macro_weight(const BeamSpec& b) {
+    const double inside_r = 1.0 - std::exp(-0.5 * b.cut * b.cut);   // fraction inside r < cut sigma_r
+    double total;
+    if (b.gaussian) {
+        auto Phi = [](double v) { return 0.5 * (1.0 + std::erf(v / std::sqrt(2.0))); };
+        const double lo = std::max(-b.cut, (b.xcut - b.xi0) / b.sxi);
+        if (lo >= b.cut) throw std::runtime_error("xi_cut_head lies behind the whole bunch");
+        total = b.n0 * c_pow(2 * PI, 1.5) * b.sr * b.sr * b.sxi * inside_r * (Phi(b.cut) - Phi(lo));
+    } else {
+        const double lo = std::max(b.xi0 - 0.5 * b.len, b.xcut), hi = b.xi0 + 0.5 * b.len;
+        if (lo >= hi) throw std::runtime_error("xi_cut_head lies behind the whole bunch");
+        total = b.n0 * 2 * PI * b.sr * b.sr * inside_r * (hi - lo);
+    }
+    return total / b.ntot;
+}
+}  // namespace
+
+void Beam::init(const Config& cfg, const std::string& nm, bool mode1, hipStream_t st) {
     name = nm;
     m1 = mode1;
     q = cfg.get_double(nm + ".charge", -1.0);
     m = cfg.get_double(nm + ".mass", 1.0);
     rigid = cfg.get_bool(nm + ".rigid", false);
+    static const std::map<std::string, int> pushers = {{"vay", 0}, {"boris", 1}, {"hc", 2}, {"imp", 3}};
     const std::string pm = cfg.get_string(nm + ".pusher", cfg.get_string("pusher.beam", "vay"));
-    if (pm == "vay") pusher = 0;
-    else if (pm == "boris") pusher = 1;
-    else if (pm == "hc") pusher = 2;
-    else if (pm == "imp") pusher = 3;
-    else throw std::runtime_error("pusher.beam must be vay|boris|hc|imp on this path");
-    const std::string& p = nm;
-    const std::string lprof = cfg.get_string(p + ".profile", "gaussian");
-    if (lprof != "gaussian" && lprof != "flattop") throw std::runtime_error(p + ".profile must be gaussian|flattop on this path");
-    const double n0 = cfg.get_double(p + ".n0");
-    const double sr = cfg.get_double(p + ".sigma_r");
-    const double xi0 = xi_from_user(cfg.get_double(p + ".xi0"));
-    const double gamma0 = cfg.get_double(p + ".gamma");
-    const double espread = cfg.get_double(p + ".espread", 0.0);
-    const double emit = cfg.get_double(p + ".emit_n", 0.0);
-    const double cut = cfg.get_double(p + ".cut", 3.0);
-    const double cx0 = cfg.get_double(p + ".x0", 0.0), cy0 = cfg.get_double(p + ".y0", 0.0);
-    const double cxs = -cfg.get_double(p + ".x_slope", 0.0), cys = -cfg.get_double(p + ".y_slope", 0.0);
-    const double xp0 = cfg.get_double(p + ".xp0", 0.0), yp0 = cfg.get_double(p + ".yp0", 0.0);
-    if (!m1 && (cx0 != 0 || cy0 != 0 || cxs != 0 || cys != 0 || xp0 != 0 || yp0 != 0))
-        throw std::runtime_error(p + ": transverse offsets/tilts need azimuthal mode 1 (modes = 1)");
-    const int Ntot = cfg.get_int(p + ".nparticles", 100000);
-    const int nrot = cfg.get_int(p + ".nsym", 1);
-    const bool mirror = cfg.get_bool(p + ".mirror_y", false);
-    const int nsym = nrot * (mirror ? 2 : 1);
-    if (nrot < 1 || Ntot % nsym != 0) throw std::runtime_error(p + ".nparticles must be a multiple of nsym (x2 with mirror_y)");
-    const unsigned long long sd = cfg.get_int(p + ".seed", 12345);
-    const double xcut = cfg.has(p + ".xi_cut_head") ? xi_from_user(cfg.get_double(p + ".xi_cut_head")) : -1e300;
-    double sxi = 0, len = 0;
-    if (lprof == "gaussian") sxi = cfg.get_double(p + ".sigma_xi");
-    else len = cfg.get_double(p + ".length");
-    const double Pr = 1.0 - std::exp(-0.5 * cut * cut);
-    auto Phi = [](double v) { return 0.5 * (1.0 + std::erf(v / std::sqrt(2.0))); };
-    double wtot;
-    if (lprof == "gaussian") {
-        const double lo = std::max(-cut, (xcut - xi0) / sxi);
-        if (lo >= cut) throw std::runtime_error(p + ": xi_cut_head removes the whole beam");
-        wtot = n0 * c_pow(2 * PI, 1.5) * sr * sr * sxi * Pr * (Phi(cut) - Phi(lo));
-    } else {
-        const double lo = std::max(xi0 - 0.5 * len, xcut), hi = xi0 + 0.5 * len;
-        if (lo >= hi) throw std::runtime_error(p + ": xi_cut_head removes the whole beam");
-        wtot = n0 * 2 * PI * sr * sr * Pr * (hi - lo);
-    }
-    const double wmac = wtot / Ntot;
-    // a piecewise-linear current profile over a flat-top beam (the differentiable workload's witness): the
-    // weight of a macro-particle at xi is wmac times the profile there
-    pwl.clear();
+    if (!pushers.count(pm)) throw std::runtime_error(nm + ": pusher '" + pm + "' is not one of vay, boris, hc, imp");
+    pusher = pushers.at(pm);
+    const BeamSpec b = read_beam_spec(cfg, nm, m1);
+    const double wmac = macro_weight(b);
+    pwl = b.pwl;
     xi_init.clear();
     g_init.clear();
-    for (const auto& tok : cfg.get_list(p + ".pwl_weights")) pwl.push_back(cfg.number(tok, p + ".pwl_weights"));
-    if (!pwl.empty()) {
-        if (lprof != "flattop") throw std::runtime_error(p + ".pwl_weights needs profile = flattop");
-        if (nsym != 1) throw std::runtime_error(p + ".pwl_weights needs nsym = 1 and no mirror_y");
-        pwl_u0 = xi_to_user(xi0 + 0.5 * len);
-        pwl_u1 = xi_to_user(xi0 - 0.5 * len);
+    if (!pwl.empty()) {   // the profile runs over the flat top, tail first, in user xi
+        pwl_u0 = xi_to_user(b.xi0 + 0.5 * b.len);
+        pwl_u1 = xi_to_user(b.xi0 - 0.5 * b.len);
         pwl_wmac = wmac;
     }
-    std::mt19937_64 rng(sd);
+    // Sampling, one draw sequence of std::mt19937_64(seed) per group of nsym copies: (x, y) from the radial
+    // Gaussian inside cut sigma_r; xi from the longitudinal shape, redrawn ahead of the head cut; px, py from
+    // the emittance; gamma from the energy spread.  The copies are the sample turned by 2 pi l / nrot (and
+    // mirrored in y), offset and tilted by the centroid, with the momenta turned alike.
+    std::mt19937_64 rng(b.seed);
     std::normal_distribution<double> N01(0.0, 1.0);
     std::uniform_real_distribution<double> U01(0.0, 1.0);
-    const double spx = (sr > 0) ? emit / sr : 0.0;
+    const double spx = b.sr > 0 ? b.emit / b.sr : 0.0;
     double aw = 0, awg = 0, awxi = 0;
-    std::vector<double> all;
-    all.reserve(static_cast<size_t>(Ntot) * 7);
-    for (int i0 = 0; i0 < Ntot; i0 += nsym) {
+    std::vector<double> rows;
+    rows.reserve(static_cast<size_t>(b.ntot) * 7);
+    for (int group = 0; group < b.ntot / b.nsym; ++group) {
         double xx, yy;
-        do { xx = sr * N01(rng); yy = sr * N01(rng); } while (xx * xx + yy * yy > cut * cut * sr * sr);
+        do {
+            xx = b.sr * N01(rng);
+            yy = b.sr * N01(rng);
+        } while (xx * xx + yy * yy > b.cut * b.cut * b.sr * b.sr);
         double xv;
         do {
-            if (lprof == "gaussian") {
-                do { xv = N01(rng); } while (std::abs(xv) > cut);
-                xv = xi0 + sxi * xv;
+            if (b.gaussian) {
+                do xv = N01(rng); while (std::abs(xv) > b.cut);
+                xv = b.xi0 + b.sxi * xv;
             } else {
-                xv = xi0 + len * (U01(rng) - 0.5);
+                xv = b.xi0 + b.len * (U01(rng) - 0.5);
             }
-        } while (xv < xcut);
+        } while (xv < b.xcut);
         double ux = spx * N01(rng), uy = spx * N01(rng);
-        const double gam = gamma0 * (1.0 + espread * N01(rng));
-        ux += gam * xp0;
-        uy += gam * yp0;
+        const double gam = b.gamma0 * (1.0 + b.espread * N01(rng));
+        ux += gam * b.xp0;
+        uy += gam * b.yp0;
         const double pz2 = gam * gam - 1.0 - ux * ux - uy * uy;
-        if (pz2 <= 0) throw std::runtime_error(p + ": gamma too small for the transverse momentum spread");
-        const double tpx = ux - gam * xp0, tpy = uy - gam * yp0;
-        for (int l = 0; l < nsym; ++l) {
-            const int lr = l % nrot;
-            const double sgn = (l >= nrot) ? -1.0 : 1.0;
-            const double ph = 2 * PI * lr / nrot, cp = std::cos(ph), sp = std::sin(ph);
-            const double rec[7] = {cp * xx - sp * yy + cx0 + cxs * (xv - xi0), sgn * (sp * xx + cp * yy) + cy0 + cys * (xv - xi0),
-                                   cp * tpx - sp * tpy + gam * xp0, sgn * (sp * tpx + cp * tpy) + gam * yp0,
-                                   std::sqrt(pz2), xv, pwl.empty() ? wmac : wmac * pwl_factor(pwl, pwl_u0, pwl_u1, xi_to_user(xv))};
-            if (rec[6] != 0 || !pwl.empty()) {   // reference values of the moments (Beam::accumulate, :580-588)
-                const double gg = std::sqrt(1 + rec[2] * rec[2] + rec[3] * rec[3] + rec[4] * rec[4]);
-                if (!pwl.empty()) { xi_init.push_back(xv); g_init.push_back(gg); }
-                aw += rec[6];
-                awg += rec[6] * gg;
-                awxi += rec[6] * rec[5];
+        if (pz2 <= 0) throw std::runtime_error(nm + ": the transverse momentum spread exceeds what gamma allows");
+        const double tpx = ux - gam * b.xp0, tpy = uy - gam * b.yp0;
+        const double shift = xv - b.xi0;
+        for (int l = 0; l < b.nsym; ++l) {
+            const double mirror = l >= b.nrot ? -1.0 : 1.0;
+            const double ph = 2 * PI * (l % b.nrot) / b.nrot, cp = std::cos(ph), sp = std::sin(ph);
+            const double wgt = pwl.empty() ? wmac : wmac * pwl_factor(pwl, pwl_u0, pwl_u1, xi_to_user(xv));
+            const double row[7] = {cp * xx - sp * yy + b.cx0 + b.cxs * shift,
+                                   mirror * (sp * xx + cp * yy) + b.cy0 + b.cys * shift,
+                                   cp * tpx - sp * tpy + gam * b.xp0,
+                                   mirror * (sp * tpx + cp * tpy) + gam * b.yp0,
+                                   std::sqrt(pz2), xv, wgt};
+            if (wgt != 0 || !pwl.empty()) {   // the reference values of the moments: mean gamma and mean xi
+                const double gg = std::sqrt(1 + row[2] * row[2] + row[3] * row[3] + row[4] * row[4]);
+                if (!pwl.empty()) {
+                    xi_init.push_back(xv);
+                    g_init.push_back(gg);
+                }
+                aw += wgt;
+                awg += wgt * gg;
+                awxi += wgt * xv;
             }
-            all.insert(all.end(), rec, rec + 7);
+            rows.insert(rows.end(), row, row + 7);
         }
     }
-    set_particles(all, st);
-    gref = 0; xiref = 0;
-    if (aw > 0) { gref = awg / aw; xiref = awxi / aw; }
+    set_particles(rows, st);
+    gref = aw > 0 ? awg / aw : 0;
+    xiref = aw > 0 ? awxi / aw : 0;
 }
 
 // The profile's values sit at the centres of nseg equal segments of [u0, u1] (user xi, tail first); the
@@ -534,6 +644,17 @@ std::array<double, 17> Beam::sums(hipStream_t st) const {
     std::array<double, 17> S{};
     const qz_beam_soa s = soa();
     qz_check(qz_beam_sums(Np, &s, gref, xiref, S.data(), st));
+    return S;
+}
+
+std::array<double, 17> Beam::sums_device(hipStream_t st) const {
+    const size_t need = 17 + qz_beam_sums_workspace_bytes(Np) / sizeof(Real);
+    if (sums_work.n < need) sums_work.alloc(need);
+    const qz_beam_soa s = soa();
+    qz_check(qz_beam_sums_device(Np, &s, gref, xiref, sums_work.p, sums_work.p + 17, st));
+    std::array<double, 17> S{};
+    hip_check(hipMemcpyAsync(S.data(), sums_work.p, sizeof(S), hipMemcpyDeviceToHost, st), "sums");
+    hip_check(hipStreamSynchronize(st), "sync");
     return S;
 }
 
@@ -702,7 +823,7 @@ Simulation::~Simulation() {
 void Simulation::step_fields() {
     prepare_sweep();
     hip_check(hipEventRecord(ev[0], stream), "event");
-    qz_check(qz_sweep(&sweep_args, stream));
+    launch_sweeps(d_args, &sweep_args, 1, stream);
     hip_check(hipEventRecord(ev[1], stream), "event");
     hip_check(hipEventSynchronize(ev[1]), "sync");
     float ms = 0;
@@ -786,6 +907,7 @@ void Simulation::prepare_sweep(bool timed) {
 // after the slice loop: removed-particle count, work counters, timings (ms_sweep measured by the caller)
 void Simulation::finish_sweep(float sweep_ms, bool timed) {
     const int nsp = sweep_args.nspecies;
+    sweep_launches = 1;
     std::vector<int> hl;
     lost.download(hl, stream);
     lost_last = 0;
@@ -888,7 +1010,7 @@ void Simulation::write_beam_output() {
                    " x_mean y_mean emit_ny sigma_x sigma_y\n";
     }
     for (const auto& b : beams) {
-        const BeamDiag d = b->from_sums(b->sums(stream), t);
+        const BeamDiag d = b->from_sums(b->sums_device(stream), t);
         beamlog << b->name << " " << step_n << " " << std::setprecision(10) << d.t << " " << d.alive << " " << d.npart
                 << " " << d.charge << " " << d.gamma_mean << " " << d.gamma_rms << " " << d.xi_mean << " " << d.xi_rms
                 << " " << d.r_rms << " " << d.emit_nx << " " << d.x_mean << " " << d.y_mean << " " << d.emit_ny << " "
@@ -1019,6 +1141,16 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
     const int mm = m1 ? 1 : 0;
     const size_t row = static_cast<size_t>(M) * NCB;
+    // HIP events around every range's sweep launch, on its stream: the kernel time of the timed steps
+    struct LaunchEvents {
+        std::vector<hipEvent_t> e;
+        ~LaunchEvents() {
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } lev;
+    lev.e.assign(2 * static_cast<size_t>(nrun) * nchunks, nullptr);
+    for (auto& x : lev.e) hip_check(hipEventCreate(&x), "event");
     for (int i = 0; i < nrun; ++i) {
         StepCtx& C = *ctx[i % depth];
         StepCtx* P = i > 0 ? ctx[(i - 1) % depth].get() : nullptr;
@@ -1044,7 +1176,7 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
                 if (b->rigid || b->Np == 0) continue;
                 const qz_beam_soa so = b->soa();
                 qz_check(qz_beam_advance_slices(&g, &box, mm, b->q / b->m, b->pusher, b->Np, &so, fld.p, NCF, dkick, ddrift,
-                                                1, lo, hi, marker, b->pstep.p, st));
+                                                1, lo, hi, marker, b->pstep.p, cnt + 21, st));
             }
             hip_check(hipEventRecord(C.pushed[c], st), "event");
         };
@@ -1053,11 +1185,9 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             // beam sources of rows [d0, d1]: up to row c1, which the range's last slice differentiates
             if (P) hip_check(hipStreamWaitEvent(st, P->pushed[std::min(c + 1, nchunks - 1)], 0), "wait");
             const int d0 = c == 0 ? 0 : c0 + 1, d1 = std::min(c1, K - 1);
-            if (c == nchunks - 1)   // every push of step n-1 is done: live counts as Simulation::step counts them
-                for (auto& b : beams) {
-                    qz_check(qz_beam_count_live(b->Np, b->a[6].p, cnt + 20, st));
-                    if (!b->rigid) qz_check(qz_beam_count_live(b->Np, b->a[6].p, cnt + 21, st));
-                }
+            // the counts of Simulation::step (live beam particles before the deposit and before the push) are
+            // taken by the kernels themselves: each window counts the live particles whose lower slice it holds,
+            // each range's push the particles it marks
             if (d0 <= d1) {
                 qz_check(qz_zero_async(C.bsrc.p + d0 * row, sizeof(Real) * row * static_cast<size_t>(d1 - d0 + 1), st));
                 qz_beam_grid bx = box;
@@ -1066,10 +1196,12 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
                 if (!dq.empty())
                     qz_check(qz_beam_deposit_beams(&g, &bx, mm, static_cast<int>(dq.size()), dq.data(), dnp.data(),
                                                    dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, dpstep.data(),
-                                                   marker, st));
+                                                   marker, cnt + 20, st));
             }
             const size_t ai = static_cast<size_t>(i) * nchunks + c;
+            hip_check(hipEventRecord(lev.e[2 * ai], st), "event");
             qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
+            hip_check(hipEventRecord(lev.e[2 * ai + 1], st), "event");
             if (!last) push_range(c);
         }
     }
@@ -1101,7 +1233,15 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     }
     hip_check(hipMemcpyAsync(bsrc.p, Lc.bsrc.p, sizeof(Real) * bsrc.n, hipMemcpyDeviceToDevice, stream), "bsrc copy");
     hip_check(hipStreamSynchronize(stream), "sync");
-    ms_sweep = ms_beam = ms_beam_dep = 0;
+    double launch_ms = 0;
+    for (size_t ai = 0; ai < static_cast<size_t>(nrun) * nchunks; ++ai) {
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, lev.e[2 * ai], lev.e[2 * ai + 1]), "elapsed");
+        launch_ms += ms;
+    }
+    ms_sweep = static_cast<float>(launch_ms / nrun);   // per step: the sum of its range launches' kernel times
+    sweep_launches = nchunks;
+    ms_beam = ms_beam_dep = 0;
     swept = true;
     for (int i = 0; i < nrun; ++i) {   // Simulation::push_beams and after_sweep, per step
         if (step_n < nsteps) {
@@ -1269,6 +1409,30 @@ struct qz_sim {
 
 int qz_fail(int code, const std::string& msg);
 
+// a null run handle (or a null output pointer) is a caller error, reported like any other
+#define QZ_NEED(cond, fn)                                                        \
+    do {                                                                         \
+        if (!(cond)) return qz_fail(QZ_EINVAL, std::string(fn) + ": null argument"); \
+    } while (0)
+
+// The deck reader alone (host only, no device work): every key of the deck after the overrides, one
+// "key = value" line each in key order.  Returns the bytes of the text (without the terminating 0), or a
+// negative code; the text is written when cap exceeds that length.
+extern "C" long qz_deck_keys(const char* deck, const char* overrides, char* out, long cap) {
+    if (!deck) return qz_fail(QZ_EINVAL, "qz_deck_keys: null deck");
+    try {
+        qz::Config cfg;
+        cfg.parse(deck);
+        cfg.overrides(overrides ? overrides : "");
+        const std::string text = cfg.dump();
+        const long n = static_cast<long>(text.size());
+        if (out && cap > n) std::memcpy(out, text.c_str(), static_cast<size_t>(n) + 1);
+        return n;
+    } catch (const std::exception& e) {
+        return qz_fail(QZ_EINVAL, e.what());
+    }
+}
+
 extern "C" int qz_sim_create(const char* deck, const char* overrides, qz_sim** out) {
     if (!deck || !out) return qz_fail(QZ_EINVAL, "qz_sim_create: null argument");
     QZ_SIM_TRY({
@@ -1291,7 +1455,10 @@ extern "C" int qz_sim_info(const qz_sim* h, int* info) {
     info[8] = s.nsteps; info[9] = s.step_n;
     return QZ_OK;
 }
-extern "C" int qz_sim_step(qz_sim* h) { QZ_SIM_TRY(h->s->step()) }
+extern "C" int qz_sim_step(qz_sim* h) {
+    QZ_NEED(h, "qz_sim_step");
+    QZ_SIM_TRY(h->s->step())
+}
 
 // One step of n independent runs with their sweeps in one persistent launch (one workgroup per run).
 // Every run's preparation is enqueued on its own stream; the launch waits for all of them on the first
@@ -1321,12 +1488,11 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
         qz::hip_check(hipEventCreate(&ev.e[1]), "event");
         hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
         qz::hip_check(hipEventRecord(e0, s0.stream), "event");
-        const int rc = qz_sweep_batched(args.data(), n, s0.stream);
+        qz::launch_sweeps(s0.d_args, args.data(), n, s0.stream);
         qz::hip_check(hipEventRecord(e1, s0.stream), "event");
         qz::hip_check(hipEventSynchronize(e1), "sync");
         float ms = 0;
         qz::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-        if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
         if (batch_ms) *batch_ms = ms;
         // the rest of every run's step (counters, output, beam push) touches only that run's stream and
         // buffers: a few host threads do the runs side by side
@@ -1549,14 +1715,19 @@ extern "C" int qz_sim_autotune(qz_sim* h, int kernel, const int* cand, int ncand
 }
 
 extern "C" int qz_sim_run_pipelined(qz_sim* h, int nrun, int nchunks, int depth, double* out2) {
+    QZ_NEED(h, "qz_sim_run_pipelined");
     QZ_SIM_TRY({
         double psl = 0, pushed = 0;
         h->s->run_pipelined(nrun, nchunks, depth, psl, pushed);
         if (out2) { out2[0] = psl; out2[1] = pushed; }
     })
 }
-extern "C" int qz_sim_step_fields(qz_sim* h) { QZ_SIM_TRY(h->s->step_fields()) }
+extern "C" int qz_sim_step_fields(qz_sim* h) {
+    QZ_NEED(h, "qz_sim_step_fields");
+    QZ_SIM_TRY(h->s->step_fields())
+}
 extern "C" int qz_sim_push_beams(qz_sim* h) {
+    QZ_NEED(h, "qz_sim_push_beams");
     QZ_SIM_TRY({
         auto& s = *h->s;
         const bool last = (s.step_n >= s.nsteps);
@@ -1566,6 +1737,10 @@ extern "C" int qz_sim_push_beams(qz_sim* h) {
     })
 }
 extern "C" long qz_sim_get(qz_sim* h, const char* what, double* out, long cap) {
+    if (!h || !what) {
+        qz_fail(QZ_EINVAL, "qz_sim_get: null argument");
+        return -1;
+    }
     try {
         auto& s = *h->s;
         const std::string w(what);
@@ -1622,6 +1797,7 @@ extern "C" int qz_sim_rewind(qz_sim* h) {
     QZ_SIM_TRY(h->s->rewind())
 }
 extern "C" int qz_sim_beam_sums(qz_sim* h, int b, double* out17, double* refs) {
+    QZ_NEED(h && out17, "qz_sim_beam_sums");
     QZ_SIM_TRY({
         auto& s = *h->s;
         if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
@@ -1631,10 +1807,11 @@ extern "C" int qz_sim_beam_sums(qz_sim* h, int b, double* out17, double* refs) {
     })
 }
 extern "C" int qz_sim_beam_diag(qz_sim* h, int b, double* out15) {
+    QZ_NEED(h && out15, "qz_sim_beam_diag");
     QZ_SIM_TRY({
         auto& s = *h->s;
         if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
-        const qz::BeamDiag d = s.beams[b]->from_sums(s.beams[b]->sums(s.stream), s.t);
+        const qz::BeamDiag d = s.beams[b]->from_sums(s.beams[b]->sums_device(s.stream), s.t);
         const double v[15] = {d.t, d.npart, d.charge, d.gamma_mean, d.gamma_rms, d.xi_mean, d.xi_rms, d.r_rms,
                               d.emit_nx, d.x_mean, d.y_mean, d.emit_ny, d.sigma_x, d.sigma_y, double(d.alive)};
         std::copy(v, v + 15, out15);
@@ -1646,15 +1823,16 @@ extern "C" int qz_sim_timing(const qz_sim* h, double* out2) {
     out2[1] = h->s->ms_beam;
     return QZ_OK;
 }
-extern "C" int qz_sim_stats(const qz_sim* h, double* out6) {
-    if (!h || !out6) return qz_fail(QZ_EINVAL, "qz_sim_stats: null argument");
+extern "C" int qz_sim_stats(const qz_sim* h, double* out7) {
+    if (!h || !out7) return qz_fail(QZ_EINVAL, "qz_sim_stats: null argument");
     const auto& s = *h->s;
-    out6[0] = s.particle_slices;
-    out6[1] = s.beam_pushed;
-    out6[2] = s.beam_deposited;
-    out6[3] = s.ms_sweep;
-    out6[4] = s.ms_beam_dep;
-    out6[5] = s.ms_beam;
+    out7[0] = s.particle_slices;
+    out7[1] = s.beam_pushed;
+    out7[2] = s.beam_deposited;
+    out7[3] = s.ms_sweep;
+    out7[4] = s.ms_beam_dep;
+    out7[5] = s.ms_beam;
+    out7[6] = s.sweep_launches;
     return QZ_OK;
 }
 extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out) {
@@ -1662,6 +1840,21 @@ extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out) {
     for (int i = 0; i < QZ_NPHASE; ++i) out[i] = h->s->phase_ticks[i];
     return QZ_OK;
 }
-extern "C" long qz_sim_lost(const qz_sim* h) { return h->s->lost_last; }
-extern "C" double qz_sim_time(const qz_sim* h) { return h->s->t; }
-extern "C" int qz_sim_write_fields(qz_sim* h, const char* path) { QZ_SIM_TRY(h->s->write_fields(path)) }
+extern "C" long qz_sim_lost(const qz_sim* h) {
+    if (!h) {
+        qz_fail(QZ_EINVAL, "qz_sim_lost: null argument");
+        return -1;
+    }
+    return h->s->lost_last;
+}
+extern "C" double qz_sim_time(const qz_sim* h) {
+    if (!h) {
+        qz_fail(QZ_EINVAL, "qz_sim_time: null argument");
+        return std::nan("");
+    }
+    return h->s->t;
+}
+extern "C" int qz_sim_write_fields(qz_sim* h, const char* path) {
+    QZ_NEED(h && path, "qz_sim_write_fields");
+    QZ_SIM_TRY(h->s->write_fields(path))
+}

@@ -121,8 +121,13 @@ struct Beam {
     void set_profile(const std::vector<double>& values, hipStream_t st);   // new weights from the profile
     void set_particles(const std::vector<double>& p7, hipStream_t st);
     qz_beam_soa soa() const { return {a[0].p, a[1].p, a[2].p, a[3].p, a[4].p, a[5].p, a[6].p}; }
+    // the 17 sums of Beam::sums in the reference's index order (host copy of the particles): the checker's form
     std::array<double, 17> sums(hipStream_t st) const;
+    // the same sums reduced on the device (qz_beam_sums_device, fixed tree order): the beam log and the
+    // differentiable objective; only 17 doubles cross to the host
+    std::array<double, 17> sums_device(hipStream_t st) const;
     BeamDiag from_sums(const std::array<double, 17>& S, Real t) const;
+    mutable DevBuf<Real> sums_work;   // 17 results, then the reduction's block partials
 };
 
 // ---------------------------------------------------------------------------- Simulation (src/Simulation.*)
@@ -171,10 +176,12 @@ public:
     DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..16] sweep phase ticks, [20] beam deposited, [21] pushed
     DevBuf<char> dep_work;
     qz_sweep_args sweep_args{};
+    DevBuf<qz_sweep_args> d_args;   // this run's device copy of the sweep arguments (batched: every run's)
     std::vector<Real> bg_host;   // host side of the background upload (kept alive for the async copy)
     bool bg_ready = false;       // the background is the same every step: computed and uploaded once
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
+    int sweep_launches = 1;      // sweep kernel launches of the last step (pipelined: its slice ranges)
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
     double phase_ticks[QZ_NPHASE] = {};
     hipStream_t stream = nullptr;

@@ -41,50 +41,62 @@ extern "C" int qz_device_info(char* arch, int arch_len, int* n_cu) {
 }
 
 namespace qz {
-// Thomas, one thread per system (src/Tridiag.cpp:111-126); c' and d' kept in x and a scratch-free way:
-// the forward sweep stores d' in x and c' is recomputed in the backward sweep from the same operands,
-// so no workspace is needed and every value is formed exactly as in the reference.
+// Thomas, one thread per system (src/Tridiag.cpp:111-126), every value formed exactly as the reference forms it.
+// Element i of system s sits at s * ss + i * es of a, b, c, d and x; the forward sweep keeps d' in x and c' in
+// the caller's workspace (system s at s * M), which the backward sweep reads.
 __global__ void thomas_batched_kernel(const Real* __restrict__ a, const Real* __restrict__ b, const Real* __restrict__ c,
                                       const Real* __restrict__ d, Real* __restrict__ x, Real* __restrict__ cpw, int M,
-                                      int batch) {
+                                      int batch, long long ss, long long es) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= batch) return;
-    const size_t o = static_cast<size_t>(s) * M;
+    const long long o = static_cast<long long>(s) * ss;
     const Real *A = a + o, *B = b + o, *C = c + o, *D = d + o;
     Real* X = x + o;
-    Real* CP = cpw + o;
+    Real* CP = cpw + static_cast<size_t>(s) * M;
     Real cp = C[0] / B[0];
     Real dp = D[0] / B[0];
     CP[0] = cp;
     X[0] = dp;
     for (int i = 1; i < M; ++i) {
-        const Real m = Real(1) / (B[i] - A[i] * cp);
-        cp = C[i] * m;
-        dp = (D[i] - A[i] * dp) * m;
+        const long long e = i * es;
+        const Real m = Real(1) / (B[e] - A[e] * cp);
+        cp = C[e] * m;
+        dp = (D[e] - A[e] * dp) * m;
         CP[i] = cp;
-        X[i] = dp;
+        X[e] = dp;
     }
-    Real xn = X[M - 1];
+    Real xn = X[(M - 1) * es];
     for (int i = M - 2; i >= 0; --i) {
-        xn = X[i] - CP[i] * xn;
-        X[i] = xn;
+        xn = X[i * es] - CP[i] * xn;
+        X[i * es] = xn;
     }
 }
 }  // namespace qz
 
+extern "C" size_t qz_tridiag_workspace_doubles(int M, int batch) {
+    return (M < 1 || batch < 1) ? 0 : static_cast<size_t>(M) * static_cast<size_t>(batch);
+}
+
 extern "C" int qz_tridiag_thomas_batched(const double* a, const double* b, const double* c, const double* d, double* x,
-                                         int M, int batch, void* stream) {
-    if (!a || !b || !c || !d || !x) return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: null argument");
+                                         int M, int batch, long long sys_stride, long long elem_stride, double* work,
+                                         void* stream) {
     if (M < 1 || batch < 0) return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: bad extent");
     if (batch == 0) return QZ_OK;
+    if (!a || !b || !c || !d || !x || !work) return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: null argument");
+    if (elem_stride < 1 || (batch > 1 && sys_stride < 1))
+        return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: strides must be positive");
+    // systems may interleave (element-major layouts), but no two elements of the batch may share an address:
+    // (s, i) and (s + ds, i - di) collide when ds * sys_stride = di * elem_stride with di <= M - 1
+    const long long span = (M - 1) * elem_stride + 1;
+    if (batch > 1 && sys_stride < span)
+        for (long long si = 1; si < batch && si * sys_stride < span; ++si)
+            if ((si * sys_stride) % elem_stride == 0)
+                return qz_fail(QZ_EINVAL, "qz_tridiag_thomas_batched: strides make two systems overlap");
     hipStream_t st = static_cast<hipStream_t>(stream);
-    double* cp = nullptr;
-    QZ_HIP(hipMallocAsync(reinterpret_cast<void**>(&cp), sizeof(double) * static_cast<size_t>(M) * batch, st));
     const int tb = 64;
-    hipLaunchKernelGGL(qz::thomas_batched_kernel, dim3((batch + tb - 1) / tb), dim3(tb), 0, st, a, b, c, d, x, cp, M, batch);
-    hipError_t e = hipGetLastError();
-    QZ_HIP(hipFreeAsync(cp, st));
-    if (e != hipSuccess) return qz_fail_hip(e, "thomas launch");
+    hipLaunchKernelGGL(qz::thomas_batched_kernel, dim3((batch + tb - 1) / tb), dim3(tb), 0, st, a, b, c, d, x, work, M,
+                       batch, sys_stride, elem_stride);
+    QZ_HIP(hipGetLastError());
     return QZ_OK;
 }
 
@@ -156,6 +168,18 @@ static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
 }
 
 extern "C" int qz_sweep(const qz_sweep_args* args, void* stream) { return sweep_many(args, 1, stream); }
+extern "C" int qz_sweep_release(void* stream) {
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int dev = 0;
+    QZ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_args_mu);
+    auto it = g_args.find({dev, st});
+    if (it == g_args.end()) return QZ_OK;
+    QZ_HIP(hipStreamSynchronize(st));   // its kernels may still read the block
+    if (it->second.p) QZ_HIP(hipFree(it->second.p));
+    g_args.erase(it);
+    return QZ_OK;
+}
 extern "C" int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream) {
     return sweep_many(runs, nruns, stream);
 }

@@ -24,10 +24,11 @@ enum BRec : int { R_WQ = 0, R_TK, R_T, R_OMVZ, R_VZ, R_VR, R_VT, R_C, R_S, R_VX,
 // every particle once, also those that slip into a later range.
 __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, Real qm, int mode, int i0, int np,
                                     qz_beam_soa P, const Real* __restrict__ fld, int nc, Real dt_kick, Real dt,
-                                    int move, int s_lo, int s_hi, int step, int* __restrict__ pstep) {
+                                    int move, int s_lo, int s_hi, int step, int* __restrict__ pstep,
+                                    unsigned long long* __restrict__ nmarked) {
     const int i = i0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    if (P.w[i] == Real(0)) return;
+    const bool live = i < np && P.w[i] != Real(0);
+    if (!live) return;
     const Real xmin = bg.xi_min, idxi = Real(1) / bg.dxi;
     const Real xmax = bg.xi_min + (bg.nxi - 1) * bg.dxi;
     const int k0 = bg.k0, nloc = bg.nloc;
@@ -55,6 +56,8 @@ __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, R
         // the ranges this step has still to push) is not pushed again
         if (k < s_lo || k >= s_hi || pstep[i] >= step) return;
         pstep[i] = step;
+        // the live particles this step pushes, each once: Simulation::step's count before the push
+        if (nmarked) atomicAdd(nmarked, 1ull);
     }
     const int k1 = ngp ? k : k + 1;
     int j; Real t;
@@ -129,9 +132,11 @@ __global__ void __launch_bounds__(256) beam_count_live_kernel(int np, const Real
 // i0: the particle's index in the deposit's common index space (beams after one another).  pstep (optional):
 // particles whose marker is `marker` or later are left out (a pipelined step's deposit skips the particles
 // that step, or a later one, has already pushed)
+// live (optional): live particles whose lower slice lies in the window's rows [k0, k0 + nloc) -- each live
+// particle of the box once over a step's windows, the count Simulation::step takes before its deposit.
 __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int i0,
                                 const int* __restrict__ pstep, int marker, int* __restrict__ key,
-                                Real* __restrict__ rec, int* __restrict__ counts) {
+                                Real* __restrict__ rec, int* __restrict__ counts, unsigned long long* __restrict__ live) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
     const int gi = i0 + i;
@@ -146,6 +151,7 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     if (pstep && pstep[i] >= marker) return;
     const Real wi = P.w[i];
     if (wi == Real(0)) return;
+    if (live && k >= k0) atomicAdd(live, 1ull);
     const Real X = P.x[i], Y = P.y[i];
     const Real r = dsqrt(X * X + Y * Y);
     if (r > g.R) return;
@@ -435,7 +441,7 @@ extern "C" size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N) {
 // the first beam's contributions in particle order, then the next beam's, as nbeams deposits one after another do.
 static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
                          const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
-                         const int* const* pstep, int marker, hipStream_t st) {
+                         const int* const* pstep, int marker, unsigned long long* live, hipStream_t st) {
     long ntot = 0;
     for (int b = 0; b < nbeams; ++b) {
         if (np[b] < 0) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
@@ -466,7 +472,7 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
     for (int b = 0, i0 = 0; b < nbeams; i0 += np[b], ++b)
         if (np[b] > 0)
             hipLaunchKernelGGL(beam_key_kernel, dim3((np[b] + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q[b], np[b], p[b],
-                               i0, pstep ? pstep[b] : nullptr, marker, key, rec, counts);
+                               i0, pstep ? pstep[b] : nullptr, marker, key, rec, counts, live);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
     const int nb = (n1 + 1023) / 1024;
     if (nb <= kScanOne) {
@@ -488,14 +494,16 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
 extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,
                                const qz_beam_soa* p, double* bsrc, int nc, void* work, void* stream) {
     if (!p) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
-    return deposit_beams(grid, box, modes, 1, &q, &np, p, bsrc, nc, work, nullptr, 0, static_cast<hipStream_t>(stream));
+    return deposit_beams(grid, box, modes, 1, &q, &np, p, bsrc, nc, work, nullptr, 0, nullptr,
+                         static_cast<hipStream_t>(stream));
 }
 
 extern "C" int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams,
                                      const double* q, const int* np, const qz_beam_soa* p, double* bsrc, int nc,
-                                     void* work, const int* const* pstep, int marker, void* stream) {
+                                     void* work, const int* const* pstep, int marker, unsigned long long* live_count,
+                                     void* stream) {
     if (nbeams < 1 || !q || !np || !p) return qz_fail(QZ_EINVAL, "qz_beam_deposit_beams: null argument or no beam");
-    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, pstep, marker,
+    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, pstep, marker, live_count,
                          static_cast<hipStream_t>(stream));
 }
 
@@ -511,7 +519,7 @@ extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* b
     const int tb = g_block[QZ_K_BEAM_ADVANCE];
     hipLaunchKernelGGL(beam_advance_kernel, dim3((np - i0 + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
                        *grid, *box, modes == 1 ? 1 : 0, qm, pusher, i0, np, *p, fld, nc, dt_kick, dt, move, 0, 0, 0,
-                       static_cast<int*>(nullptr));
+                       static_cast<int*>(nullptr), static_cast<unsigned long long*>(nullptr));
     QZ_HIP(hipGetLastError());
     return QZ_OK;
 }
@@ -519,7 +527,7 @@ extern "C" int qz_beam_advance(const qz_radial_grid* grid, const qz_beam_grid* b
 extern "C" int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double qm,
                                       int pusher, int np, const qz_beam_soa* p, const double* fld, int nc,
                                       double dt_kick, double dt, int move, int s_lo, int s_hi, int step, int* pstep,
-                                      void* stream) {
+                                      unsigned long long* pushed_count, void* stream) {
     if (!grid || !box || !p || !fld || !pstep) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: null argument");
     if (np < 0 || grid->N < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: bad extent");
     if (!box->ngp && box->nloc < 2) return qz_fail(QZ_EINVAL, "qz_beam_advance_slices: linear shape needs >= 2 slices");
@@ -529,7 +537,7 @@ extern "C" int qz_beam_advance_slices(const qz_radial_grid* grid, const qz_beam_
     const int tb = g_block[QZ_K_BEAM_ADVANCE];
     hipLaunchKernelGGL(beam_advance_kernel, dim3((np + tb - 1) / tb), dim3(tb), 0, static_cast<hipStream_t>(stream),
                        *grid, *box, modes == 1 ? 1 : 0, qm, pusher, 0, np, *p, fld, nc, dt_kick, dt, move, s_lo, s_hi,
-                       step, pstep);
+                       step, pstep, pushed_count);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
 }

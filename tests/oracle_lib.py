@@ -76,7 +76,7 @@ def lib():
         L.or_sim_get.restype = C.c_long
         L.or_sim_get.argtypes = [C.c_void_p, C.c_char_p, DP, C.c_long]
         L.or_sim_beam_sums.argtypes = [C.c_void_p, C.c_int, DP, DP]
-        L.or_sim_beam_diag.argtypes = [C.c_void_p, C.c_int, DP]
+        L.or_sim_beam_diag.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         L.or_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
         L.or_sim_write_axis.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
@@ -302,9 +302,11 @@ class Sim:
         check(lib().or_sim_beam_sums(self.h, b, _p(out), _p(refs)))
         return out, refs
 
-    def beam_diag(self, b):
+    def beam_diag(self, b, order="index"):
+        """Beam log values from the index-order sums (the reference's) or, order="tree", from the sums in the
+        product's on-device reduction order."""
         out = np.zeros(15)
-        check(lib().or_sim_beam_diag(self.h, b, _p(out)))
+        check(lib().or_sim_beam_diag(self.h, b, int(order == "tree"), _p(out)))
         keys = ["t", "npart", "charge", "gamma_mean", "gamma_rms", "xi_mean", "xi_rms", "r_rms", "emit_nx", "x_mean",
                 "y_mean", "emit_ny", "sigma_x", "sigma_y", "alive"]
         return dict(zip(keys, out))

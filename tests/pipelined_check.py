@@ -14,14 +14,18 @@ DECKS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "decks")
 CASES = [
     (os.path.join(DECKS, "gain.in"), {"time.steps": 4, "driver.nparticles": 20000, "xi.min": -8, "witness.gamma": 3,
                                       "witness.sigma_xi": 1.0, "witness.xi0": -4, "witness.n0": 0.001}, 3, 32, 3),
-    (configs.deck("c2"), {"time.steps": 6}, 4, 128, 4),
+    # the bench's c2 setting: two warm-up steps, then the workload's 20 time steps with 20 in flight over 128
+    # slice ranges (the run has 23 steps, so the last timed step is followed by a push like every other)
+    (configs.deck("c2"), {"time.steps": 23}, 20, 128, 20),
 ]
 bad = 0
 for deck, ov, steps, chunks, depth in CASES:
     a = quarz_amd.Simulation(deck, ov)
     b = quarz_amd.Simulation(deck, ov)
-    a.step()
-    b.step()
+    warm = 2 if depth == 20 else 1
+    for _ in range(warm):
+        a.step()
+        b.step()
     for _ in range(steps):
         a.step()
     b.run_pipelined(steps, chunks=chunks, depth=depth)

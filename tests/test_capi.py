@@ -66,7 +66,7 @@ def test_entry_points_reject_bad_arguments_before_any_gpu_call():
 
     def dep(nbeams, nc):
         return L.qz_beam_deposit_beams(C.byref(g), C.byref(box), 0, nbeams, q, n, C.byref(soa), dummy, nc, dummy,
-                                       None, 0, None)
+                                       None, 0, None, None)
 
     assert dep(0, 5) == -1 and "no beam" in quarz_amd.last_error()
     assert dep(2, 5) == -1 and "bad extent" in quarz_amd.last_error()       # N < 2
@@ -78,8 +78,15 @@ def test_entry_points_reject_bad_arguments_before_any_gpu_call():
     assert L.qz_beam_advance(C.byref(g), C.byref(box), 0, -1.0, 0, 0, 5, C.byref(soa), dummy, 18, 1.0, 1.0, 1,
                              None) == -1
     assert "nc must be" in quarz_amd.last_error()
-    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 0, 1, None) == -1
+    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 0, 1, 1, 1, dummy, None) == -1
     assert "bad extent" in quarz_amd.last_error()
+    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 4, 2, 4, 1, None, None) == -1
+    assert "null argument" in quarz_amd.last_error()                         # no workspace
+    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 4, 2, 2, 1, dummy, None) == -1
+    assert "overlap" in quarz_amd.last_error()                               # system 1 starts inside system 0
+    assert L.qz_tridiag_thomas_batched(dummy, dummy, dummy, dummy, dummy, 4, 3, 2, 4, dummy, None) == -1
+    assert "overlap" in quarz_amd.last_error()                               # (2, 0) is (0, 1)
+    assert L.qz_tridiag_workspace_doubles(5, 3) == 15
 
 
 def test_sweep_rejects_unsupported_sizes_before_any_gpu_call():
@@ -90,3 +97,26 @@ def test_sweep_rejects_unsupported_sizes_before_any_gpu_call():
     blob = (C.c_char * 4096)()           # larger than qz_sweep_args, all zero: modes 0, N 0
     assert L.qz_sweep(blob, None) == -2 and "radial cells" in quarz_amd.last_error()
     assert L.qz_sweep_batched(blob, 0, None) == -1 and "no runs" in quarz_amd.last_error()
+
+
+def test_null_handles_are_errors():
+    """Every qz_sim_* entry point answers a null run handle (or a null output) with QZ_EINVAL and a message,
+    before any device work."""
+    import ctypes as C
+    import math
+    import quarz_amd
+    L = quarz_amd.lib()
+    out = (C.c_double * 32)()
+    for fn, args in [("qz_sim_step", ()), ("qz_sim_step_fields", ()), ("qz_sim_push_beams", ()),
+                     ("qz_sim_run_pipelined", (2, 4, 2, None)), ("qz_sim_beam_sums", (0, out, None)),
+                     ("qz_sim_beam_diag", (0, out)), ("qz_sim_write_fields", (b"/tmp/x.bin",)),
+                     ("qz_sim_timing", (out,)), ("qz_sim_stats", (out,)), ("qz_sim_phase_ticks", (out,)),
+                     ("qz_sim_snapshot", ()), ("qz_sim_rewind", ()), ("qz_sim_info", ((C.c_int * 10)(),))]:
+        assert getattr(L, fn)(None, *args) == -1, fn
+        assert "null" in quarz_amd.last_error(), (fn, quarz_amd.last_error())
+    L.qz_sim_get.restype = C.c_long
+    assert L.qz_sim_get(None, b"fld", out, 32) == -1 and "null" in quarz_amd.last_error()
+    L.qz_sim_lost.restype = C.c_long
+    assert L.qz_sim_lost(None) == -1 and "null" in quarz_amd.last_error()
+    L.qz_sim_time.restype = C.c_double
+    assert math.isnan(L.qz_sim_time(None)) and "null" in quarz_amd.last_error()

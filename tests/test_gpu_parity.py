@@ -50,11 +50,17 @@ def test_thomas_batched_bit_exact(torch_dev, qa, oracle, M):
     dev = [torch.tensor(v, dtype=torch.float64, device="cuda") for v in (a, b, c, d)]
     x = torch.zeros((batch, M), dtype=torch.float64, device="cuda")
     qa.thomas_batched(*dev, x)
+    # the same systems element-major (element i of system s at i * batch + s)
+    devT = [t.t().contiguous() for t in dev]
+    xT = torch.zeros((M, batch), dtype=torch.float64, device="cuda")
+    qa.thomas_batched(*devT, xT, element_major=True)
     torch.cuda.synchronize()
     got = x.cpu().numpy()
+    gotT = xT.cpu().numpy().T
     for s in range(batch):
         ref = oracle.thomas(a[s].copy(), b[s].copy(), c[s].copy(), d[s].copy())
         assert _same(got[s], ref), _report(f"system {s}", got[s], ref)
+        assert _same(gotT[s], ref), _report(f"element-major system {s}", gotT[s], ref)
 
 
 # ------------------------------------------------------------------------------------------ beam kernels
@@ -211,9 +217,12 @@ def test_steps_gain_bit_exact(qa, oracle, decks):
     for k in qa.BEAM_ARRAYS:
         a, b = g.get(f"beam1.{k}"), o.get(f"beam1.{k}")
         assert _same(a, b), _report(f"witness {k}", a, b)
-    da, db = g.beam_diag(1), o.beam_diag(1)
+    # the beam log values: the product reduces the sums on the device, in a fixed tree order the oracle restates
+    da, db = g.beam_diag(1), o.beam_diag(1, order="tree")
     for k in da:
         assert da[k] == db[k], (k, da[k], db[k])
+    # the reference's index-order sums (host copy) bit for bit
+    assert np.array_equal(g.beam_sums(1)[0], o.beam_sums(1)[0])
 
 
 def test_betatron_m1_steps_bit_exact(qa, oracle, decks):
@@ -279,7 +288,8 @@ def test_axis_file_equals_oracle(qa, oracle, decks, tmp_path, deck, ov):
 
 def test_beam_log_equals_oracle(qa, oracle, decks, tmp_path):
     """beams.txt (Simulation::write_beam_output, one line per beam and step, after the sweep and before the
-    push): the header, and every value equal to the oracle's beam log of the same beam state at 10 digits."""
+    push): the header, and every value equal to the oracle's beam log of the same beam state at 10 digits (the
+    sums reduced on the device, in the tree order the oracle restates)."""
     ov = {"time.steps": 3, "driver.nparticles": 20000, "xi.min": -8}
     g = qa.Simulation(f"{decks}/gain.in", {**ov, "output.dir": str(tmp_path)})
     o = oracle.Sim(f"{decks}/gain.in", ov)
@@ -287,7 +297,7 @@ def test_beam_log_equals_oracle(qa, oracle, decks, tmp_path):
     for n in range(3):
         # the oracle's beams as the GPU run logs them: after step n - 1's push, at the time of step n
         for b in range(o.nbeams):
-            d = o.beam_diag(b)
+            d = o.beam_diag(b, order="tree")
             want.append((n, b, d))
         g.step()
         o.step()
@@ -325,19 +335,65 @@ def test_torch_keeps_the_device_after_the_library(qa):
 
 
 def test_c1_workload_bit_exact(qa, oracle):
-    """The c1 workload at full size (m = 0, 256 uniform cells, 1601 slices, 1e5-particle driver), three steps:
-    fields, diagnostics, beam sources and the driver's phase space equal the oracle's bit for bit."""
+    """The c1 workload at full size and length (m = 0, 256 uniform cells, 1601 slices, 1e5-particle driver, its
+    10 time steps: 11 sweeps and 10 pushes): fields, diagnostics, beam sources, the driver's phase space and its
+    beam-log moments after the 10 steps equal the oracle's bit for bit."""
     from quarz_amd import configs
     deck = configs.deck("c1")
-    g = qa.Simulation(deck, {"time.steps": 3})
-    o = oracle.Sim(deck, {"time.steps": 3})
-    for _ in range(3):
+    g = qa.Simulation(deck)
+    o = oracle.Sim(deck)
+    assert g.nsteps == 10
+    for _ in range(11):
         g.step()
         o.step()
     _compare_sims(g, o)
     for k in qa.BEAM_ARRAYS:
         a, b = g.get(f"beam0.{k}"), o.get(f"beam0.{k}")
         assert _same(a, b), _report(f"driver {k}", a, b)
+    da, db = g.beam_diag(0), o.beam_diag(0, order="tree")
+    for k in da:
+        assert da[k] == db[k], (k, da[k], db[k])
+    # the reference's index-order moments after the 10 steps, bit for bit; the device order differs from them
+    # only in the last bits (the north star's moment gate is 1e-9)
+    sg, so = g.beam_sums(0)[0], o.beam_sums(0)[0]
+    assert np.array_equal(sg, so)
+    di = o.beam_diag(0)
+    for k in ("gamma_mean", "gamma_rms", "r_rms", "emit_nx", "xi_mean", "xi_rms"):
+        assert abs(da[k] - di[k]) <= 1e-12 * abs(di[k]), (k, da[k], di[k])
+
+
+@pytest.mark.parametrize("m1", [False, True])
+def test_two_mobile_species_bit_exact(qa, oracle, decks, m1):
+    """Electrons and mobile hydrogen ions (two species in one sweep: the positive species' deposit into n_i, the
+    split of the particle range between species, the second species' push).  m = 0: the shape of the
+    reference's ion-motion validation (examples/pwfa_ion_motion.in, stretched 512 cells, plasma to r = 7, ppc 4
+    each, about 3.9e3 particles per slice) with smaller beams, three steps.  m = 1: an offset driver, 8 azimuths
+    per ring for both species, three steps."""
+    if not m1:
+        deck = f"{decks}/pwfa_ion_motion.in"
+        ov = {"xi.min": -7.3, "time.steps": 2, "driver.nparticles": 40000, "witness.nparticles": 20000}
+    else:
+        deck = f"{decks}/linear.in"
+        ov = {"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
+              "driver.nparticles": 20000, "driver.sigma_r": 0.5, "driver.rigid": 0, "time.steps": 2, "time.dt": 5,
+              "plasma.species": "electrons ions", "electrons.ppc": 1, "ions.ppc": 1, "ions.charge": 1,
+              "ions.mass": 1836.15, "ions.mobile": 1}
+    g = qa.Simulation(deck, ov)
+    o = oracle.Sim(deck, ov)
+    assert o.nspecies == 2
+    for _ in range(3):
+        g.step()
+        o.step()
+    _compare_sims(g, o)
+    for bi in range(o.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            a, b = g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")
+            assert _same(a, b), _report(f"beam{bi} {k}", a, b)
+    for sp in range(2):
+        for k in ("x", "y", "px", "py", "dl", "w"):
+            a, b = g.get(f"sp{sp}.{k}"), o.get(f"sp{sp}.{k}")
+            assert _same(a, b), _report(f"species {sp} {k}", a, b)
+    assert g.lost() == o.lost()
 
 
 def test_c2_workload_pipelined_bit_exact(qa, oracle, tmp_path):
@@ -382,6 +438,41 @@ def test_c5_runs_batched_bit_exact(qa, oracle, tmp_path):
             for k in qa.BEAM_ARRAYS:
                 a, b = g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")
                 assert _same(a, b), _report(f"run {i} beam{bi} {k}", a, b)
+
+
+def test_c5_bench_batch_extremes_equal_plain(qa, tmp_path):
+    """The bench's c5 batch: all 256 runs stepped as a Batch with graph replays (the eager start-up step, then
+    four replays, the first of which captures the graph) for the workload's 5 steps.  The runs at the extremes (fewest and most cells,
+    fewest and most slices) equal the same decks stepped one run at a time, bit for bit."""
+    from quarz_amd import configs
+    runs = configs.c5_runs(256, steps=7, workdir=str(tmp_path))
+    sims = [qa.Simulation(deck) for deck, _ in runs]
+    batch = qa.Batch(sims, graph=True)
+    try:
+        for _ in range(5):
+            batch.step()
+        replays = batch.replays
+    finally:
+        batch.close()
+    assert replays == 4
+    info = [i for _, i in runs]
+    pick = sorted({min(range(256), key=lambda i: (info[i]["N"], info[i]["K"])),
+                   max(range(256), key=lambda i: (info[i]["N"], info[i]["K"])),
+                   min(range(256), key=lambda i: (info[i]["K"], info[i]["N"])),
+                   max(range(256), key=lambda i: (info[i]["K"], info[i]["N"]))})
+    for i in pick:
+        p = qa.Simulation(runs[i][0])
+        for _ in range(5):
+            p.step()
+        for w in ("fld", "diag", "bsrc"):
+            a, b = getattr(sims[i], w), getattr(p, w)
+            assert _same(a, b), _report(f"run {i} ({info[i]}) {w}", a, b)
+        for bi in range(2):
+            for k in qa.BEAM_ARRAYS:
+                a, b = sims[i].get(f"beam{bi}.{k}"), p.get(f"beam{bi}.{k}")
+                assert _same(a, b), _report(f"run {i} beam{bi} {k}", a, b)
+        assert sims[i].lost() == p.lost()
+        del p
 
 
 # ------------------------------------------------------------------------------------------ batched runs
@@ -491,9 +582,16 @@ def test_pipelined_steps_equal_plain(qa, decks, deck, ov, steps, chunks, depth):
     b = qa.Simulation(f"{decks}/{deck}", ov)
     a.step()
     b.step()   # the leapfrog start-up step
-    for _ in range(steps):
-        a.step()
-    b.run_pipelined(steps, chunks=chunks, depth=depth)
+    psl = pushed = 0.0
+    for i in range(steps):
+        a.step()   # step 1 + i; the last one (1 + i == nsteps) has no push
+        st = a.stats()
+        psl += st["particle_slices"]
+        pushed += st["beam_pushed"] if 1 + i < a.nsteps else 0.0
+    bpsl, bpushed = b.run_pipelined(steps, chunks=chunks, depth=depth)
+    # the work counts of the overlapped steps are the serial ones (live particles before each deposit and push)
+    assert (bpsl, bpushed) == (psl, pushed)
+    assert b.stats()["beam_deposited"] == a.stats()["beam_deposited"]
     for w in ("fld", "diag", "bsrc"):
         ga, gb = getattr(b, w), getattr(a, w)
         assert _same(ga, gb), _report(f"pipelined {w}", ga, gb)
@@ -602,7 +700,7 @@ def test_beam_sums_device_matches_host(torch_dev, qa, np_):
     assert got[16] == ref[16] == np.count_nonzero(host["w"])
     # absolute scale of every sum: the same sums over |terms|
     w = host["w"]
-    g = np.sqrt(1 + host["px"] ** 2 + host["py"] ** 2 + host["pz"] ** 2)
+    g = np.sqrt(1 + host["px"] * host["px"] + host["py"] * host["py"] + host["pz"] * host["pz"])
     G, XI = g - gref, host["xi"] - xiref
     X, Y, PX, PY = host["x"], host["y"], host["px"], host["py"]
     terms = [np.ones_like(w), G, G * G, XI, XI * XI, X * X + Y * Y, X, Y, X * X, Y * Y, PX, PY, PX * PX, PY * PY,
@@ -610,3 +708,21 @@ def test_beam_sums_device_matches_host(torch_dev, qa, np_):
     for c, v in enumerate(terms):
         scale = np.abs(w * v).sum()
         assert abs(got[c] - ref[c]) <= 1e-12 * scale + 1e-300, (c, got[c], ref[c], scale)
+    # and exactly the fixed tree order: blocks of 4096 particles, lane t of 256 accumulating t, t + 256, ... in
+    # index order (dead particles skipped), lanes halved pairwise, block partials added in block order
+    live = w != 0
+    vals = np.stack([live.astype(np.float64), w] + [w * v for v in terms[1:]])   # 17 x np: count, w, w v
+    tree = np.zeros(17)
+    for lo in range(0, np_, 4096):
+        acc = np.zeros((17, 256))
+        for i0 in range(lo, min(np_, lo + 4096), 256):
+            n = min(256, np_ - i0)
+            blk = vals[:, i0:i0 + n]
+            acc[:, :n] = np.where(live[i0:i0 + n], acc[:, :n] + blk, acc[:, :n])
+        h = 128
+        while h > 0:
+            acc[:, :h] = acc[:, :h] + acc[:, h:2 * h]
+            h //= 2
+        tree = tree + acc[:, 0]
+    order = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 0]   # device layout: w, w v..., count
+    assert np.array_equal(got, tree[order]), (got, tree[order])

@@ -90,7 +90,7 @@ def test_rewind_and_new_profile_equal_oracle(qa, oracle):
     for bi in range(2):
         for k in qa.BEAM_ARRAYS:
             assert _same(g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")), (bi, k)
-    da, db = g.beam_diag(1), o.beam_diag(1)
+    da, db = g.beam_diag(1), o.beam_diag(1, order="tree")
     for k in da:
         assert da[k] == db[k], (k, da[k], db[k])
 
@@ -98,7 +98,8 @@ def test_rewind_and_new_profile_equal_oracle(qa, oracle):
 @pytest.mark.gpu
 def test_batched_objective_and_fd_gradient(qa, oracle):
     """The workload at full size: J of 2 S + 1 profiles in one batched step.  J(v) equals a plain run's, one
-    gradient component equals the oracle's central difference bit for bit, and the gradient predicts the
+    gradient component equals the oracle's central difference bit for bit (J from moments summed in the
+    on-device reduction's order, which the oracle restates), and the gradient predicts the
     central difference along a random direction (at this size J is smooth at the scale of eps: eps 1e-6 and
     1e-7 agree to about 1e-5 relative; a smaller box with 10x fewer beam particles is not)."""
     from quarz_amd import witness
@@ -130,7 +131,7 @@ def _oracle_J(oracle, deck, v, s, step):
     assert old in deck
     o = oracle.Sim(deck.replace(old, "witness.pwl_weights = " + " ".join(f"{x:.17g}" for x in vv)))
     o.step()
-    return witness.relative_spread(o.beam_diag(1))
+    return witness.relative_spread(o.beam_diag(1, order="tree"))
 
 
 @pytest.mark.gpu
