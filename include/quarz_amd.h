@@ -129,8 +129,8 @@ int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int mod
  * whose marker is `marker` or later are left out.  A step run range by range deposits a range after it has pushed
  * earlier ranges; the particles it pushed there were deposited before their push and must not be seen again
  * at their new places (a slow particle can slip into a later range).  live_count (optional, device): adds the
- * live particles whose lower slice is in rows [k0, k0 + nloc), so the windows of one step count every live
- * particle of the box once. */
+ * live particles whose lower slice is in rows [k0, k0 + nloc) (the first window of the box also those before it,
+ * the last one those after it), so the windows of one step count every live particle once. */
 int qz_beam_deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
                           const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
                           const int* const* pstep, int marker, unsigned long long* live_count, void* stream);

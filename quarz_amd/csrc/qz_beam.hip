@@ -132,8 +132,9 @@ __global__ void __launch_bounds__(256) beam_count_live_kernel(int np, const Real
 // i0: the particle's index in the deposit's common index space (beams after one another).  pstep (optional):
 // particles whose marker is `marker` or later are left out (a pipelined step's deposit skips the particles
 // that step, or a later one, has already pushed)
-// live (optional): live particles whose lower slice lies in the window's rows [k0, k0 + nloc) -- each live
-// particle of the box once over a step's windows, the count Simulation::step takes before its deposit.
+// live (optional): live particles whose lower slice lies in the window's rows [k0, k0 + nloc), those before the
+// box in the box's first window and those after it in its last -- each live particle once over a step's windows,
+// the count Simulation::step takes before its deposit.
 __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int np, qz_beam_soa P, int i0,
                                 const int* __restrict__ pstep, int marker, int* __restrict__ key,
                                 Real* __restrict__ rec, int* __restrict__ counts, unsigned long long* __restrict__ live) {
@@ -147,11 +148,15 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     const bool ngp = bg.ngp != 0;
     const Real sx = (P.xi[i] - xmin) * idxi;
     const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
+    if (live) {   // the window's share of the live count: particles before (after) the box go to the first (last)
+        const bool first = k0 == 0, last = k0 + nloc >= bg.nxi;
+        if ((k >= k0 || first) && (k < k0 + nloc || last) && !(pstep && pstep[i] >= marker) && P.w[i] != Real(0))
+            atomicAdd(live, 1ull);
+    }
     if (k < k0 - 1 || k >= k0 + nloc) return;
     if (pstep && pstep[i] >= marker) return;
     const Real wi = P.w[i];
     if (wi == Real(0)) return;
-    if (live && k >= k0) atomicAdd(live, 1ull);
     const Real X = P.x[i], Y = P.y[i];
     const Real r = dsqrt(X * X + Y * Y);
     if (r > g.R) return;
