@@ -174,6 +174,15 @@ int qz_sweep(const qz_sweep_args* args, void* stream);
 
 /* Many independent runs in one persistent launch, one workgroup per run (not in the reference). */
 int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream);
+/* No counterpart in the reference: qz_sweep for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).  It allocates nothing: the argument block
+ * and the run's workspace (args->ws is ignored) live in a static device array of the library, about 0.9 MB per
+ * GPU for the life of the process, which limits the grid to 2049 nodes (larger grids: QZ_ERANGE).  Every call on
+ * one GPU shares that array: calls whose sweeps overlap in time race, and a launch that continues a slice range
+ * (k_begin > 0) reads the B+ warm start the previous call on that GPU left there.  The block is written by a small
+ * kernel, so a captured call keeps its arguments in the graph. */
+int qz_sweep_noalloc(const qz_sweep_args* args, void* stream);
+
 /* Frees the argument staging buffer qz_sweep / qz_sweep_batched keep for `stream` on the current device, after
  * the stream's work is done: call it before destroying a stream the sweeps were launched on (a new stream can
  * reuse the handle).  No counterpart in the reference. */
@@ -193,6 +202,10 @@ int qz_sim_info(const qz_sim* s, int* info10);
 /* one step of Simulation::run (src/Simulation.cpp:1376-1459): sweep, then the beam push unless last */
 int qz_sim_step(qz_sim* s);
 int qz_sim_step_fields(qz_sim* s);   /* the sweep only */
+/* The part of Simulation::step_fields before the slice loop (beam deposit, plasma load, background; the run's
+ * stream is synchronised on return) and the argument block of its sweep, for a caller that launches the sweep
+ * itself (qz_sweep, qz_sweep_noalloc) on the run's arrays.  The run's step counter does not advance. */
+int qz_sim_prepare_sweep(qz_sim* s, qz_sweep_args* out);
 /* One step of n independent runs (same modes) whose sweeps share one persistent launch, one workgroup per
  * run (qz_sweep_batched); beam deposits and pushes run per run.  batch_ms (optional) receives the time of
  * the batched sweep launch.  Not in the reference: it batches what Simulation::run does for one run. */

@@ -78,6 +78,9 @@ def lib():
         L.qz_deck_keys.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_long]
         L.qz_sim_step.argtypes = [C.c_void_p]
         L.qz_sim_step_fields.argtypes = [C.c_void_p]
+        L.qz_sim_prepare_sweep.argtypes = [C.c_void_p, C.c_void_p]
+        L.qz_sweep.argtypes = [C.c_void_p, C.c_void_p]
+        L.qz_sweep_noalloc.argtypes = [C.c_void_p, C.c_void_p]
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
         L.qz_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.qz_batch_step.argtypes = [C.c_void_p, DP]

@@ -1726,6 +1726,15 @@ extern "C" int qz_sim_step_fields(qz_sim* h) {
     QZ_NEED(h, "qz_sim_step_fields");
     QZ_SIM_TRY(h->s->step_fields())
 }
+extern "C" int qz_sim_prepare_sweep(qz_sim* h, qz_sweep_args* out) {
+    QZ_NEED(h && out, "qz_sim_prepare_sweep");
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        s.prepare_sweep();
+        qz::hip_check(hipStreamSynchronize(s.stream), "sync");
+        *out = s.sweep_args;
+    })
+}
 extern "C" int qz_sim_push_beams(qz_sim* h) {
     QZ_NEED(h, "qz_sim_push_beams");
     QZ_SIM_TRY({

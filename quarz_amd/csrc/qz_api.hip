@@ -183,3 +183,71 @@ extern "C" int qz_sweep_release(void* stream) {
 extern "C" int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream) {
     return sweep_many(runs, nruns, stream);
 }
+
+// ---------------------------------------------------------------------------- the sweep without allocation
+// For callers that cannot allocate device memory (code under HIP graph capture, serving loops): one run's sweep
+// whose argument block and workspace live in this library's static device array `scratch`, so the call makes
+// no allocation at all.  Every GPU holds its own copy of the array for the life of the process; its address on
+// a device is looked up once, with that device current (hipGetSymbolAddress), and only that address is handed
+// to kernels and copies.  Calls on one GPU share the array: two calls whose sweeps overlap in time race, and
+// the workspace keeps what the last call left (a slice range continuing an earlier one reads its B+ from it).
+namespace {
+constexpr size_t kScratchArgBytes = 4096;                       // one qz_sweep_args block, rounded up
+constexpr int kScratchMaxM = QZ_NOALLOC_MAX_NODES;              // grid nodes N + 1
+constexpr size_t kScratchBytes = kScratchArgBytes + sizeof(double) * QZ_SWEEP_WS_ARRAYS * kScratchMaxM;
+static_assert(sizeof(qz_sweep_args) <= kScratchArgBytes, "the argument block must fit its scratch slot");
+}  // namespace
+
+__device__ __align__(256) char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, char*> g_scratch_ptr;   // device -> address of `scratch` on it
+
+int scratch_on_current_device(char** out) {
+    int dev = 0;
+    QZ_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return qz_fail_hip(e, "qz_sweep_noalloc: hipGetSymbolAddress(scratch)");
+        it = g_scratch_ptr.emplace(dev, static_cast<char*>(p)).first;
+    }
+    *out = it->second;
+    return QZ_OK;
+}
+}  // namespace
+
+namespace qz {
+// the argument block goes to device memory as a kernel argument: a graph that captures the call keeps the values
+// in its kernel node (a copy from host memory would read the caller's memory again at every replay)
+__global__ void put_args_kernel(qz_sweep_args a, qz_sweep_args* __restrict__ dst) {
+    constexpr int nw = static_cast<int>(sizeof(qz_sweep_args) / sizeof(unsigned int));
+    const unsigned int* src = reinterpret_cast<const unsigned int*>(&a);
+    unsigned int* out = reinterpret_cast<unsigned int*>(dst);
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) out[i] = src[i];
+}
+}  // namespace qz
+
+extern "C" int qz_sweep_noalloc(const qz_sweep_args* args, void* stream) {
+    if (!args) return qz_fail(QZ_EINVAL, "qz_sweep_noalloc: null argument");
+    const int M = args->grid.N + 1;
+    if (M > kScratchMaxM)
+        return qz_fail(QZ_ERANGE, "qz_sweep_noalloc: " + std::to_string(M) + " grid nodes exceed the scratch array's " +
+                                      std::to_string(kScratchMaxM));
+    char* base = nullptr;
+    const int rc = scratch_on_current_device(&base);
+    if (rc) return rc;
+    qz_sweep_args a = *args;
+    a.ws = reinterpret_cast<double*>(base + kScratchArgBytes);   // the run's workspace: the scratch after the block
+    const int v = qz_validate_sweep(a);
+    if (v) return v;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    qz_sweep_args* dev_args = reinterpret_cast<qz_sweep_args*>(base);
+    static_assert(sizeof(qz_sweep_args) % sizeof(unsigned int) == 0, "the block is copied in 4-byte words");
+    hipLaunchKernelGGL(qz::put_args_kernel, dim3(1), dim3(64), 0, st, a, dev_args);
+    QZ_HIP(hipGetLastError());
+    return qz_sweep_launch(dev_args, &a, 1, st);
+}

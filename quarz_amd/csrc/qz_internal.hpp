@@ -17,6 +17,9 @@ int qz_fail_hip(hipError_t e, const std::string& where);
         if (_e != hipSuccess) return qz_fail_hip(_e, #call);      \
     } while (0)
 
+// arrays of M doubles in a run's sweep workspace, and the largest grid (nodes) of the sweep that allocates nothing
+constexpr int QZ_SWEEP_WS_ARRAYS = 56;
+constexpr int QZ_NOALLOC_MAX_NODES = 2049;
 size_t qz_sweep_workspace_doubles_impl(int M);
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
 int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_OK or a QZ_E* code

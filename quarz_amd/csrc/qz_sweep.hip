@@ -972,6 +972,7 @@ long sweep_lds_needed(bool m1, int N, int pt, int L) {
 // ============================================================================ host launch
 using namespace qz;
 
+static_assert(W_NARR == QZ_SWEEP_WS_ARRAYS, "the workspace size the no-allocation sweep reserves");
 size_t qz_sweep_workspace_doubles_impl(int M) { return static_cast<size_t>(W_NARR) * M; }
 
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream) {

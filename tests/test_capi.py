@@ -120,3 +120,16 @@ def test_null_handles_are_errors():
     assert L.qz_sim_lost(None) == -1 and "null" in quarz_amd.last_error()
     L.qz_sim_time.restype = C.c_double
     assert math.isnan(L.qz_sim_time(None)) and "null" in quarz_amd.last_error()
+
+
+def test_noalloc_sweep_rejects_grids_beyond_its_scratch():
+    """qz_sweep_noalloc keeps the run's workspace in a static array of the library sized for 2049 nodes: a larger
+    grid is refused with QZ_ERANGE before any device call."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    args = (C.c_char * 4096)()
+    # qz_radial_grid leads the block: the r, inv_h and inv_V pointers, then N
+    C.c_int.from_buffer(args, 3 * C.sizeof(C.c_void_p)).value = 100000
+    assert L.qz_sweep_noalloc(args, None) == -2
+    assert "scratch" in quarz_amd.last_error()
