@@ -319,7 +319,10 @@ def main():
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c4": 50, "c5": 3}.get(args.config, 20)
+        args.steps = {"c3": 10, "c4": 50, "c5": 3}.get(args.config, 20)
+    if args.config == "c3":
+        args.depth = 0         # runs of several workgroups step one step at a time
+        args.warmup = min(args.warmup, 1)
     # The pipelined run wants one hardware queue per step in flight; HIP's default is 4 per process, read when
     # HIP starts (so set before torch is imported), never above 32.  The batched launches (c4, c5) keep the
     # default: with 32 queues their long persistent launch ran about 1.5x slower (c4: 412 against 276 ms).
@@ -347,6 +350,9 @@ def main():
     total = warmup + args.steps + 1
     sim = quarz_amd.Simulation(deck, {"time.steps": total})
     modes = sim.m1
+    group_times = None
+    if sim.groups > 1:   # workgroups per run for the sweep, timed on the first slices of the first step
+        group_times = sim.tune_groups([8, 16, 32], nslices=64)
     shapes = sim.autotune(cache=args.tune_cache)   # launch shapes of the beam kernels for this run
     # warm-up steps, one at a time (the first starts the leapfrog); the last one's HIP-event kernel times
     # give the per-launch figures (sweep, beam deposit, beam push)
@@ -426,9 +432,11 @@ def main():
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
         "launch_shapes": shapes,
+        "sweep_groups": sim.groups,
+        "sweep_group_times_ms": group_times,
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
-                     "kernel": "sweep_kernel",
+                     "kernel": "sweep_kernel" if sim.groups == 1 else f"sweep_multi_kernel ({sim.groups} workgroups)",
                      "launches_per_step": launches,
                      "avg_launch_ms": t_kernel * 1e3 / launches,
                      "flops_per_launch": flops_per_sweep / launches,
@@ -460,7 +468,8 @@ def main():
     out["beam_roofline"] = beam
     if not args.no_cpu:
         # the GPU box's share of the host is 16 cores per GPU
-        out["cpu_baseline"] = cpu_baseline(args.config, min(16, physical_cores()))
+        # a bounded sample for the long box: the first 200 of its 16001 slices
+        out["cpu_baseline"] = cpu_baseline(args.config, min(16, physical_cores()), 200 if args.config == "c3" else 0)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 

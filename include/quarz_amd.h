@@ -100,6 +100,11 @@ typedef struct qz_sweep_args {
     double* ws;                /* per-run workspace of qz_sweep_workspace_doubles(M) doubles */
     unsigned long long* stats; /* optional: stats[0] += live particles of every slice (particle-slices),
                                   stats[1..QZ_NPHASE] += wall-clock ticks of the sweep phases */
+    int groups;                /* workgroups of the run: 0 or 1 -- one workgroup (at most 4096 plasma particles and a
+                                  grid whose solve sets fit one compute unit); G > 1 -- G cooperating workgroups of at
+                                  most 4096 particles each, for larger runs (no phase clocks) */
+    double* mws;               /* G > 1: the run's workspace of qz_sweep_multi_workspace_doubles(M, np, G, modes)
+                                  doubles (per-slice lists, counts and the barrier of the workgroups) */
 } qz_sweep_args;
 
 const char* qz_last_error(void);
@@ -170,9 +175,14 @@ int qz_beam_sums_device(int np, const qz_beam_soa* p, double gref, double xiref,
  * argument blocks are staged in a device buffer kept per (device, stream), allocated on a stream's first
  * call (so that call is not capturable in a graph). */
 size_t qz_sweep_workspace_doubles(int M);
+/* Workspace of a run swept by `groups` cooperating workgroups (qz_sweep_args.groups > 1): M nodes, np mobile plasma
+ * particles (all species), modes 0 or 1.  The workgroups are resident together (a cooperative launch): a launch of
+ * more workgroups than the device holds at once is refused with QZ_ERANGE. */
+size_t qz_sweep_multi_workspace_doubles(int M, int np, int groups, int modes);
 int qz_sweep(const qz_sweep_args* args, void* stream);
 
-/* Many independent runs in one persistent launch, one workgroup per run (not in the reference). */
+/* Many independent runs in one persistent launch, one workgroup per run, or each its qz_sweep_args.groups
+ * cooperating workgroups when any run has groups > 1 (not in the reference). */
 int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream);
 /* No counterpart in the reference: qz_sweep for callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid allocation).  It allocates nothing: the argument block
@@ -245,6 +255,7 @@ int qz_sim_stats(const qz_sim* s, double* out7);
  * S+ node sums, Picard staging, Picard rhs, Picard solves, store, push */
 int qz_sim_phase_ticks(const qz_sim* s, double* out16);
 long qz_sim_lost(const qz_sim* s);
+int qz_sim_groups(const qz_sim* s);   /* workgroups per run of the sweep (1: the one-workgroup kernel) */
 double qz_sim_time(const qz_sim* s);
 int qz_sim_write_fields(qz_sim* s, const char* path);
 
@@ -272,6 +283,11 @@ int qz_get_block(int kernel);
  * (the push, or the whole deposit into a scratch source array), reps launches each after one warm-up launch;
  * ms_out[i] receives candidate i's mean time, and the fastest is set.  The run's own state is not changed. */
 int qz_sim_autotune(qz_sim* s, int kernel, const int* candidates, int ncand, int reps, double* ms_out);
+/* Autotuner of the sweep's workgroups per run: each candidate count (1: the one-workgroup kernel, when the run fits
+ * it) sweeps the first nslices slices of the coming step, timed with HIP events into ms_out[i] (-1: a count the
+ * run cannot use); the fastest is kept for the run.  Before the first step only.  Results do not depend on the
+ * count.  Not in the reference. */
+int qz_sim_tune_groups(qz_sim* s, const int* candidates, int ncand, int nslices, double* ms_out);
 /* "<gfx arch name>|<compute units>CU" of the current device, the key under which launch shapes are cached */
 int qz_device_tag(char* out, int n);
 

@@ -79,6 +79,7 @@ def lib():
         L.qz_sim_step.argtypes = [C.c_void_p]
         L.qz_sim_step_fields.argtypes = [C.c_void_p]
         L.qz_sim_prepare_sweep.argtypes = [C.c_void_p, C.c_void_p]
+        L.qz_sim_tune_groups.argtypes = [C.c_void_p, IP, C.c_int, C.c_int, DP]
         L.qz_sweep.argtypes = [C.c_void_p, C.c_void_p]
         L.qz_sweep_noalloc.argtypes = [C.c_void_p, C.c_void_p]
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
@@ -101,6 +102,7 @@ def lib():
         L.qz_sim_timing.argtypes = [C.c_void_p, DP]
         L.qz_sim_stats.argtypes = [C.c_void_p, DP]
         L.qz_sim_phase_ticks.argtypes = [C.c_void_p, DP]
+        L.qz_sim_groups.argtypes = [C.c_void_p]
         L.qz_sim_lost.restype = C.c_long
         L.qz_sim_lost.argtypes = [C.c_void_p]
         L.qz_sim_time.restype = C.c_double
@@ -231,6 +233,14 @@ class Simulation:
                 json.dump(table, f, indent=1)
         return chosen
 
+    def tune_groups(self, candidates, nslices=64):
+        """Workgroups per run for this run's sweep: each candidate count sweeps the first `nslices` slices of the
+        coming step (before the first step); the fastest is kept.  Returns {count: ms} (-1: not usable)."""
+        cand = (C.c_int * len(candidates))(*candidates)
+        out = np.zeros(len(candidates))
+        check(lib().qz_sim_tune_groups(self.h, cand, len(candidates), int(nslices), out.ctypes.data_as(DP)))
+        return {int(c): float(t) for c, t in zip(candidates, out)}
+
     def run_pipelined(self, steps, chunks=64, depth=8):
         """`steps` steps with up to `depth` in flight, each split into `chunks` slice ranges (bit-identical to
         calling step() `steps` times; the run must have taken its first step).  Returns (particle-slices,
@@ -247,6 +257,11 @@ class Simulation:
 
     def lost(self):
         return lib().qz_sim_lost(self.h)
+
+    @property
+    def groups(self):
+        """Workgroups of this run's sweep (1: the one-workgroup kernel)."""
+        return lib().qz_sim_groups(self.h)
 
     @property
     def t(self):

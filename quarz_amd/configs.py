@@ -5,6 +5,9 @@ c1  m = 0 blowout, uniform 256 cells to R = 8, ppc 4, xi in [-16, 0] step 0.01 (
 c2  m = 0 + 1, stretched 512 cells to R = 10 (5e-4 at the axis, 0.05 at the wall), 8 azimuths per ring,
     xi in [-20, 0] step 0.005 (4001 slices), offset driver (1e6 particles) and witness (5e5), Picard 3,
     dt 10, 20 steps.
+c3  m = 0 + 1, stretched 1024 cells to R = 12, electrons and mobile hydrogen ions (ppc 2 x 8 azimuths each,
+    32768 particles per slice), xi in [-80, 0] step 0.005 (16001 slices), driver 2e7 and witness 4e6 particles,
+    Picard 3, dt 25, 10 steps; swept by 8 cooperating workgroups.
 c4  m = 0, the c2 node list (512 stretched cells to R = 10), xi in [-19.99, 0] step 0.01 (2000 slices), the c1
     driver with 5e5 particles, and a flat-top witness over xi in [-10, -6] (2e5 particles, linear deposit)
     whose current follows a 16-value piecewise-linear profile, the parameters of the differentiable step.
@@ -151,6 +154,65 @@ witness.seed = 777
 witness.pwl_weights = {pwl}
 """
 
+# c3: the long box at scale.  The workload leaves open the driver's gamma, energy spread and emittance (the c1
+# values), its centre (xi0 = -8: the bunch, 3 sigma_xi either side, lies inside the box) and the witness's centre,
+# length, density, gamma and emittance (xi0 = -20, sigma_xi 0.2, n0 10, gamma 2000, emittance 0: the c2 witness
+# behind the first bucket).  Mobile hydrogen ions: no neutralizing background.
+_C3 = """
+xi.convention = z-t
+modes = 1
+solver.picard = 3
+grid.type = file
+grid.file = {grid_file}
+xi.min = {xi_min}
+xi.max = 0
+xi.step = 0.005
+time.dt = 25
+time.steps = {steps}
+pusher.beam = vay
+pusher.ab_order = 3
+plasma.species = electrons ions
+electrons.ppc = 2
+electrons.ntheta = 8
+ions.charge = 1
+ions.mass = 1836
+ions.mobile = 1
+ions.ppc = 2
+ions.ntheta = 8
+beams = driver witness
+driver.n0 = 1
+driver.sigma_r = 0.5
+driver.sigma_xi = 2.0
+driver.xi0 = {driver_xi0}
+driver.gamma = 2000
+driver.espread = 0.01
+driver.emit_n = 0.002
+driver.cut = 3
+driver.nparticles = {nd}
+witness.n0 = 10
+witness.sigma_r = 0.05
+witness.sigma_xi = 0.2
+witness.xi0 = {witness_xi0}
+witness.gamma = 2000
+witness.emit_n = 0
+witness.cut = 3
+witness.nparticles = {nw}
+"""
+
+
+def c3_deck(workdir=None, xi_min=-80, steps=10, nd=20000000, nw=4000000, driver_xi0=-8, witness_xi0=-20):
+    """Deck of the long-box workload (1024 stretched cells to R = 12, electrons and mobile ions, 32768 plasma
+    particles per slice); the box and the beams can be shortened for parity tests."""
+    workdir = workdir or tempfile.mkdtemp(prefix="quarz_amd_")
+    nodes, _ = stretched_nodes(1024, 12.0, 5e-4, 0.05)
+    path = os.path.join(workdir, "grid_c3.txt")
+    with open(path, "w") as f:
+        f.write("# 1024 cells, 5e-4 at the axis, 0.05 at the wall, R = 12\n")
+        f.writelines(f"{x:.17g}\n" for x in nodes)
+    return _C3.format(grid_file=path, xi_min=xi_min, steps=steps, nd=nd, nw=nw, driver_xi0=driver_xi0,
+                      witness_xi0=witness_xi0)
+
+
 C4_SEGMENTS = 16
 
 
@@ -238,6 +300,8 @@ DESCRIPTION = {
     "c1": "m0 blowout, uniform 256 cells R=8, ppc4, 1601 slices, driver 1e5, vay, dt20",
     "c2": "m0+1 stretched 512 cells (5e-4..0.05) R=10, 8 azimuths, 4001 slices, driver 1e6 + witness 5e5, "
           "picard 3, vay, dt10",
+    "c3": "m0+1 stretched 1024 cells (5e-4..0.05) R=12, electrons + mobile H+ (ppc 2 x 8 azimuths each), "
+          "16001 slices, driver 2e7 + witness 4e6, picard 3, vay, dt25",
     "c4": "m0 stretched 512 cells (5e-4..0.05) R=10, ppc4, 2000 slices, driver 5e5 + flat-top witness 2e5 "
           "with a 16-value piecewise-linear current profile, vay, dt20",
     "c5": "256 independent m0 runs, N in {128..1024} stretched (1e-3..0.05), K in [1000, 8000], "
@@ -257,6 +321,8 @@ def deck(name, workdir=None):
             f.write("# 512 cells, 5e-4 at the axis, 0.05 at the wall, R = 10\n")
             f.writelines(f"{x:.17g}\n" for x in nodes)
         return _C2.format(grid_file=path)
+    if name == "c3":
+        return c3_deck(workdir=workdir)
     if name == "c4":
         return c4_deck(workdir=workdir)
     raise KeyError(name)

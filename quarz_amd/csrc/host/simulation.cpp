@@ -61,15 +61,42 @@ template struct DevBuf<qz_sweep_args>;
 
 // Stage n sweep argument blocks in the run's own device buffer and launch them on `st`.  The buffer is reused
 // only by this run's later launches on the same stream; a larger batch first waits for the stream.
+// With several workgroups per run the block map follows the argument blocks in the same buffer, and every run's
+// barrier counter and error flag start at zero.
 static void launch_sweeps(DevBuf<qz_sweep_args>& buf, const qz_sweep_args* host, int n, hipStream_t st) {
-    for (int i = 0; i < n; ++i)
+    bool multi = false;
+    for (int i = 0; i < n; ++i) {
         if (qz_validate_sweep(host[i]) != QZ_OK) throw std::runtime_error(qz_last_error());
-    if (buf.n < static_cast<size_t>(n)) {
+        multi = multi || host[i].groups > 1;
+    }
+    std::vector<int> map = multi ? qz_sweep_block_map(host, n) : std::vector<int>();
+    const size_t map_blocks = (sizeof(int) * map.size() + sizeof(qz_sweep_args) - 1) / sizeof(qz_sweep_args);
+    if (buf.n < static_cast<size_t>(n) + map_blocks) {
         hip_check(hipStreamSynchronize(st), "sync");
-        buf.alloc(static_cast<size_t>(n));
+        buf.alloc(static_cast<size_t>(n) + map_blocks);
     }
     hip_check(hipMemcpyAsync(buf.p, host, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st), "sweep arguments");
-    qz_check(qz_sweep_launch(buf.p, host, n, st));
+    if (!multi) {
+        qz_check(qz_sweep_launch(buf.p, host, n, st));
+        return;
+    }
+    int* dmap = reinterpret_cast<int*>(buf.p + n);
+    hip_check(hipMemcpy(dmap, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice), "block map");
+    for (int i = 0; i < n; ++i) {
+        if (!host[i].mws) throw std::runtime_error("a multi-workgroup launch needs every run's workspace");
+        qz_check(qz_zero_async(host[i].mws, 64 * sizeof(double), st));
+    }
+    qz_check(qz_sweep_multi_launch(buf.p, dmap, host, n, st));
+}
+
+// a multi-workgroup sweep whose workgroups stopped waiting for each other (never in a correct launch) is an error
+static void check_multi_sweep(const DevBuf<Real>& mws, int groups, hipStream_t st) {
+    if (groups <= 1) return;
+    unsigned int flag = 0;
+    hip_check(hipMemcpyAsync(&flag, reinterpret_cast<const unsigned int*>(mws.p) + 2, sizeof(flag), hipMemcpyDeviceToHost,
+                             st), "sweep flag");
+    hip_check(hipStreamSynchronize(st), "sync");
+    if (flag) throw std::runtime_error("multi-workgroup sweep: a grid barrier timed out (workgroups not co-resident)");
 }
 
 // ---------------------------------------------------------------------------- RadialGrid (src/RadialGrid.cpp)
@@ -698,7 +725,7 @@ static bool known_key(const std::string& key) {
         "grid.type", "grid.rmax", "grid.dr", "grid.regions", "grid.max_ratio", "grid.file", "xi.max", "xi.min",
         "xi.step", "time.dt", "time.steps", "time.start", "time.first_step", "plasma.seed", "beams.xi_shape",
         "plasma.species", "plasma.neutralize", "plasma.rmax", "plasma.radial", "beams", "pusher.beam",
-        "pusher.ab_order", "pusher.delta_min", "pusher.max_qsa_factor", "output.dir", "output.every",
+        "pusher.ab_order", "pusher.delta_min", "pusher.max_qsa_factor", "output.dir", "output.every", "sweep.groups",
         "output.beam_every", "input.allow_unknown", "units.n0", "units.n0_cm3"};
     static const char* const suffix[] = {
         ".charge", ".mass", ".mobile", ".ppc", ".ntheta", ".uth", ".density", ".rigid", ".pusher", ".profile", ".n0",
@@ -791,6 +818,17 @@ Simulation::Simulation(const std::string& deck, const std::string& overrides) { 
     hip_check(hipMemsetAsync(diag.p, 0, sizeof(Real) * diag.n, stream), "memset");
     bg.alloc(6 * static_cast<size_t>(M));
     ws.alloc(qz_sweep_workspace_doubles(M));
+    // workgroups of the sweep: one when the run fits one (at most 4096 plasma particles and solve sets that fit
+    // one compute unit's LDS), else one per 4096 particles; sweep.groups (this library's key) forces a count
+    {
+        int np = 0;
+        for (const auto& s : species)
+            if (s->mobile) np += s->cap;
+        const int want = cfg.get_int("sweep.groups", 0);
+        if (want < 0) throw std::runtime_error("sweep.groups must be 0 (automatic) or a workgroup count");
+        groups = want > 0 ? want : (qz_sweep_fits_one_group(m1, grid.N, np) ? 1 : (np + 4095) / 4096);
+        if (groups > 1) mws.alloc(qz_sweep_multi_workspace_doubles(M, np, groups, m1 ? 1 : 0));
+    }
     lost.alloc(QZ_MAX_SPECIES);
     counters.alloc(24);
     int maxnp = 0;
@@ -901,6 +939,8 @@ void Simulation::prepare_sweep(bool timed) {
     A.lost = lost.p;
     A.ws = ws.p;
     A.stats = counters.p;
+    A.groups = groups;
+    A.mws = groups > 1 ? mws.p : nullptr;
     sweep_args = A;
 }
 
@@ -908,6 +948,7 @@ void Simulation::prepare_sweep(bool timed) {
 void Simulation::finish_sweep(float sweep_ms, bool timed) {
     const int nsp = sweep_args.nspecies;
     sweep_launches = 1;
+    check_multi_sweep(mws, groups, stream);
     std::vector<int> hl;
     lost.download(hl, stream);
     lost_last = 0;
@@ -1058,6 +1099,8 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no output "
                                  "directory, steps left)");
     if (step_n + nrun - 1 > nsteps) throw std::runtime_error("run_pipelined: more steps than time.steps leaves");
+    if (groups > 1) throw std::runtime_error("run_pipelined: runs of one workgroup only (this run has " +
+                                             std::to_string(groups) + ")");
     // ranges of at least two slices: a range's deposit then needs only the previous step's pushes up to the
     // next range (a particle in the deposit window has its gather slice below the range after next)
     nchunks = std::max(1, std::min(nchunks, K / 2));
@@ -1470,6 +1513,9 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
             if (!hs[i]) throw std::runtime_error("qz_sim_step_batched: null run");
             if (hs[i]->s->done) throw std::runtime_error("qz_sim_step_batched: a run has ended (time.steps reached)");
             if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_sim_step_batched: runs differ in modes");
+            if ((hs[i]->s->groups > 1) != (hs[0]->s->groups > 1))
+                throw std::runtime_error("qz_sim_step_batched: the runs of one step have one workgroup each, or all "
+                                         "several");
         }
         std::vector<qz_sweep_args> args(n);
         qz::Simulation& s0 = *hs[0]->s;
@@ -1561,6 +1607,7 @@ extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out
         for (int i = 0; i < n; ++i) {
             if (!hs[i]) throw std::runtime_error("qz_batch_create: null run");
             if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_batch_create: runs differ in modes");
+            if (hs[i]->s->groups > 1) throw std::runtime_error("qz_batch_create: runs of one workgroup only");
             b->sims.push_back(hs[i]);
         }
         b->use_graph = use_graph != 0;
@@ -1657,6 +1704,53 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
     })
 }
 extern "C" long qz_batch_replays(const qz_batch* b) { return b ? b->replays : -1; }
+
+// Autotuner of the sweep's workgroups per run (quarz_amd.h): every candidate count sweeps the first nslices
+// slices of the coming step; the fastest is kept.  Before the first step only: the samples overwrite the plasma
+// arrays and the first rows of fld and diag, which the step reloads and rewrites.
+extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nslices, double* ms_out) {
+    if (!h || !cand || ncand < 1 || nslices < 1 || !ms_out) return qz_fail(QZ_EINVAL, "qz_sim_tune_groups: bad arguments");
+    QZ_SIM_TRY({
+        qz::Simulation& s = *h->s;
+        if (s.swept) throw std::runtime_error("qz_sim_tune_groups: tune before the first step");
+        int np = 0;
+        for (const auto& sp : s.species)
+            if (sp->mobile) np += sp->cap;
+        struct Events {
+            hipEvent_t e[2] = {nullptr, nullptr};
+            ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+        } ev;
+        qz::hip_check(hipEventCreate(&ev.e[0]), "event");
+        qz::hip_check(hipEventCreate(&ev.e[1]), "event");
+        int best = -1;
+        for (int i = 0; i < ncand; ++i) {
+            ms_out[i] = -1;   // a count the run cannot use
+            const int G = cand[i];
+            if (G < 1 || (G == 1 && !qz_sweep_fits_one_group(s.m1, s.grid.N, np)) || np > G * 4096) continue;
+            s.groups = G;
+            s.mws.release();
+            if (G > 1) s.mws.alloc(qz_sweep_multi_workspace_doubles(s.M, np, G, s.m1 ? 1 : 0));
+            s.prepare_sweep(false);
+            qz_sweep_args a = s.sweep_args;
+            a.k_end = std::min(nslices, s.K);
+            a.stats = nullptr;
+            qz::hip_check(hipEventRecord(ev.e[0], s.stream), "event");
+            qz::launch_sweeps(s.d_args, &a, 1, s.stream);
+            qz::hip_check(hipEventRecord(ev.e[1], s.stream), "event");
+            qz::hip_check(hipEventSynchronize(ev.e[1]), "sync");
+            qz::check_multi_sweep(s.mws, s.groups, s.stream);
+            float ms = 0;
+            qz::hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "elapsed");
+            ms_out[i] = ms;
+            if (best < 0 || ms < ms_out[best]) best = i;
+        }
+        if (best < 0) throw std::runtime_error("qz_sim_tune_groups: no candidate fits this run");
+        s.groups = cand[best];
+        s.mws.release();
+        if (s.groups > 1) s.mws.alloc(qz_sweep_multi_workspace_doubles(s.M, np, s.groups, s.m1 ? 1 : 0));
+        qz::hip_check(hipStreamSynchronize(s.stream), "sync");
+    })
+}
 
 // Autotuner step for one beam kernel (quarz_amd.h): candidates timed on scratch copies of the largest beam
 extern "C" int qz_sim_autotune(qz_sim* h, int kernel, const int* cand, int ncand, int reps, double* ms_out) {
@@ -1848,6 +1942,10 @@ extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out) {
     if (!h || !out) return qz_fail(QZ_EINVAL, "qz_sim_phase_ticks: null argument");
     for (int i = 0; i < QZ_NPHASE; ++i) out[i] = h->s->phase_ticks[i];
     return QZ_OK;
+}
+extern "C" int qz_sim_groups(const qz_sim* h) {
+    if (!h) return qz_fail(QZ_EINVAL, "qz_sim_groups: null argument");
+    return h->s->groups;
 }
 extern "C" long qz_sim_lost(const qz_sim* h) {
     if (!h) {

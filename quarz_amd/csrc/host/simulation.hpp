@@ -182,6 +182,8 @@ public:
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     int sweep_launches = 1;      // sweep kernel launches of the last step (pipelined: its slice ranges)
+    int groups = 1;              // workgroups of the sweep (1: the one-workgroup kernel)
+    DevBuf<Real> mws;            // groups > 1: the multi-workgroup workspace
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
     double phase_ticks[QZ_NPHASE] = {};
     hipStream_t stream = nullptr;

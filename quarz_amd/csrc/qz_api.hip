@@ -124,6 +124,8 @@ int qz_validate_sweep(const qz_sweep_args& a) {
     if (a.ncf != (a.modes ? 18 : 6) || a.ncb != (a.modes ? 15 : 5) || a.ncd < (a.modes ? 9 : 3))
         return qz_fail(QZ_EINVAL, "sweep: component counts do not match modes");
     if (!a.ws || !a.fld || !a.diag || !a.bsrc || !a.bg || !a.lost) return qz_fail(QZ_EINVAL, "sweep: null array");
+    if (a.groups < 0) return qz_fail(QZ_EINVAL, "sweep: negative groups");
+    if (a.groups > 1 && !a.mws) return qz_fail(QZ_EINVAL, "sweep: a run of several workgroups needs its workspace mws");
     for (int s = 0; s < a.nspecies; ++s)
         if (a.sp[s].np < 0 || a.sp[s].np > a.sp[s].cap || !a.sp[s].hist) return qz_fail(QZ_EINVAL, "sweep: bad species");
     return QZ_OK;
@@ -137,6 +139,9 @@ namespace {
 struct ArgsBuf {
     qz_sweep_args* p = nullptr;
     int cap = 0;
+    int* map = nullptr;         // block -> (run, group) of a multi-workgroup launch
+    int mapcap = 0;
+    std::vector<int> hmap;      // host side of the map (alive while the copy is staged)
 };
 std::mutex g_args_mu;
 std::map<std::pair<int, hipStream_t>, ArgsBuf> g_args;
@@ -164,7 +169,29 @@ static int sweep_many(const qz_sweep_args* runs, int n, void* stream) {
         b.cap = n;
     }
     QZ_HIP(hipMemcpyAsync(b.p, runs, sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice, st));
-    return qz_sweep_launch(b.p, runs, n, st);
+    bool multi = false;
+    for (int i = 0; i < n; ++i) multi = multi || runs[i].groups > 1;
+    if (!multi) return qz_sweep_launch(b.p, runs, n, st);
+    // several workgroups per run: the block map, and every run's barrier counter and error flag at zero
+    b.hmap = qz_sweep_block_map(runs, n);
+    const int need = static_cast<int>(b.hmap.size());
+    if (b.mapcap < need) {
+        if (b.map) {
+            QZ_HIP(hipStreamSynchronize(st));
+            QZ_HIP(hipFree(b.map));
+            b.map = nullptr;
+            b.mapcap = 0;
+        }
+        QZ_HIP(hipMalloc(reinterpret_cast<void**>(&b.map), sizeof(int) * need));
+        b.mapcap = need;
+    }
+    QZ_HIP(hipMemcpyAsync(b.map, b.hmap.data(), sizeof(int) * need, hipMemcpyHostToDevice, st));
+    for (int i = 0; i < n; ++i) {
+        if (!runs[i].mws) return qz_fail(QZ_EINVAL, "sweep: in a multi-workgroup launch every run needs mws");
+        const int rc = qz_zero_async(runs[i].mws, 64 * sizeof(double), st);
+        if (rc) return rc;
+    }
+    return qz_sweep_multi_launch(b.p, b.map, runs, n, st);
 }
 
 extern "C" int qz_sweep(const qz_sweep_args* args, void* stream) { return sweep_many(args, 1, stream); }
@@ -177,6 +204,7 @@ extern "C" int qz_sweep_release(void* stream) {
     if (it == g_args.end()) return QZ_OK;
     QZ_HIP(hipStreamSynchronize(st));   // its kernels may still read the block
     if (it->second.p) QZ_HIP(hipFree(it->second.p));
+    if (it->second.map) QZ_HIP(hipFree(it->second.map));
     g_args.erase(it);
     return QZ_OK;
 }
@@ -240,6 +268,7 @@ extern "C" int qz_sweep_noalloc(const qz_sweep_args* args, void* stream) {
     char* base = nullptr;
     const int rc = scratch_on_current_device(&base);
     if (rc) return rc;
+    if (args->groups > 1) return qz_fail(QZ_EINVAL, "qz_sweep_noalloc: one workgroup per run (groups 0 or 1)");
     qz_sweep_args a = *args;
     a.ws = reinterpret_cast<double*>(base + kScratchArgBytes);   // the run's workspace: the scratch after the block
     const int v = qz_validate_sweep(a);

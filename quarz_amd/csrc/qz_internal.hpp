@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <string>
+#include <vector>
 
 #include "../../include/quarz_amd.h"
 
@@ -23,6 +24,13 @@ constexpr int QZ_NOALLOC_MAX_NODES = 2049;
 size_t qz_sweep_workspace_doubles_impl(int M);
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
 int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_OK or a QZ_E* code
+// the one-workgroup sweep holds a run of np plasma particles on N cells (m1: modes 0 and 1)
+bool qz_sweep_fits_one_group(bool m1, int N, int np);
+// G cooperating workgroups per run (qz_sweep_args.groups > 1); dev_map: block -> (run, group) pairs
+int qz_sweep_multi_launch(const qz_sweep_args* dev_runs, const int* dev_map, const qz_sweep_args* host_runs, int nruns,
+                          hipStream_t stream);
+// the block map of a multi-workgroup launch of these runs (2 ints per block)
+std::vector<int> qz_sweep_block_map(const qz_sweep_args* runs, int nruns);
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);
 // zero `bytes` (a multiple of 4) at p with a kernel: unlike a memset node it is safe inside a captured graph
 int qz_zero_async(void* p, size_t bytes, hipStream_t st);

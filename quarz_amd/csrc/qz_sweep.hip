@@ -975,6 +975,12 @@ using namespace qz;
 static_assert(W_NARR == QZ_SWEEP_WS_ARRAYS, "the workspace size the no-allocation sweep reserves");
 size_t qz_sweep_workspace_doubles_impl(int M) { return static_cast<size_t>(W_NARR) * M; }
 
+bool qz_sweep_fits_one_group(bool m1, int N, int np) {
+    if (np > kBS * 8) return false;
+    const int pt = np <= 2 * kBS ? 2 : (np <= 4 * kBS ? 4 : 8);
+    return sweep_lds_needed(m1, N, pt, np) <= kLdsBytes;
+}
+
 int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream) {
     int maxnp = 0, maxN = 0;
     const bool m1 = host_runs[0].modes == 1;

@@ -1,0 +1,92 @@
+"""GPU parity of the sweep run by several cooperating workgroups (qz_sweep_args.groups > 1): bit-identical to the
+serial oracle, on small decks with a forced workgroup count (uneven particle shares, one and two mobile species,
+m = 0 and m = 1) and at the long-box workload's shape (1024 stretched cells, electrons and mobile ions, 32768
+plasma particles per slice, 8 workgroups) in a short box."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def qa():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import quarz_amd
+    quarz_amd.lib()
+    return quarz_amd
+
+
+def _same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a, b)
+
+
+def _report(name, a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    bad = np.nonzero(a != b)
+    n = len(bad[0])
+    first = tuple(int(x[0]) for x in bad) if n else None
+    return f"{name}: {n} of {a.size} differ; first at {first}: {a[first] if n else ''} vs {b[first] if n else ''}"
+
+
+def _compare(qa, g, o, steps):
+    for _ in range(steps):
+        g.step()
+        o.step()
+    for w in ("fld", "diag", "bsrc"):
+        a, b = getattr(g, w), getattr(o, w)
+        assert _same(a, b), _report(w, a, b)
+    for bi in range(o.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            a, b = g.get(f"beam{bi}.{k}"), o.get(f"beam{bi}.{k}")
+            assert _same(a, b), _report(f"beam{bi} {k}", a, b)
+    for sp in range(o.nspecies):
+        for k in ("x", "y", "px", "py", "dl", "w"):
+            a, b = g.get(f"sp{sp}.{k}"), o.get(f"sp{sp}.{k}")
+            assert _same(a, b), _report(f"species {sp} {k}", a, b)
+    assert g.lost() == o.lost()
+
+
+M1_DECK = {"modes": 1, "xi.min": -5, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05, "driver.nparticles": 20000,
+           "driver.sigma_r": 0.5, "driver.rigid": 0, "electrons.ppc": 1, "time.steps": 2, "time.dt": 5}
+ION_DECK = {"xi.min": -7.3, "time.steps": 2, "driver.nparticles": 40000, "witness.nparticles": 20000}
+
+
+@pytest.mark.parametrize("case,groups", [("m1", 2), ("m1", 3), ("ions", 3), ("ions", 4)])
+def test_forced_groups_equal_oracle(qa, oracle, decks, case, groups):
+    """m = 1 with 1600 mobile electrons on 2 or 3 workgroups (shares of 1024, 576 / 1024, 576, 0), and the m = 0
+    ion-motion shape (3936 electrons and ions) on 3 or 4 workgroups: three steps each, bit for bit."""
+    deck, ov = (f"{decks}/linear.in", M1_DECK) if case == "m1" else (f"{decks}/pwfa_ion_motion.in", ION_DECK)
+    g = qa.Simulation(deck, dict(ov, **{"sweep.groups": groups}))
+    o = oracle.Sim(deck, ov)
+    _compare(qa, g, o, 3)
+
+
+def test_c3_shape_bit_exact(qa, oracle, tmp_path):
+    """The long-box workload's plasma and grid (m = 0 + 1, 1024 stretched cells to R = 12, electrons and mobile
+    ions with 16384 particles per slice each, Picard 3) on 8 workgroups, in a box of 301 slices with a short
+    offset driver and a witness: three steps, bit for bit."""
+    from quarz_amd import configs
+    deck = configs.c3_deck(workdir=str(tmp_path), xi_min=-1.5, steps=2, nd=100000, nw=20000, driver_xi0=-0.6,
+                           witness_xi0=-1.2)
+    ov = {"driver.sigma_xi": 0.2, "driver.x0": 0.05, "driver.n0": 2, "witness.sigma_xi": 0.05}
+    g = qa.Simulation(deck, ov)
+    o = oracle.Sim(deck, ov)
+    assert g.M == 1025 and g.K == 301
+    _compare(qa, g, o, 3)
+
+
+def test_batched_multi_runs_equal_plain(qa, decks):
+    """Two runs of several workgroups each in one batched launch: each equals its own plain launch."""
+    ovs = [dict(M1_DECK, **{"sweep.groups": 2}), dict(M1_DECK, **{"sweep.groups": 3, "xi.min": -4, "grid.dr": 0.05})]
+    plain = [qa.Simulation(f"{decks}/linear.in", ov) for ov in ovs]
+    batch = [qa.Simulation(f"{decks}/linear.in", ov) for ov in ovs]
+    for _ in range(3):
+        for s in plain:
+            s.step()
+        qa.step_batched(batch)
+    for i, (a, b) in enumerate(zip(batch, plain)):
+        for w in ("fld", "diag", "bsrc"):
+            assert _same(getattr(a, w), getattr(b, w)), (i, w)
