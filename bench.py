@@ -348,7 +348,10 @@ def main():
     warmup = max(args.warmup, 1) if pipelined else args.warmup   # the pipeline continues a started run
     depth = min(args.steps, 32) if args.depth < 0 else args.depth
     total = warmup + args.steps + 1
+    t_setup = time.perf_counter()
     sim = quarz_amd.Simulation(deck, {"time.steps": total})
+    print(f"# {args.config} set up in {time.perf_counter() - t_setup:.1f} s ({sim.groups} workgroups per run)",
+          file=sys.stderr, flush=True)
     modes = sim.m1
     group_times = None
     if sim.groups > 1:   # workgroups per run for the sweep, timed on the first slices of the first step
@@ -357,9 +360,10 @@ def main():
     # warm-up steps, one at a time (the first starts the leapfrog); the last one's HIP-event kernel times
     # give the per-launch figures (sweep, beam deposit, beam push)
     eager = None
-    for _ in range(warmup):
+    for i in range(warmup):
         sim.step()
         eager = sim.stats()
+        print(f"# warm-up step {i + 1}/{warmup}: sweep {eager['sweep_ms']:.0f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     psl = pushes = dep_particles = 0.0
     sweep_ms = beam_ms = dep_ms = 0.0
@@ -372,9 +376,11 @@ def main():
         # kernel time of the timed steps: every range launch bracketed by HIP events on its own stream
         timed_sweep_ms, launches = st["sweep_ms"], int(st["sweep_launches"])
     else:
-        for _ in range(args.steps):
+        for i in range(args.steps):
             sim.step()
             st = sim.stats()
+            print(f"# step {i + 1}/{args.steps}: sweep {st['sweep_ms']:.0f} ms ({time.perf_counter() - t0:.1f} s)",
+                  file=sys.stderr, flush=True)
             psl += st["particle_slices"]
             pushes += st["beam_pushed"]
             dep_particles += st["beam_deposited"]
