@@ -64,6 +64,16 @@ typedef struct qz_species_state {
 } qz_species_state;
 
 #define QZ_MAX_SPECIES 2
+
+/* Cold plasma rings loaded at the head of the box by the sweep itself (PlasmaSpecies::load,
+ * src/PlasmaSpecies.cpp:202-295): ring r (r = ppc j + sub) is the sub-annulus sub of cell j at its volume
+ * centroid, with weight density pi (rb^2 - ra^2) / ntheta per particle; particle l of ring r sits at angle
+ * 2 pi (l + (r % 2) / 2) / ntheta.  The first nring rings are loaded (those beyond plasma.rmax carry no weight). */
+typedef struct qz_plasma_rings {
+    int ppc, ntheta, nring;
+    double density;
+    const double* cs;     /* device: cos then sin of the 2 ntheta angles, turn 0 first (4 ntheta doubles) */
+} qz_plasma_rings;
 #define QZ_NPHASE 16      /* sweep phase clocks: see qz_sim_phase_ticks */
 
 /* Everything one quasi-static sweep reads and writes (Simulation::step_fields, src/Simulation.cpp:636-795). */
@@ -105,6 +115,9 @@ typedef struct qz_sweep_args {
                                   most 4096 particles each, for larger runs (no phase clocks) */
     double* mws;               /* G > 1: the run's workspace of qz_sweep_multi_workspace_doubles(M, np, G, modes)
                                   doubles (per-slice lists, counts and the barrier of the workgroups) */
+    int load;                  /* 1: a launch that starts at slice 0 first loads every mobile species' cold rings
+                                  (rings[s]) into its arrays and clears its AB history; 0: the arrays hold the load */
+    qz_plasma_rings rings[QZ_MAX_SPECIES];
 } qz_sweep_args;
 
 const char* qz_last_error(void);

@@ -378,6 +378,7 @@ void PlasmaSpecies::init(const Config& cfg, const std::string& nm, const RadialG
 // that order.  Particles of zero weight are not loaded.
 void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st) {
     (void)z;   // the longitudinal profile is flat on this path
+    if (device_rings()) return;   // the sweep loads the rings and clears the history itself
     if (have_template) {   // the rings of the first load (device copies: capturable, no host work)
         const size_t bytes = sizeof(Real) * static_cast<size_t>(Np);
         const DevBuf<Real>* from[6] = {&t_x, &t_y, &t_px, &t_py, &t_dl, &t_w};
@@ -426,6 +427,16 @@ void PlasmaSpecies::load(const RadialGrid& g, Real z, unsigned long long seed, h
     }
     have_template = true;
     qz_check(qz_zero_async(hist.p, sizeof(Real) * hist.n, st));
+    // whole rings are loaded or skipped, and the skipped ones lie beyond the loaded ones (radius grows with ring)
+    nring = Np / ntheta;
+    std::vector<Real> cs(4 * static_cast<size_t>(ntheta));
+    for (int t = 0; t < 2; ++t)
+        for (int l = 0; l < ntheta; ++l) {
+            const double th = 2 * PI * (l + t * 0.5) / ntheta;
+            cs[t * ntheta + l] = std::cos(th);
+            cs[2 * ntheta + t * ntheta + l] = std::sin(th);
+        }
+    d_cs.upload(cs, st);
 }
 
 // host deposit of the loaded rings (background / neutralization), serial index order
@@ -941,6 +952,14 @@ void Simulation::prepare_sweep(bool timed) {
     A.stats = counters.p;
     A.groups = groups;
     A.mws = groups > 1 ? mws.p : nullptr;
+    // cold plasma: the sweep loads the rings at the head itself
+    A.load = nsp > 0 ? 1 : 0;
+    nsp = 0;
+    for (auto& s : species) {
+        if (!s->mobile) continue;
+        if (!s->device_rings()) A.load = 0;
+        A.rings[nsp++] = {s->ppc, s->ntheta, s->nring, s->density, s->d_cs.p};
+    }
     sweep_args = A;
 }
 
@@ -1201,8 +1220,9 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         const bool last = n >= nsteps;   // the last sweep has no beam push (src/Simulation.cpp:1430-1446)
         const int marker = n + 1;
         hipStream_t st = C.st;
-        // plasma at the head (the rings of the first load) with a fresh AB history
-        for (size_t s = 0; s < mob.size(); ++s) {
+        // plasma at the head (the rings of the first load) with a fresh AB history; a cold plasma is loaded by
+        // the first range's sweep itself
+        for (size_t s = 0; s < mob.size() && !sweep_args.load; ++s) {
             const size_t bytes = sizeof(Real) * static_cast<size_t>(mob[s]->Np);
             const DevBuf<Real>* src[6] = {&mob[s]->t_x, &mob[s]->t_y, &mob[s]->t_px, &mob[s]->t_py, &mob[s]->t_dl,
                                           &mob[s]->t_w};

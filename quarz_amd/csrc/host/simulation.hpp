@@ -83,6 +83,11 @@ struct PlasmaSpecies {
     // kept in HBM and later loads are device copies of it.
     DevBuf<Real> t_x, t_y, t_px, t_py, t_dl, t_w;
     bool have_template = false;
+    // A cold plasma (uth = 0) is loaded by the sweep kernel itself at the head of the box (qz_plasma_rings):
+    // the rings loaded and the cos / sin table of their angles.
+    int nring = 0;
+    DevBuf<Real> d_cs;
+    bool device_rings() const { return have_template && uth == 0; }
 
     void init(const Config& cfg, const std::string& nm, const RadialGrid& g, bool mode1);
     void load(const RadialGrid& g, Real z, unsigned long long seed, hipStream_t st);

@@ -125,6 +125,13 @@ int qz_validate_sweep(const qz_sweep_args& a) {
         return qz_fail(QZ_EINVAL, "sweep: component counts do not match modes");
     if (!a.ws || !a.fld || !a.diag || !a.bsrc || !a.bg || !a.lost) return qz_fail(QZ_EINVAL, "sweep: null array");
     if (a.groups < 0) return qz_fail(QZ_EINVAL, "sweep: negative groups");
+    if (a.load)
+        for (int s = 0; s < a.nspecies; ++s) {
+            const qz_plasma_rings& rg = a.rings[s];
+            if (rg.ppc < 1 || rg.ntheta < 1 || rg.nring < 0 || rg.nring > a.grid.N * rg.ppc || !rg.cs ||
+                rg.nring * rg.ntheta != a.sp[s].np)
+                return qz_fail(QZ_EINVAL, "sweep: the ring load of a species does not match its particle count");
+        }
     if (a.groups > 1 && !a.mws) return qz_fail(QZ_EINVAL, "sweep: a run of several workgroups needs its workspace mws");
     for (int s = 0; s < a.nspecies; ++s)
         if (a.sp[s].np < 0 || a.sp[s].np > a.sp[s].cap || !a.sp[s].hist) return qz_fail(QZ_EINVAL, "sweep: bad species");

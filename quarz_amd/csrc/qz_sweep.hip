@@ -167,6 +167,10 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
         }                                                    \
     } while (0)
     __syncthreads();
+    if (kb == 0 && A.load) {   // the plasma at the head of the box, made here instead of copied in
+        for (int s = 0; s < nsp; ++s) load_rings(A.rings[s], A.sp[s], ord, s_r, tid, A.sp[s].np, kBS);
+        __syncthreads();
+    }
 
     // per-thread particle cache of the slice (pp is a compile-time index in every loop): cell, node-list
     // positions, and the geometry that costs divisions; momenta and weights are re-read where needed

@@ -9,6 +9,7 @@
 #include <type_traits>
 
 #include "qz_device.hpp"
+#include "../../include/quarz_amd.h"
 
 namespace qz {
 
@@ -334,5 +335,29 @@ struct Stencil {
     }
     __device__ __forceinline__ Real over_r(const Real* f, int j) const { return (j == 0 ? f[1] : f[j]) * ir[j]; }
 };
+
+// The cold rings of one species at the head of the box (PlasmaSpecies::load, src/PlasmaSpecies.cpp:202-295, with
+// the host load's expression order): particles [i0, i1) of `st`, node radii r (LDS), and a fresh AB history.
+__device__ __forceinline__ void load_rings(const qz_plasma_rings& rg, const qz_species_state& st, int ord,
+                                           const Real* r, int i0, int i1, int stride) {
+    constexpr Real PI = 3.14159265358979323846;
+    const int nt = rg.ntheta, ppc = rg.ppc;
+    for (int i = i0; i < i1; i += stride) {
+        const int ring = i / nt, l = i - ring * nt;
+        const int j = ring / ppc, sub = ring - j * ppc;
+        const Real hc = r[j + 1] - r[j];
+        const Real ra = r[j] + Real(sub) * hc / Real(ppc), rb = r[j] + Real(sub + 1) * hc / Real(ppc);
+        const Real rc = (Real(2.0) / Real(3.0)) * (rb * rb * rb - ra * ra * ra) / (rb * rb - ra * ra);
+        const Real volume = (Real(1.0) * rg.density) * PI * (rb * rb - ra * ra);
+        const int a = (ring % 2) * nt + l;
+        st.x[i] = nt == 1 ? rc : rc * rg.cs[a];
+        st.y[i] = nt == 1 ? Real(0) : rc * rg.cs[2 * nt + a];
+        st.px[i] = Real(0);
+        st.py[i] = Real(0);
+        st.dl[i] = Real(1);   // gamma - p_z of a particle at rest
+        st.w[i] = Real(1.0) * volume / Real(nt);
+        for (int v = 0; v < 5 * ord; ++v) st.hist[static_cast<size_t>(v) * st.cap + i] = Real(0);
+    }
+}
 
 }  // namespace qz

@@ -208,6 +208,12 @@ __global__ __launch_bounds__(mBS) void sweep_multi_kernel(const qz_sweep_args* _
     const int ord = A.ab_order;
     const int niter = M1 ? A.picard : 1;
     __syncthreads();
+    if (kb == 0 && A.load) {   // this workgroup's particles of the plasma at the head of the box
+        const int own1 = own0 + mBS * PT;
+        if (nsp > 0) load_rings(A.rings[0], A.sp[0], ord, s_r, own0 + tid, min(own1, np0), mBS);
+        if (nsp > 1) load_rings(A.rings[1], A.sp[1], ord, s_r, max(own0, np0) - np0 + tid, min(own1 - np0, np1), mBS);
+        __syncthreads();
+    }
 
     int cj[PT], lrank[PT], cpl[PT], cpr[PT];
     Real ct[PT], cc[PT], cs[PT], cir[PT], cinvd[PT], cgam[PT];

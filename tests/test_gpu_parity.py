@@ -362,6 +362,22 @@ def test_c1_workload_bit_exact(qa, oracle):
         assert abs(da[k] - di[k]) <= 1e-12 * abs(di[k]), (k, da[k], di[k])
 
 
+def test_thermal_plasma_bit_exact(qa, oracle, decks):
+    """A warm plasma (electrons.uth > 0: momenta drawn from std::normal_distribution at the first load, the same
+    every step) keeps the host load and its device copies; a cold one is loaded by the sweep kernel itself (every
+    other test).  Three steps bit for bit."""
+    ov = {"time.steps": 2, "driver.nparticles": 20000, "xi.min": -6, "electrons.uth": 0.01}
+    g = qa.Simulation(f"{decks}/gain.in", ov)
+    o = oracle.Sim(f"{decks}/gain.in", ov)
+    for _ in range(3):
+        g.step()
+        o.step()
+    _compare_sims(g, o)
+    for k in ("x", "y", "px", "py", "dl", "w"):
+        a, b = g.get(f"sp0.{k}"), o.get(f"sp0.{k}")
+        assert _same(a, b), _report(f"electrons {k}", a, b)
+
+
 @pytest.mark.parametrize("m1", [False, True])
 def test_two_mobile_species_bit_exact(qa, oracle, decks, m1):
     """Electrons and mobile hydrogen ions (two species in one sweep: the positive species' deposit into n_i, the
