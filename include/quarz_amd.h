@@ -206,6 +206,23 @@ int qz_sweep_batched(const qz_sweep_args* runs, int nruns, void* stream);
  * kernel, so a captured call keeps its arguments in the graph. */
 int qz_sweep_noalloc(const qz_sweep_args* args, void* stream);
 
+/* sizeof(qz_sweep_args), for bindings that mirror the struct */
+size_t qz_sweep_args_bytes(void);
+
+/* The state one slice block hands to the next (Simulation::make_message, src/Simulation.cpp:851-870;
+ * PlasmaSpecies::pack_state / unpack_state, src/PlasmaSpecies.cpp:317-364), packed in device memory: the header
+ * [step][dt][last], one zero particle count per beam, per mobile species [Np][x][y][px][py][Delta][w][fresh]
+ * [AB history slabs (order, 5)][pushes][xi][xi of each history slot], B+ (r0 i0 r1 i1 r2 i2, M each) and the
+ * beam sources of the block's last slice (M x 15, component fastest).  Device copies and small kernels only: the
+ * message can go to a peer GPU or a checkpoint file without a host round trip.  pack: the run's state after a
+ * launch that ended at slice k_next (one workgroup per run).  unpack: into another run's arrays (its species
+ * counts in *args updated, args->load cleared); a launch from k_begin = k_next then continues the sweep bit for
+ * bit; guard (optional, device, M x 15) receives the beam-source row. */
+size_t qz_message_doubles(const qz_sweep_args* args, int nbeams);
+int qz_pack_message(const qz_sweep_args* args, int nbeams, int step, double dt, int last, int k_next, double* msg,
+                    void* stream);
+int qz_unpack_message(qz_sweep_args* args, int nbeams, const double* msg, double* guard, void* stream);
+
 /* Frees the argument staging buffer qz_sweep / qz_sweep_batched keep for `stream` on the current device, after
  * the stream's work is done: call it before destroying a stream the sweeps were launched on (a new stream can
  * reuse the handle).  No counterpart in the reference. */

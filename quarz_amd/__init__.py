@@ -28,6 +28,13 @@ class RadialGridC(C.Structure):
     _fields_ = [("r", C.c_void_p), ("inv_h", C.c_void_p), ("inv_V", C.c_void_p), ("N", C.c_int), ("R", C.c_double)]
 
 
+class SweepArgsHeadC(C.Structure):
+    """The leading fields of qz_sweep_args (include/quarz_amd.h): enough to set a launch's slice range on a block
+    that qz_sim_prepare_sweep filled."""
+    _fields_ = [("grid", RadialGridC), ("modes", C.c_int), ("picard", C.c_int), ("nslices", C.c_int),
+                ("k_begin", C.c_int), ("k_end", C.c_int)]
+
+
 class BeamGridC(C.Structure):
     _fields_ = [("xi_min", C.c_double), ("dxi", C.c_double), ("nxi", C.c_int), ("k0", C.c_int), ("nloc", C.c_int),
                 ("ngp", C.c_int)]
@@ -82,6 +89,13 @@ def lib():
         L.qz_sim_tune_groups.argtypes = [C.c_void_p, IP, C.c_int, C.c_int, DP]
         L.qz_sweep.argtypes = [C.c_void_p, C.c_void_p]
         L.qz_sweep_noalloc.argtypes = [C.c_void_p, C.c_void_p]
+        L.qz_sweep_args_bytes.restype = C.c_size_t
+        L.qz_sweep_args_bytes.argtypes = []
+        L.qz_message_doubles.restype = C.c_size_t
+        L.qz_message_doubles.argtypes = [C.c_void_p, C.c_int]
+        L.qz_pack_message.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_void_p,
+                                      C.c_void_p]
+        L.qz_unpack_message.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
         L.qz_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.qz_batch_step.argtypes = [C.c_void_p, DP]

@@ -101,6 +101,7 @@ extern "C" int qz_tridiag_thomas_batched(const double* a, const double* b, const
 }
 
 extern "C" size_t qz_sweep_workspace_doubles(int M) { return qz_sweep_workspace_doubles_impl(M); }
+extern "C" size_t qz_sweep_args_bytes(void) { return sizeof(qz_sweep_args); }
 
 extern "C" int qz_device_tag(char* out, int n) {
     if (!out || n < 1) return qz_fail(QZ_EINVAL, "qz_device_tag: no buffer");

@@ -133,3 +133,18 @@ def test_noalloc_sweep_rejects_grids_beyond_its_scratch():
     C.c_int.from_buffer(args, 3 * C.sizeof(C.c_void_p)).value = 100000
     assert L.qz_sweep_noalloc(args, None) == -2
     assert "scratch" in quarz_amd.last_error()
+
+
+def test_message_entry_points_reject_bad_arguments():
+    """qz_pack_message / qz_unpack_message check their arguments on the host before any device call."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    assert L.qz_message_doubles(None, 1) == 0
+    assert L.qz_pack_message(None, 1, 0, 0.0, 0, 1, None, None) == -1 and "null" in quarz_amd.last_error()
+    assert L.qz_unpack_message(None, 1, None, None, None) == -1 and "null" in quarz_amd.last_error()
+    args = (C.c_char * 4096)()   # a zeroed block: no workspace
+    dummy = C.c_void_p(8)
+    assert L.qz_pack_message(args, 1, 0, 0.0, 0, 0, dummy, None) == -1 and "ws" in quarz_amd.last_error()
+    assert L.qz_unpack_message(args, 1, dummy, None, None) == -1 and "ws" in quarz_amd.last_error()
+    assert 0 < C.sizeof(quarz_amd.SweepArgsHeadC) < L.qz_sweep_args_bytes() <= 4096
