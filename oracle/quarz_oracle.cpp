@@ -1293,6 +1293,11 @@ struct Beam {
     double gref = 0, xiref = 0;
     int Np = 0;
     std::vector<Real> x, y, px, py, pz, xi, w;
+    // the piecewise-linear weight profile (empty without <beam>.pwl_weights): values, flat-top ends in user xi,
+    // the macro weight at profile value 1, and every particle's xi as sampled (the adjoint's weight derivative)
+    std::vector<double> pwl;
+    double pwl_u0 = 0, pwl_u1 = 0, pwl_wmac = 0;
+    std::vector<double> xi_init;
 
     // Beam ctor for profile = gaussian (src/Beam.cpp:53-207) and init_gaussian (src/Beam.cpp:483-570)
     void init(const Config& cfg, const std::string& nm, bool mode1) {
@@ -1424,6 +1429,9 @@ struct Beam {
         set_particles(local);
         gref = 0; xiref = 0;
         if (aw > 0) { gref = awg / aw; xiref = awxi / aw; }
+        this->pwl = pwl;
+        pwl_u0 = pu0; pwl_u1 = pu1; pwl_wmac = wmac;
+        xi_init = xi;
     }
     void set_particles(const std::vector<double>& p) {
         Np = static_cast<int>(p.size() / 7);
@@ -1885,10 +1893,21 @@ struct Sim {
     }
 
     // src/Simulation.cpp:636-795
+    // test hooks of the adjoint checks: added to the beam sources after the deposit and to the fields after
+    // the sweep (empty: no change)
+    std::vector<Real> bsrc_delta, fld_delta;
+    std::vector<double> adj_lf, adj_lb, adj_lw;   // the adjoints of the last or_sim_spread_grad
+    void add_delta(std::vector<Real>& a, const std::vector<Real>& d) const {
+        if (!d.empty())
+            for (size_t i = 0; i < a.size(); ++i) a[i] += d[i];
+    }
+
     template <bool PAR> void step_fields() {
         deposit_beams<PAR>();
+        add_delta(bsrc, bsrc_delta);
         load_plasma<PAR>();
         sweep_slices<PAR>();
+        add_delta(fld, fld_delta);
     }
 
     // one iteration of Simulation::run (src/Simulation.cpp:1376-1459) without output writing
@@ -2038,6 +2057,8 @@ using namespace orc;
 #define OR_TRY(...)                                    \
     try { __VA_ARGS__; return 0; }                     \
     catch (const std::exception& e) { last_error = e.what(); return -1; }
+
+#include "adjoint_oracle.inc"
 
 extern "C" {
 
@@ -2296,6 +2317,32 @@ int or_sim_beam_diag(void* h, int b, int tree, double* out15) {
                           d.emit_nx, d.x_mean, d.y_mean, d.emit_ny, d.sigma_x, d.sigma_y, double(d.alive)};
     std::copy(v, v + 15, out15);
     return 0;
+}
+// J = gamma_rms / gamma_mean of beam b after one step, and dJ/dv of its weight profile (nseg values), by the
+// reverse sweep of adjoint_oracle.inc; advances the run by that step
+int or_sim_spread_grad(void* h, int b, double* J, double* grad, int nseg) {
+    Sim& s = *static_cast<Sim*>(h);
+    OR_TRY({
+        if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
+        if (nseg != static_cast<int>(s.beams[b].pwl.size())) throw std::runtime_error("profile size");
+        adjoint::spread_grad(s, b, J, grad);
+    })
+}
+// sets a test hook (0: beam sources, 1: fields) to n doubles (n = 0 clears it)
+int or_sim_set_delta(void* h, int which, const double* d, long n) {
+    Sim& s = *static_cast<Sim*>(h);
+    std::vector<Real>& v = which == 0 ? s.bsrc_delta : s.fld_delta;
+    const size_t want = which == 0 ? s.bsrc.size() : s.fld.size();
+    if (n != 0 && static_cast<size_t>(n) != want) { last_error = "or_sim_set_delta: size"; return -1; }
+    v.assign(d, d + n);
+    return 0;
+}
+// after or_sim_spread_grad: the adjoints it formed (0: fields, 1: beam sources, 2: the beam's weights)
+long or_sim_adjoint_get(void* h, int which, double* out, long cap) {
+    Sim& s = *static_cast<Sim*>(h);
+    const std::vector<double>& v = which == 0 ? s.adj_lf : which == 1 ? s.adj_lb : s.adj_lw;
+    if (out && cap >= static_cast<long>(v.size())) std::copy(v.begin(), v.end(), out);
+    return static_cast<long>(v.size());
 }
 int or_sim_write_fields(void* h, const char* path) { OR_TRY(write_fields(*static_cast<Sim*>(h), path)) }
 int or_sim_write_axis(void* h, const char* path) { OR_TRY(write_axis(*static_cast<Sim*>(h), path)) }

@@ -78,6 +78,10 @@ def lib():
         L.or_sim_beam_sums.argtypes = [C.c_void_p, C.c_int, DP, DP]
         L.or_sim_beam_diag.argtypes = [C.c_void_p, C.c_int, C.c_int, DP]
         L.or_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
+        L.or_sim_spread_grad.argtypes = [C.c_void_p, C.c_int, DP, DP, C.c_int]
+        L.or_sim_set_delta.argtypes = [C.c_void_p, C.c_int, DP, C.c_long]
+        L.or_sim_adjoint_get.restype = C.c_long
+        L.or_sim_adjoint_get.argtypes = [C.c_void_p, C.c_int, DP, C.c_long]
         L.or_sim_write_axis.argtypes = [C.c_void_p, C.c_char_p]
         _lib = L
     return _lib
@@ -253,6 +257,26 @@ class Sim:
 
     def step_fields(self):
         check(lib().or_sim_step_fields(self.h))
+
+    def spread_grad(self, b, nseg):
+        """One step; J = gamma_rms / gamma_mean of beam b after it and dJ/dv of its weight profile (the
+        oracle's reverse sweep)."""
+        J = np.zeros(1)
+        g = np.zeros(nseg)
+        check(lib().or_sim_spread_grad(self.h, b, _p(J), _p(g), nseg))
+        return J[0], g
+
+    def set_delta(self, which, d):
+        """Test hook: d is added to the beam sources (which 0) or the fields (1) of every later step."""
+        d = np.ascontiguousarray(d, dtype=np.float64).ravel()
+        check(lib().or_sim_set_delta(self.h, which, _p(d) if d.size else None, d.size))
+
+    def adjoint(self, which):
+        """The adjoints of the last spread_grad: 0 fields (K, M, NCF), 1 beam sources (K, M, NCB), 2 weights."""
+        n = lib().or_sim_adjoint_get(self.h, which, None, 0)
+        out = np.zeros(n)
+        lib().or_sim_adjoint_get(self.h, which, _p(out), n)
+        return out
 
     def sweep_partial(self, nk):
         check(lib().or_sim_sweep_partial(self.h, nk))
