@@ -118,6 +118,9 @@ typedef struct qz_sweep_args {
     int load;                  /* 1: a launch that starts at slice 0 first loads every mobile species' cold rings
                                   (rings[s]) into its arrays and clears its AB history; 0: the arrays hold the load */
     qz_plasma_rings rings[QZ_MAX_SPECIES];
+    double* traj;              /* optional, one workgroup: (k_end - k_begin) x 6 x np doubles, np = the species' counts
+                                  summed: x, y, px, py, Delta, w of every particle (species 0 then 1) at the start
+                                  of each slice of the launch (the adjoint's recomputed segments) */
 } qz_sweep_args;
 
 const char* qz_last_error(void);
@@ -223,6 +226,41 @@ int qz_pack_message(const qz_sweep_args* args, int nbeams, int step, double dt, 
                     void* stream);
 int qz_unpack_message(qz_sweep_args* args, int nbeams, const double* msg, double* guard, void* stream);
 
+/* The adjoint of one m = 0 step (no counterpart in the reference, which has no adjoint; the forward it
+ * differentiates is Simulation::step_fields, src/Simulation.cpp:636-795, and Beam::advance,
+ * src/Beam.cpp:948-1131).  One call walks the slices [k_lo, k_hi) of a one-workgroup run backwards: from dJ/dfld
+ * (K, M, 6) it accumulates dJ/dbsrc (K, M, 5), through the plasma's response, slice by slice.  traj holds the
+ * plasma state at the start of each slice of the range (a qz_sweep launch over [k_lo, k_hi) with args.traj set:
+ * the checkpointed recomputation); ring (order x 5 x np) carries the plasma state's adjoint from one call to the
+ * next (zero before the call with k_hi = K); lam_bsrc is accumulated (zero it before the first call).  Particle
+ * removals and wall reflections are fixed events.  Grids up to 1023 cells; one workgroup, 512 lanes. */
+typedef struct qz_adjoint_args {
+    const double* traj;
+    int k_lo, k_hi;
+    const double* lam_fld;
+    double* lam_bsrc;
+    double* ring;
+    double* work;              /* qz_sweep_adjoint_workspace_doubles(M, np) */
+} qz_adjoint_args;
+size_t qz_sweep_adjoint_workspace_doubles(int M, int np);
+int qz_sweep_adjoint(const qz_sweep_args* args, const qz_adjoint_args* adj, void* stream);
+/* Adjoints of the beam side of the step (m = 0).  qz_beam_spread_adjoint: dJ/dgamma and the direct dJ/dw of
+ * J = sigma / mu, the weighted energy spread of the beam's live particles (Beam::sums / from_sums,
+ * src/Beam.cpp:1133-1192), given mu, sigma and the weight sum W.  qz_beam_kick_adjoint: lam_fld += dJ/dfld through
+ * the kick of Beam::advance (Vay pusher; started = 0 adds the first push's half back-kick) from dJ/dgamma after it,
+ * with the beam as it was before the push.  qz_beam_deposit_adjoint: lam_w += dJ/dw through Beam::deposit
+ * (src/Beam.cpp:725-785) from dJ/dbsrc.  qz_profile_gradient: grad = sum_i lam_w_i wmac d profile(u_i) / d v for a
+ * piecewise-linear weight profile of nseg values on [u0, u1] (u: each particle's xi as sampled, user units). */
+int qz_beam_spread_adjoint(int np, const qz_beam_soa* beam, double mu, double sigma, double wsum, double* lam_gamma,
+                           double* lam_w, void* stream);
+int qz_beam_kick_adjoint(const qz_radial_grid* grid, const qz_beam_grid* box, double qm, int pusher, int np,
+                         const qz_beam_soa* before, const double* fld, int ncf, double dt_kick, int started,
+                         const double* lam_gamma, double* lam_fld, void* stream);
+int qz_beam_deposit_adjoint(const qz_radial_grid* grid, const qz_beam_grid* box, double q, int np,
+                            const qz_beam_soa* before, const double* lam_bsrc, int ncb, double* lam_w, void* stream);
+int qz_profile_gradient(int np, const double* u, const double* lam_w, int nseg, double u0, double u1, double wmac,
+                        double* grad, void* stream);
+
 /* Frees the argument staging buffer qz_sweep / qz_sweep_batched keep for `stream` on the current device, after
  * the stream's work is done: call it before destroying a stream the sweeps were launched on (a new stream can
  * reuse the handle).  No counterpart in the reference. */
@@ -270,6 +308,13 @@ int qz_sim_beam_sums(qz_sim* s, int b, double* out17, double* refs2);
  * times the profile at the xi it was sampled at.  snapshot keeps the beams and the step state as they are;
  * rewind returns to them, so one step can be evaluated again with other profile values. */
 int qz_sim_set_beam_profile(qz_sim* s, int b, const double* values, int nseg);
+/* One step and the gradient of its objective, by the adjoint (m = 0; no counterpart in the reference): J =
+ * gamma_rms / gamma_mean of beam b after the step and dJ/dv of b's nseg weight-profile values.  The sweep runs in
+ * checkpointed segments of `every` slices; the reverse sweep recomputes each from its checkpoint (qz_sweep_adjoint).
+ * ms2 (optional): milliseconds of the checkpointed sweep and of the reverse part.  The run ends one step on. */
+int qz_sim_spread_grad(qz_sim* s, int b, int every, double* J, double* grad, int nseg, double* ms2);
+/* Number of values of beam b's weight profile (<beam>.pwl_weights; 0 without one), or a negative error. */
+int qz_sim_profile_size(qz_sim* s, int b);
 int qz_sim_snapshot(qz_sim* s);
 int qz_sim_rewind(qz_sim* s);
 /* times of the last sweep and beam push (ms, HIP events on the simulation's stream) */

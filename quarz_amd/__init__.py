@@ -124,6 +124,20 @@ def lib():
         L.qz_sim_write_fields.argtypes = [C.c_void_p, C.c_char_p]
         L.qz_sim_set_beam_profile.argtypes = [C.c_void_p, C.c_int, DP, C.c_int]
         L.qz_sim_snapshot.argtypes = [C.c_void_p]
+        L.qz_sim_spread_grad.argtypes = [C.c_void_p, C.c_int, C.c_int, DP, DP, C.c_int, DP]
+        L.qz_sim_profile_size.argtypes = [C.c_void_p, C.c_int]
+        L.qz_sweep_adjoint_workspace_doubles.restype = C.c_size_t
+        L.qz_sweep_adjoint_workspace_doubles.argtypes = [C.c_int, C.c_int]
+        L.qz_sweep_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qz_beam_spread_adjoint.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, C.c_double,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qz_beam_kick_adjoint.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_double, C.c_int, C.c_int,
+                                           C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qz_beam_deposit_adjoint.argtypes = [C.POINTER(RadialGridC), C.POINTER(BeamGridC), C.c_double, C.c_int,
+                                              C.POINTER(BeamSoAC), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.qz_profile_gradient.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.c_void_p, C.c_void_p]
         L.qz_sim_rewind.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -326,6 +340,24 @@ class Simulation:
         macro-particle's weight becomes the flat-top weight times the profile at its sampled xi."""
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().qz_sim_set_beam_profile(self.h, b, v.ctypes.data_as(DP), v.size))
+
+    def spread_grad(self, b, every=64):
+        """One step, and by the adjoint: J = gamma_rms / gamma_mean of beam b after it and dJ/dv of b's weight
+        profile (m = 0; the sweep checkpointed every `every` slices).  Returns (J, grad, (forward ms, reverse ms))."""
+        nseg = int(self.profile_size(b))
+        J = np.zeros(1)
+        g = np.zeros(nseg)
+        ms = np.zeros(2)
+        check(lib().qz_sim_spread_grad(self.h, b, every, J.ctypes.data_as(DP), g.ctypes.data_as(DP), nseg,
+                                       ms.ctypes.data_as(DP)))
+        return J[0], g, (ms[0], ms[1])
+
+    def profile_size(self, b):
+        """Number of values of beam b's weight profile (0 without one)."""
+        n = lib().qz_sim_profile_size(self.h, b)
+        if n < 0:
+            raise QuarzError(last_error())
+        return n
 
     def snapshot(self):
         """Keep the beams and the step state as they are now (see rewind)."""

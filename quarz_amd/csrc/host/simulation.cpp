@@ -1322,6 +1322,121 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
 // component its name[16] and [K][M] doubles; with m = 1 the _c / _s parts are 2 Re / -2 Im of the mode (vectors:
 // e_r and e_theta from the +1/-1 helical parts, :902-928).  Streamed: each component is formed in HBM from fld,
 // diag or bsrc and copied to the host alone, so the host holds K x M doubles instead of the whole field store.
+// ---------------------------------------------------------------------------- the adjoint of one step (m = 0)
+namespace {
+void grow(DevBuf<Real>& b, size_t n) {
+    if (b.n < n) b.alloc(n);
+}
+}  // namespace
+
+void Simulation::spread_value_grad(int bi, int every, double& J, std::vector<double>& grad) {
+    if (done) throw std::runtime_error("the run has ended (time.steps reached)");
+    if (m1) throw std::runtime_error("adjoint: modes = 0 only");
+    if (groups > 1) throw std::runtime_error("adjoint: a one-workgroup sweep only (sweep.groups = 1)");
+    if (bi < 0 || bi >= static_cast<int>(beams.size())) throw std::runtime_error("adjoint: no such beam");
+    if (every < 1) throw std::runtime_error("adjoint: checkpoint interval must be at least one slice");
+    Beam& B = *beams[bi];
+    if (B.pwl.empty()) throw std::runtime_error("adjoint: " + B.name + " has no weight profile (pwl_weights)");
+    if (B.rigid) throw std::runtime_error("adjoint: " + B.name + " is rigid");
+    if (B.pusher != 0) throw std::runtime_error("adjoint: the Vay beam pusher only");
+    if (step_n >= nsteps) throw std::runtime_error("adjoint: the step must push the beams (time.steps)");
+    if (!adj) adj = std::make_unique<AdjointBufs>();
+    AdjointBufs& W = *adj;
+    const size_t Nb = static_cast<size_t>(B.Np);
+    // the beam before its push, and the kick the push will give
+    for (int v = 0; v < 7; ++v) {
+        grow(W.before[v], std::max<size_t>(Nb, 1));
+        if (Nb) hip_check(hipMemcpyAsync(W.before[v].p, B.a[v].p, sizeof(Real) * Nb, hipMemcpyDeviceToDevice, stream),
+                          "beam copy");
+    }
+    const bool started = B.started;
+    const double dkick = have_prev ? 0.5 * (dt_prev + dt) : dt;
+    // ---- the step, its sweep in checkpointed segments
+    prepare_sweep();
+    const qz_sweep_args A = sweep_args;
+    const int nb = static_cast<int>(beams.size());
+    const int nseg = (K + every - 1) / every;
+    const size_t msg = qz_message_doubles(&A, nb);
+    grow(W.ckpt, msg * static_cast<size_t>(nseg));
+    hip_check(hipEventRecord(ev[0], stream), "event");
+    for (int s = 0; s < nseg; ++s) {
+        const int lo = s * every, hi = std::min(K, lo + every);
+        // segment 0 of a cold plasma loads its rings in the kernel: no state to keep
+        if (s > 0 || !A.load) qz_check(qz_pack_message(&A, nb, step_n, dt, 0, lo, W.ckpt.p + msg * s, stream));
+        qz_sweep_args a = A;
+        a.k_begin = lo;
+        a.k_end = hi;
+        a.load = lo == 0 ? A.load : 0;
+        launch_sweeps(d_args, &a, 1, stream);
+    }
+    hip_check(hipEventRecord(ev[1], stream), "event");
+    hip_check(hipEventSynchronize(ev[1]), "sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev[0], ev[1]), "elapsed");
+    finish_sweep(ms);
+    sweep_launches = nseg;
+    ms_adj_forward = ms;
+    after_sweep();
+    // ---- the objective
+    const std::array<double, 17> S = B.sums_device(stream);
+    const BeamDiag d = B.from_sums(S, t);
+    if (!(d.gamma_mean > 0) || !(d.gamma_rms > 0)) throw std::runtime_error("adjoint: the beam has no energy spread");
+    J = d.gamma_rms / d.gamma_mean;
+    hip_check(hipEventRecord(ev[0], stream), "event");
+    grow(W.lam_gamma, std::max<size_t>(Nb, 1));
+    grow(W.lam_w, std::max<size_t>(Nb, 1));
+    const qz_beam_soa after = B.soa();
+    qz_check(qz_beam_spread_adjoint(B.Np, &after, d.gamma_mean, d.gamma_rms, S[0], W.lam_gamma.p, W.lam_w.p, stream));
+    // ---- the kick
+    const qz_radial_grid g = grid.view();
+    const qz_beam_soa before{W.before[0].p, W.before[1].p, W.before[2].p, W.before[3].p, W.before[4].p,
+                             W.before[5].p, W.before[6].p};
+    const size_t nf = static_cast<size_t>(K) * M * NCF, nbs = static_cast<size_t>(K) * M * NCB;
+    grow(W.lam_fld, nf);
+    grow(W.lam_bsrc, nbs);
+    qz_check(qz_zero_async(W.lam_fld.p, sizeof(Real) * nf, stream));
+    qz_check(qz_zero_async(W.lam_bsrc.p, sizeof(Real) * nbs, stream));
+    qz_check(qz_beam_kick_adjoint(&g, &box, B.q / B.m, B.pusher, B.Np, &before, fld.p, NCF, dkick, started ? 1 : 0,
+                                  W.lam_gamma.p, W.lam_fld.p, stream));
+    // ---- the reverse sweep, segment by segment from the box's tail
+    int np = 0;
+    for (int s = 0; s < A.nspecies; ++s) np += A.sp[s].np;
+    const size_t nring = static_cast<size_t>(A.ab_order) * 5 * np;
+    grow(W.ring, std::max<size_t>(nring, 1));
+    grow(W.traj, std::max<size_t>(static_cast<size_t>(every) * 6 * np, 1));
+    grow(W.work, qz_sweep_adjoint_workspace_doubles(M, np));
+    qz_check(qz_zero_async(W.ring.p, sizeof(Real) * std::max<size_t>(nring, 1), stream));
+    for (int s = nseg - 1; s >= 0; --s) {
+        const int lo = s * every, hi = std::min(K, lo + every);
+        qz_sweep_args a = A;
+        if (s > 0 || !A.load) qz_check(qz_unpack_message(&a, nb, W.ckpt.p + msg * s, nullptr, stream));
+        a.k_begin = lo;
+        a.k_end = hi;
+        a.load = lo == 0 && (s == 0 && A.load) ? 1 : 0;
+        a.traj = W.traj.p;
+        a.stats = nullptr;
+        launch_sweeps(d_args, &a, 1, stream);
+        const qz_adjoint_args J{W.traj.p, lo, hi, W.lam_fld.p, W.lam_bsrc.p, W.ring.p, W.work.p};
+        qz_check(qz_sweep_adjoint(&a, &J, stream));
+    }
+    // ---- the beam deposit and the profile
+    qz_check(qz_beam_deposit_adjoint(&g, &box, B.q, B.Np, &before, W.lam_bsrc.p, NCB, W.lam_w.p, stream));
+    if (W.u_beam != bi || W.u.n < Nb) {
+        std::vector<Real> u(B.xi_init.size());
+        for (size_t i = 0; i < u.size(); ++i) u[i] = xi_to_user(B.xi_init[i]);
+        W.u.upload(u, stream);
+        W.u_beam = bi;
+    }
+    const int S_ = static_cast<int>(B.pwl.size());
+    grow(W.grad, static_cast<size_t>(S_));
+    qz_check(qz_profile_gradient(B.Np, W.u.p, W.lam_w.p, S_, B.pwl_u0, B.pwl_u1, B.pwl_wmac, W.grad.p, stream));
+    hip_check(hipEventRecord(ev[1], stream), "event");
+    grad.assign(static_cast<size_t>(S_), 0.0);
+    hip_check(hipMemcpyAsync(grad.data(), W.grad.p, sizeof(Real) * S_, hipMemcpyDeviceToHost, stream), "gradient");
+    hip_check(hipStreamSynchronize(stream), "sync");
+    hip_check(hipEventElapsedTime(&ms_adj_reverse, ev[0], ev[1]), "elapsed");
+}
+
 void Simulation::write_fields(const std::string& fn) {
     struct Comp { const char* name; int src; int c0; double s0; int c1; double s1; };   // src: 0 fld, 1 diag, 2 bsrc
     std::vector<Comp> comps = {
@@ -1910,6 +2025,25 @@ extern "C" int qz_sim_set_beam_profile(qz_sim* h, int b, const double* values, i
         if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
         s.beams[b]->set_profile(std::vector<double>(values, values + nseg), s.stream);
     })
+}
+extern "C" int qz_sim_spread_grad(qz_sim* h, int b, int every, double* J, double* grad, int nseg, double* ms2) {
+    if (!h || !J || !grad) return qz_fail(QZ_EINVAL, "qz_sim_spread_grad: null argument");
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        if (b < 0 || b >= static_cast<int>(s.beams.size())) throw std::runtime_error("no such beam");
+        if (nseg != static_cast<int>(s.beams[b]->pwl.size()))
+            throw std::runtime_error("qz_sim_spread_grad: the profile has " + std::to_string(s.beams[b]->pwl.size()) +
+                                     " values");
+        std::vector<double> g;
+        s.spread_value_grad(b, every, *J, g);
+        std::copy(g.begin(), g.end(), grad);
+        if (ms2) { ms2[0] = s.ms_adj_forward; ms2[1] = s.ms_adj_reverse; }
+    })
+}
+extern "C" int qz_sim_profile_size(qz_sim* h, int b) {
+    if (!h) return qz_fail(QZ_EINVAL, "qz_sim_profile_size: null run");
+    if (b < 0 || b >= static_cast<int>(h->s->beams.size())) return qz_fail(QZ_EINVAL, "qz_sim_profile_size: no such beam");
+    return static_cast<int>(h->s->beams[b]->pwl.size());
 }
 extern "C" int qz_sim_snapshot(qz_sim* h) {
     if (!h) return qz_fail(QZ_EINVAL, "qz_sim_snapshot: null run");

@@ -161,6 +161,18 @@ public:
     // the beams and the step state as they are now, and back to them (repeated evaluations of one step)
     void snapshot();
     void rewind();
+    // One step (m = 0) and the gradient of its objective J = gamma_rms / gamma_mean of beam b after the step with
+    // respect to b's weight-profile values, by the adjoint: the sweep runs in segments of `every` slices with the
+    // plasma state saved at each segment's start; the reverse sweep recomputes each segment from its
+    // checkpoint, last segment first.  The run ends the call one step on, as after step().
+    void spread_value_grad(int b, int every, double& J, std::vector<double>& grad);
+    struct AdjointBufs {
+        DevBuf<Real> ckpt, traj, ring, work, lam_fld, lam_bsrc, lam_gamma, lam_w, u, grad;
+        DevBuf<Real> before[7];   // the beam before its push
+        int u_beam = -1;          // the beam whose sampled xi `u` holds
+    };
+    std::unique_ptr<AdjointBufs> adj;
+    float ms_adj_forward = 0, ms_adj_reverse = 0;   // the checkpointed sweep and the reverse part of the last call
 
     bool m1 = false;
     int picard = 1;

@@ -134,6 +134,7 @@ int qz_validate_sweep(const qz_sweep_args& a) {
                 return qz_fail(QZ_EINVAL, "sweep: the ring load of a species does not match its particle count");
         }
     if (a.groups > 1 && !a.mws) return qz_fail(QZ_EINVAL, "sweep: a run of several workgroups needs its workspace mws");
+    if (a.traj && a.groups > 1) return qz_fail(QZ_EINVAL, "sweep: the trajectory buffer needs a one-workgroup run");
     for (int s = 0; s < a.nspecies; ++s)
         if (a.sp[s].np < 0 || a.sp[s].np > a.sp[s].cap || !a.sp[s].hist) return qz_fail(QZ_EINVAL, "sweep: bad species");
     return QZ_OK;

@@ -216,9 +216,19 @@ __global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restr
                 const qz_species_state& st = s1 ? A.sp[1] : A.sp[0];
                 const int li = s1 ? gi - np0 : gi;
                 const Real wi = st.w[li];
+                // the state at the start of the slice, for the adjoint's recomputation
+                Real* tr = A.traj ? A.traj + static_cast<size_t>(k - kb) * 6 * NP + gi : nullptr;
+                if (tr) tr[5 * static_cast<size_t>(NP)] = wi;
                 if (wi != Real(0)) {
                     const Real X = st.x[li], Y = st.y[li];
                     const Real PX = st.px[li], PY = st.py[li], D = st.dl[li];
+                    if (tr) {
+                        tr[0] = X;
+                        tr[NP] = Y;
+                        tr[2 * static_cast<size_t>(NP)] = PX;
+                        tr[3 * static_cast<size_t>(NP)] = PY;
+                        tr[4 * static_cast<size_t>(NP)] = D;
+                    }
                     const Real r = dsqrt(X * X + Y * Y);
                     Real c, s_;
                     if (r > Real(0)) { c = X / r; s_ = Y / r; } else { c = Real(1); s_ = Real(0); }

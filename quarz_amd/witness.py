@@ -9,13 +9,16 @@ spread after one time step,
 with the weighted moments of the reference's beam log (src/Beam.cpp:1157-1192).  One evaluation is one full
 step: beam deposit, the plasma sweep, the beam push.
 
-Gradient: central finite differences of the FP64 step,
+Gradient, two ways:
 
-    dJ/dv_s = (J(v + eps e_s) - J(v - eps e_s)) / (2 eps),
+  - "adjoint" (the default): the hand-written adjoint of the step (qz_sim_spread_grad): the sweep checkpointed
+    every 64 slices, the reverse xi sweep recomputing each segment from its checkpoint, the adjoints of the
+    beam kick, the beam deposit and the objective.  One forward and one reverse sweep per gradient.
+  - "fd": central finite differences of the FP64 step, dJ/dv_s = (J(v + eps e_s) - J(v - eps e_s)) / (2 eps),
+    the 2S + 1 evaluations run as one batched step (one persistent sweep launch, one workgroup per run): the
+    check of the adjoint.
 
-with the 2S + 1 evaluations run as one batched step (one persistent sweep launch, one workgroup per run).
-The hand-written adjoint sweep is not built; `SpreadFunction` puts this finite-difference gradient behind
-torch.autograd, and `optimise` runs Adam on the profile with it.
+`spread` puts either behind torch.autograd, and `optimise` runs Adam on the profile with it.
 """
 from concurrent.futures import ThreadPoolExecutor
 
@@ -33,9 +36,11 @@ class SpreadObjective:
     """J(v) for many profiles v at once.  Holds `nruns` copies of the run made from `deck`, each with a
     snapshot of its freshly sampled beams: every evaluation rewinds a copy, sets its profile and steps it."""
 
-    def __init__(self, deck, beam=1, nseg=16, nruns=None):
-        self.beam, self.nseg = beam, nseg
-        self.sims = [Simulation(deck) for _ in range(nruns or 2 * nseg + 1)]
+    def __init__(self, deck, beam=1, nseg=16, nruns=None, overrides=None, method="adjoint", every=64):
+        if method not in ("adjoint", "fd"):
+            raise ValueError("method is 'adjoint' or 'fd'")
+        self.beam, self.nseg, self.method, self.every = beam, nseg, method, every
+        self.sims = [Simulation(deck, overrides) for _ in range(nruns or (1 if method == "adjoint" else 2 * nseg + 1))]
         for s in self.sims:
             s.snapshot()
         self.evaluations = 0
@@ -69,9 +74,18 @@ class SpreadObjective:
         self.evaluations += len(sims)
         return np.array(list(self._pool.map(lambda s: relative_spread(s.beam_diag(self.beam)), sims)))
 
-    def value_and_grad(self, v, eps=1e-6):
-        """J(v) and its central finite-difference gradient, in one batched step of 2S + 1 runs."""
+    def value_and_grad(self, v, eps=1e-6, method=None):
+        """J(v) and dJ/dv: by the adjoint (one step and its reverse sweep) or by central finite differences (one
+        batched step of 2S + 1 runs)."""
         v = np.asarray(v, dtype=np.float64)
+        if (method or self.method) == "adjoint":
+            s = self.sims[0]
+            s.rewind()
+            s.set_beam_profile(self.beam, v)
+            J, g, ms = s.spread_grad(self.beam, self.every)
+            self.last_adjoint_ms = ms
+            self.evaluations += 1
+            return J, g
         if len(self.sims) < 2 * self.nseg + 1:
             raise ValueError("the gradient needs 2 S + 1 runs")
         prof = [v]
@@ -93,7 +107,8 @@ def _torch_function():
     import torch
 
     class SpreadFunction(torch.autograd.Function):
-        """J(v) as a torch op; its backward is the finite-difference gradient made with the forward."""
+        """J(v) as a torch op; its backward is the gradient made with the forward (adjoint or finite
+        differences, the objective's method)."""
 
         @staticmethod
         def forward(ctx, v, objective, eps):
