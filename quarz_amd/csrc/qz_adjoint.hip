@@ -38,8 +38,14 @@ enum AdjLds : int {
     A_WR, A_EZ, A_BZ, A_BR, A_BI, A_SR, A_SI, A_CHI, A_JZ, A_RHOT, A_DRHO, A_JPI,
     S_CHI, L_NARR
 };
-// global workspace arrays (M doubles each), then the particles' partial adjoints (5 x NP)
-enum AdjWork : int { G_MU0 = 0, G_MU1, G_MU2, G_RHS, G_CP0, G_DP0, G_CP1, G_DP1, G_CP2, G_DP2, G_NARR };
+// the solves' arrays (M doubles each): the operators' rows (L0's as the factors of its transpose, formed once per
+// launch), the three concurrent solutions, the c' of the two chi-dependent solves, the psi right-hand side.  In
+// LDS after the arrays above when the grid leaves room, else in the global workspace; then (global) the
+// particles' partial adjoints (5 x NP)
+enum AdjSolve : int {
+    O_C0 = 0, O_MT0, O_CPT0, O_A1D, O_B1D, O_C1D, O_A1F, O_B1F, O_C1F,
+    T_MU0, T_MU1, T_MU2, T_CP0, T_CP1, T_RHS, S_NARR
+};
 
 // ---------------------------------------------------------------- particle functions over any number type
 template <class T> struct AGeo {
@@ -185,27 +191,50 @@ __device__ __forceinline__ void scatter_gathered(const AGeo<double>& p, double l
     }
 }
 
-// T^T mu = rhs for the system FieldSolver::solve forms (src/FieldSolver.cpp:242-261; Dirichlet rows are identity
-// rows, the other diagonals b - chi); one lane, the recurrences of Thomas (src/Tridiag.cpp:111-126) on the
-// transposed diagonals
-__device__ void thomas_transposed(const double* __restrict__ a, const double* __restrict__ b,
-                                  const double* __restrict__ c, const double* chi, bool axis_regular, bool flux,
-                                  const double* rhs, double* mu, double* cp, double* dp, int N) {
-    auto diag = [&](int j) {
-        const bool dir = (j == N && !flux) || (j == 0 && !axis_regular);
-        return (dir || !chi) ? b[j] : b[j] - chi[j];
-    };
-    double b0 = diag(0);
-    cp[0] = a[1] / b0;
-    dp[0] = rhs[0] / b0;
+// T^T mu = rhs for the B+ operators of FieldSolver::solve (src/FieldSolver.cpp:242-261: Dirichlet rows are
+// identity rows, the other diagonals b - chi; row 0 is a Dirichlet row, row N too unless it is the flux row), one
+// lane, the recurrences of Thomas (src/Tridiag.cpp:111-126) on the transposed diagonals c[i-1], diag, a[i+1]
+__device__ void thomas_transposed_chi(const double* a, const double* b, const double* c, const double* chi, bool flux,
+                                      const double* rhs, double* mu, double* cp, int N) {
+    auto diag = [&](int j) { return (j == 0 || (j == N && !flux)) ? b[j] : b[j] - chi[j]; };
+    const double i0 = 1.0 / diag(0);
+    cp[0] = a[1] * i0;
+    double prev = rhs[0] * i0;
+    mu[0] = prev;
     for (int i = 1; i <= N; ++i) {
-        const double sub = c[i - 1], sup = i < N ? a[i + 1] : 0.0;
+        const double sub = c[i - 1];
         const double m = 1.0 / (diag(i) - sub * cp[i - 1]);
-        cp[i] = sup * m;
-        dp[i] = (rhs[i] - sub * dp[i - 1]) * m;
+        cp[i] = (i < N ? a[i + 1] : 0.0) * m;
+        prev = (rhs[i] - sub * prev) * m;
+        mu[i] = prev;
     }
-    mu[N] = dp[N];
-    for (int i = N - 1; i >= 0; --i) mu[i] = dp[i] - cp[i] * mu[i + 1];
+    for (int i = N - 1; i >= 0; --i) {
+        prev = mu[i] - cp[i] * prev;
+        mu[i] = prev;
+    }
+}
+// the transposed factors of L0 (no chi): mT_i = 1 / (b_i - c_{i-1} cpT_{i-1}), cpT_i = a_{i+1} mT_i
+__device__ void l0_transposed_factors(const double* a, const double* b, const double* c, double* mT, double* cpT,
+                                      int N) {
+    mT[0] = 1.0 / b[0];
+    cpT[0] = a[1] * mT[0];
+    for (int i = 1; i <= N; ++i) {
+        mT[i] = 1.0 / (b[i] - c[i - 1] * cpT[i - 1]);
+        cpT[i] = (i < N ? a[i + 1] : 0.0) * mT[i];
+    }
+}
+__device__ void l0_transposed_solve(const double* c, const double* mT, const double* cpT, const double* rhs, double* mu,
+                                    int N) {
+    double prev = rhs[0] * mT[0];
+    mu[0] = prev;
+    for (int i = 1; i <= N; ++i) {
+        prev = (rhs[i] - c[i - 1] * prev) * mT[i];
+        mu[i] = prev;
+    }
+    for (int i = N - 1; i >= 0; --i) {
+        prev = mu[i] - cpT[i] * prev;
+        mu[i] = prev;
+    }
 }
 
 // out_j of D^T l for the radial gradient stencils (FieldSolver::gradient / gradient_odd, src/FieldSolver.cpp:263-281)
@@ -234,10 +263,14 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
     const int N = A.grid.N, M = N + 1;
     const int Mp = (M + 1) & ~1;
     auto LA = [&](int a) { return lds + static_cast<size_t>(a) * Mp; };
-    auto GW = [&](int a) { return J.work + static_cast<size_t>(a) * M; };
+    // the solves' arrays: LDS when they fit after the node arrays, else global
+    const bool solve_lds = static_cast<size_t>(L_NARR + S_NARR) * Mp * 8 <= static_cast<size_t>(kAdjLdsBytes);
+    double* sbase = solve_lds ? lds + static_cast<size_t>(L_NARR) * Mp : J.work;
+    const size_t sstride = solve_lds ? static_cast<size_t>(Mp) : static_cast<size_t>(M);
+    auto SA = [&](int a) { return sbase + static_cast<size_t>(a) * sstride; };
     const int np0 = A.nspecies > 0 ? A.sp[0].np : 0, np1 = A.nspecies > 1 ? A.sp[1].np : 0;
     const int NP = np0 + np1;
-    double* part = J.work + static_cast<size_t>(G_NARR) * M;   // 5 x NP
+    double* part = J.work + static_cast<size_t>(S_NARR) * M;   // 5 x NP
     const int K = A.nslices, ord = A.ab_order;
     const double dxi = A.dxi, idxi = 1.0 / dxi, R = A.grid.R;
     const double* inv_V = A.grid.inv_V;
@@ -247,7 +280,15 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
     for (int j = tid; j < M; j += nth) {
         LA(L_R)[j] = A.grid.r[j];
         LA(L_IH)[j] = j < N ? A.grid.inv_h[j] : 0.0;
+        SA(O_C0)[j] = A.op_c[OP_L0][j];
+        SA(O_A1D)[j] = A.op_a[OP_L1D][j];
+        SA(O_B1D)[j] = A.op_b[OP_L1D][j];
+        SA(O_C1D)[j] = A.op_c[OP_L1D][j];
+        SA(O_A1F)[j] = A.op_a[OP_L1F][j];
+        SA(O_B1F)[j] = A.op_b[OP_L1F][j];
+        SA(O_C1F)[j] = A.op_c[OP_L1F][j];
     }
+    if (tid == 0) l0_transposed_factors(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], SA(O_MT0), SA(O_CPT0), N);
     __syncthreads();
     for (int k = J.k_hi - 1; k >= J.k_lo; --k) {
         const double* st = J.traj + static_cast<size_t>(k - J.k_lo) * 6 * NP;
@@ -317,15 +358,13 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
         // ---- B+ (src/Simulation.cpp:535-591, m = 0: one pass): br = L1D(chi)^-1 (-S_i1),
         //      bi = L1F(chi, jz)^-1 (grad jz + S_r1)
         if (tid == 0)
-            thomas_transposed(A.op_a[OP_L1D], A.op_b[OP_L1D], A.op_c[OP_L1D], LA(S_CHI), false, false, LA(A_BR),
-                              GW(G_MU0), GW(G_CP0), GW(G_DP0), N);
+            thomas_transposed_chi(SA(O_A1D), SA(O_B1D), SA(O_C1D), LA(S_CHI), false, LA(A_BR), SA(T_MU0), SA(T_CP0), N);
         else if (tid == 64)
-            thomas_transposed(A.op_a[OP_L1F], A.op_b[OP_L1F], A.op_c[OP_L1F], LA(S_CHI), false, true, LA(A_BI),
-                              GW(G_MU1), GW(G_CP1), GW(G_DP1), N);
+            thomas_transposed_chi(SA(O_A1F), SA(O_B1F), SA(O_C1F), LA(S_CHI), true, LA(A_BI), SA(T_MU1), SA(T_CP1), N);
         __syncthreads();
         {
-            const double* mr = GW(G_MU0);
-            double* mi = GW(G_MU1);
+            const double* mr = SA(T_MU0);
+            double* mi = SA(T_MU1);
             for (int j = tid; j < M; j += nth) {
                 const double vr = (j == 0 || j == N) ? 0.0 : mr[j];   // L1D rows 0 and N are Dirichlet rows
                 const double vi = j == 0 ? 0.0 : mi[j];                // L1F row 0 is
@@ -372,24 +411,21 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
         __syncthreads();
         // ---- Ez, Bz and psi (src/Simulation.cpp:477-532, 797-819): bz = L0^-1 t3, t3 = -(grad_odd + over_r) jp_i1;
         //      ez = L0^-1 (-drho); wr = -grad psi, psi = L0^-1 (-rhot)
-        for (int j = tid; j < M; j += nth) GW(G_RHS)[j] = -gradient_T(gm, g0, gp, ir, LA(A_WR), j, N, false);
+        for (int j = tid; j < M; j += nth) SA(T_RHS)[j] = -gradient_T(gm, g0, gp, ir, LA(A_WR), j, N, false);
         __syncthreads();
         if (tid == 0)
-            thomas_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_BZ), GW(G_MU0),
-                              GW(G_CP0), GW(G_DP0), N);
+            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), LA(A_BZ), SA(T_MU0), N);
         else if (tid == 64)
-            thomas_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_EZ), GW(G_MU1),
-                              GW(G_CP1), GW(G_DP1), N);
+            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), LA(A_EZ), SA(T_MU1), N);
         else if (tid == 128)
-            thomas_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, GW(G_RHS),
-                              GW(G_MU2), GW(G_CP2), GW(G_DP2), N);
+            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), SA(T_RHS), SA(T_MU2), N);
         __syncthreads();
-        if (tid == 0) GW(G_MU0)[N] = 0.0;   // L0's row N is a Dirichlet row
+        if (tid == 0) SA(T_MU0)[N] = 0.0;   // L0's row N is a Dirichlet row
         __syncthreads();
         for (int j = tid; j < M; j += nth) {
-            LA(A_DRHO)[j] = j < N ? -GW(G_MU1)[j] : 0.0;
-            LA(A_RHOT)[j] = j < N ? -GW(G_MU2)[j] : 0.0;
-            const double* lt = GW(G_MU0);
+            LA(A_DRHO)[j] = j < N ? -SA(T_MU1)[j] : 0.0;
+            LA(A_RHOT)[j] = j < N ? -SA(T_MU2)[j] : 0.0;
+            const double* lt = SA(T_MU0);
             LA(A_JPI)[j] = -(gradient_T(gm, g0, gp, ir, lt, j, N, true) + over_r_T(ir, lt, j));
         }
         __syncthreads();
@@ -622,7 +658,12 @@ __global__ void profile_gradient_kernel(int np, const double* __restrict__ u, co
     for (int s = threadIdx.x; s < nseg; s += blockDim.x) atomicAdd(&grad[s], sg[s]);
 }
 
-size_t adjoint_lds_bytes(int M) { return static_cast<size_t>(L_NARR) * static_cast<size_t>((M + 1) & ~1) * 8; }
+// LDS of a launch: the node arrays, and the solves' arrays when they fit beside them
+size_t adjoint_lds_bytes(int M) {
+    const size_t Mp = static_cast<size_t>((M + 1) & ~1);
+    const size_t all = static_cast<size_t>(L_NARR + S_NARR) * Mp * 8;
+    return all <= static_cast<size_t>(kAdjLdsBytes) ? all : static_cast<size_t>(L_NARR) * Mp * 8;
+}
 
 }  // namespace
 }  // namespace qz
@@ -630,7 +671,7 @@ size_t adjoint_lds_bytes(int M) { return static_cast<size_t>(L_NARR) * static_ca
 using namespace qz;
 
 extern "C" size_t qz_sweep_adjoint_workspace_doubles(int M, int np) {
-    return static_cast<size_t>(G_NARR) * (M > 0 ? M : 0) + 5 * static_cast<size_t>(np > 0 ? np : 0);
+    return static_cast<size_t>(S_NARR) * (M > 0 ? M : 0) + 5 * static_cast<size_t>(np > 0 ? np : 0);
 }
 
 extern "C" int qz_sweep_adjoint(const qz_sweep_args* a, const qz_adjoint_args* j, void* stream) {
