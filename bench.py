@@ -290,7 +290,8 @@ def bench_gradient(args):
         "config": {"workload": f"c4: {configs.DESCRIPTION['c4']}", "M": sim.M, "slices": sim.K,
                    "checkpoint_every": 64, "parallelism": "one workgroup; forward, recomputation and reverse sweep"},
         "gradient": "hand-written adjoint (reverse xi sweep, checkpoints every 64 slices)",
-        "adjoint_ms": {"checkpointed_forward": fwd_ms / iters, "reverse": rev_ms / iters, "per_gradient": wall / iters * 1e3},
+        "adjoint_ms": {"checkpointed_forward": fwd_ms / iters, "reverse": rev_ms / iters, "per_gradient": wall / iters * 1e3,
+                       "reverse_kernel_phases_last": {k: v / 1e5 for k, v in obj.sims[0].adjoint_ticks().items()}},
         "fd_check": {"by_eps": fd_check, "max_rel_err_best_of_two_eps": float(best.max()),
                      "batched_fd_gradient_ms": fd_ms, "batched_sweep_ms": fd_batch_ms,
                      "note": "J jumps where a plasma particle crosses a node (the dxi-derivative deposit uses the shape's "

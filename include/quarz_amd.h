@@ -241,7 +241,10 @@ typedef struct qz_adjoint_args {
     double* lam_bsrc;
     double* ring;
     double* work;              /* qz_sweep_adjoint_workspace_doubles(M, np) */
+    unsigned long long* ticks; /* optional: += wall-clock ticks of the reverse sweep's phases (QZ_ADJ_NPHASE) */
 } qz_adjoint_args;
+#define QZ_ADJ_NPHASE 8        /* fields, chi + forces, B+ solves, B+ nodes, deposit_S, Ez/Bz/psi solves, their nodes,
+                                  sources + deposit */
 size_t qz_sweep_adjoint_workspace_doubles(int M, int np);
 int qz_sweep_adjoint(const qz_sweep_args* args, const qz_adjoint_args* adj, void* stream);
 /* Adjoints of the beam side of the step (m = 0).  qz_beam_spread_adjoint: dJ/dgamma and the direct dJ/dw of
@@ -313,6 +316,8 @@ int qz_sim_set_beam_profile(qz_sim* s, int b, const double* values, int nseg);
  * checkpointed segments of `every` slices; the reverse sweep recomputes each from its checkpoint (qz_sweep_adjoint).
  * ms2 (optional): milliseconds of the checkpointed sweep and of the reverse part.  The run ends one step on. */
 int qz_sim_spread_grad(qz_sim* s, int b, int every, double* J, double* grad, int nseg, double* ms2);
+/* Wall-clock ticks (100 MHz clock) of the last qz_sim_spread_grad's reverse-sweep phases, QZ_ADJ_NPHASE doubles. */
+int qz_sim_adjoint_ticks(qz_sim* s, double* out);
 /* Number of values of beam b's weight profile (<beam>.pwl_weights; 0 without one), or a negative error. */
 int qz_sim_profile_size(qz_sim* s, int b);
 int qz_sim_snapshot(qz_sim* s);

@@ -126,6 +126,7 @@ def lib():
         L.qz_sim_snapshot.argtypes = [C.c_void_p]
         L.qz_sim_spread_grad.argtypes = [C.c_void_p, C.c_int, C.c_int, DP, DP, C.c_int, DP]
         L.qz_sim_profile_size.argtypes = [C.c_void_p, C.c_int]
+        L.qz_sim_adjoint_ticks.argtypes = [C.c_void_p, DP]
         L.qz_sweep_adjoint_workspace_doubles.restype = C.c_size_t
         L.qz_sweep_adjoint_workspace_doubles.argtypes = [C.c_int, C.c_int]
         L.qz_sweep_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -351,6 +352,15 @@ class Simulation:
         check(lib().qz_sim_spread_grad(self.h, b, every, J.ctypes.data_as(DP), g.ctypes.data_as(DP), nseg,
                                        ms.ctypes.data_as(DP)))
         return J[0], g, (ms[0], ms[1])
+
+    ADJ_PHASES = ["fields", "chi_forces", "bplus_solves", "bplus_nodes", "deposit_S", "poisson_solves",
+                  "poisson_nodes", "sources_deposit"]
+
+    def adjoint_ticks(self):
+        """Wall-clock ticks (100 MHz) of the last spread_grad's reverse-sweep phases."""
+        out = np.zeros(len(self.ADJ_PHASES))
+        check(lib().qz_sim_adjoint_ticks(self.h, out.ctypes.data_as(DP)))
+        return dict(zip(self.ADJ_PHASES, out))
 
     def profile_size(self, b):
         """Number of values of beam b's weight profile (0 without one)."""

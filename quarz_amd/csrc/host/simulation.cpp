@@ -1329,6 +1329,13 @@ void grow(DevBuf<Real>& b, size_t n) {
 }
 }  // namespace
 
+Simulation::AdjointBufs::~AdjointBufs() {
+    if (st2) (void)hipStreamSynchronize(st2);
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (st2) (void)hipStreamDestroy(st2);
+}
+
 void Simulation::spread_value_grad(int bi, int every, double& J, std::vector<double>& grad) {
     if (done) throw std::runtime_error("the run has ended (time.steps reached)");
     if (m1) throw std::runtime_error("adjoint: modes = 0 only");
@@ -1398,27 +1405,50 @@ void Simulation::spread_value_grad(int bi, int every, double& J, std::vector<dou
     qz_check(qz_zero_async(W.lam_bsrc.p, sizeof(Real) * nbs, stream));
     qz_check(qz_beam_kick_adjoint(&g, &box, B.q / B.m, B.pusher, B.Np, &before, fld.p, NCF, dkick, started ? 1 : 0,
                                   W.lam_gamma.p, W.lam_fld.p, stream));
-    // ---- the reverse sweep, segment by segment from the box's tail
+    // ---- the reverse sweep, segment by segment from the box's tail.  Segment s's reverse sweep (stream st2)
+    // overlaps the recomputation of segment s - 1 (the run's stream): they touch different slices and buffers.
     int np = 0;
     for (int s = 0; s < A.nspecies; ++s) np += A.sp[s].np;
     const size_t nring = static_cast<size_t>(A.ab_order) * 5 * np;
     grow(W.ring, std::max<size_t>(nring, 1));
-    grow(W.traj, std::max<size_t>(static_cast<size_t>(every) * 6 * np, 1));
+    for (auto& tb : W.traj) grow(tb, std::max<size_t>(static_cast<size_t>(every) * 6 * np, 1));
     grow(W.work, qz_sweep_adjoint_workspace_doubles(M, np));
     qz_check(qz_zero_async(W.ring.p, sizeof(Real) * std::max<size_t>(nring, 1), stream));
-    for (int s = nseg - 1; s >= 0; --s) {
+    if (W.ticks.n < QZ_ADJ_NPHASE) W.ticks.alloc(QZ_ADJ_NPHASE);
+    qz_check(qz_zero_async(W.ticks.p, sizeof(unsigned long long) * QZ_ADJ_NPHASE, stream));
+    if (!W.st2) hip_check(hipStreamCreateWithFlags(&W.st2, hipStreamNonBlocking), "stream");
+    while (W.ev.size() < 2 * static_cast<size_t>(nseg)) {
+        hipEvent_t e = nullptr;
+        hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+        W.ev.push_back(e);
+    }
+    auto recompute = [&](int s) {
         const int lo = s * every, hi = std::min(K, lo + every);
         qz_sweep_args a = A;
         if (s > 0 || !A.load) qz_check(qz_unpack_message(&a, nb, W.ckpt.p + msg * s, nullptr, stream));
         a.k_begin = lo;
         a.k_end = hi;
-        a.load = lo == 0 && (s == 0 && A.load) ? 1 : 0;
-        a.traj = W.traj.p;
+        a.load = (s == 0 && A.load) ? 1 : 0;
+        a.traj = W.traj[s & 1].p;
         a.stats = nullptr;
         launch_sweeps(d_args, &a, 1, stream);
-        const qz_adjoint_args J{W.traj.p, lo, hi, W.lam_fld.p, W.lam_bsrc.p, W.ring.p, W.work.p};
-        qz_check(qz_sweep_adjoint(&a, &J, stream));
+        hip_check(hipEventRecord(W.ev[2 * s], stream), "event");
+        return a;
+    };
+    qz_sweep_args a = recompute(nseg - 1);
+    for (int s = nseg - 1; s >= 0; --s) {
+        const int lo = s * every, hi = std::min(K, lo + every);
+        hip_check(hipStreamWaitEvent(W.st2, W.ev[2 * s], 0), "stream wait");
+        const qz_adjoint_args J{W.traj[s & 1].p, lo, hi, W.lam_fld.p, W.lam_bsrc.p, W.ring.p, W.work.p, W.ticks.p};
+        qz_check(qz_sweep_adjoint(&a, &J, W.st2));
+        hip_check(hipEventRecord(W.ev[2 * s + 1], W.st2), "event");
+        if (s > 0) {
+            // segment s - 1 reuses the trajectory buffer of segment s + 1: after that segment's reverse sweep
+            if (s + 1 < nseg) hip_check(hipStreamWaitEvent(stream, W.ev[2 * (s + 1) + 1], 0), "stream wait");
+            a = recompute(s - 1);
+        }
     }
+    hip_check(hipStreamWaitEvent(stream, W.ev[1], 0), "stream wait");
     // ---- the beam deposit and the profile
     qz_check(qz_beam_deposit_adjoint(&g, &box, B.q, B.Np, &before, W.lam_bsrc.p, NCB, W.lam_w.p, stream));
     if (W.u_beam != bi || W.u.n < Nb) {
@@ -1435,6 +1465,9 @@ void Simulation::spread_value_grad(int bi, int every, double& J, std::vector<dou
     hip_check(hipMemcpyAsync(grad.data(), W.grad.p, sizeof(Real) * S_, hipMemcpyDeviceToHost, stream), "gradient");
     hip_check(hipStreamSynchronize(stream), "sync");
     hip_check(hipEventElapsedTime(&ms_adj_reverse, ev[0], ev[1]), "elapsed");
+    std::vector<unsigned long long> tk;
+    W.ticks.download(tk, stream);
+    for (int i = 0; i < QZ_ADJ_NPHASE; ++i) W.last_ticks[i] = static_cast<double>(tk[i]);
 }
 
 void Simulation::write_fields(const std::string& fn) {
@@ -2039,6 +2072,11 @@ extern "C" int qz_sim_spread_grad(qz_sim* h, int b, int every, double* J, double
         std::copy(g.begin(), g.end(), grad);
         if (ms2) { ms2[0] = s.ms_adj_forward; ms2[1] = s.ms_adj_reverse; }
     })
+}
+extern "C" int qz_sim_adjoint_ticks(qz_sim* h, double* out) {
+    if (!h || !out) return qz_fail(QZ_EINVAL, "qz_sim_adjoint_ticks: null argument");
+    for (int i = 0; i < QZ_ADJ_NPHASE; ++i) out[i] = h->s->adj ? h->s->adj->last_ticks[i] : 0.0;
+    return QZ_OK;
 }
 extern "C" int qz_sim_profile_size(qz_sim* h, int b) {
     if (!h) return qz_fail(QZ_EINVAL, "qz_sim_profile_size: null run");

@@ -167,9 +167,18 @@ public:
     // checkpoint, last segment first.  The run ends the call one step on, as after step().
     void spread_value_grad(int b, int every, double& J, std::vector<double>& grad);
     struct AdjointBufs {
-        DevBuf<Real> ckpt, traj, ring, work, lam_fld, lam_bsrc, lam_gamma, lam_w, u, grad;
+        DevBuf<Real> ckpt, traj[2], ring, work, lam_fld, lam_bsrc, lam_gamma, lam_w, u, grad;
         DevBuf<Real> before[7];   // the beam before its push
+        DevBuf<unsigned long long> ticks;   // the reverse sweep's phase clocks
+        double last_ticks[QZ_ADJ_NPHASE] = {};
         int u_beam = -1;          // the beam whose sampled xi `u` holds
+        // the reverse sweep runs on its own stream while the run's stream recomputes the next segment
+        hipStream_t st2 = nullptr;
+        std::vector<hipEvent_t> ev;   // per segment: recomputed, reversed
+        AdjointBufs() = default;
+        AdjointBufs(const AdjointBufs&) = delete;
+        AdjointBufs& operator=(const AdjointBufs&) = delete;
+        ~AdjointBufs();
     };
     std::unique_ptr<AdjointBufs> adj;
     float ms_adj_forward = 0, ms_adj_reverse = 0;   // the checkpointed sweep and the reverse part of the last call

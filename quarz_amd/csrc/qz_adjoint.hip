@@ -38,13 +38,13 @@ enum AdjLds : int {
     A_WR, A_EZ, A_BZ, A_BR, A_BI, A_SR, A_SI, A_CHI, A_JZ, A_RHOT, A_DRHO, A_JPI,
     S_CHI, L_NARR
 };
-// the solves' arrays (M doubles each): the operators' rows (L0's as the factors of its transpose, formed once per
-// launch), the three concurrent solutions, the c' of the two chi-dependent solves, the psi right-hand side.  In
-// LDS after the arrays above when the grid leaves room, else in the global workspace; then (global) the
-// particles' partial adjoints (5 x NP)
+// the solves' arrays (M doubles each): two buffers of the reduced system (sub-diagonal, diagonal,
+// super-diagonal, right-hand side) of the parallel cyclic reduction, the solutions of the slice's five solves
+// (two at a time) and the psi right-hand side.  In LDS after the arrays above when the grid leaves room, else in
+// the global workspace; then (global) the particles' partial adjoints (5 x NP)
 enum AdjSolve : int {
-    O_C0 = 0, O_MT0, O_CPT0, O_A1D, O_B1D, O_C1D, O_A1F, O_B1F, O_C1F,
-    T_MU0, T_MU1, T_MU2, T_CP0, T_CP1, T_RHS, S_NARR
+    P_A0 = 0, P_B0, P_C0, P_D0, P_A1, P_B1, P_C1, P_D1,
+    T_MU0, T_MU1, T_MU2, T_RHS, S_NARR
 };
 
 // ---------------------------------------------------------------- particle functions over any number type
@@ -191,50 +191,55 @@ __device__ __forceinline__ void scatter_gathered(const AGeo<double>& p, double l
     }
 }
 
-// T^T mu = rhs for the B+ operators of FieldSolver::solve (src/FieldSolver.cpp:242-261: Dirichlet rows are
-// identity rows, the other diagonals b - chi; row 0 is a Dirichlet row, row N too unless it is the flux row), one
-// lane, the recurrences of Thomas (src/Tridiag.cpp:111-126) on the transposed diagonals c[i-1], diag, a[i+1]
-__device__ void thomas_transposed_chi(const double* a, const double* b, const double* c, const double* chi, bool flux,
-                                      const double* rhs, double* mu, double* cp, int N) {
-    auto diag = [&](int j) { return (j == 0 || (j == N && !flux)) ? b[j] : b[j] - chi[j]; };
-    const double i0 = 1.0 / diag(0);
-    cp[0] = a[1] * i0;
-    double prev = rhs[0] * i0;
-    mu[0] = prev;
-    for (int i = 1; i <= N; ++i) {
-        const double sub = c[i - 1];
-        const double m = 1.0 / (diag(i) - sub * cp[i - 1]);
-        cp[i] = (i < N ? a[i + 1] : 0.0) * m;
-        prev = (rhs[i] - sub * prev) * m;
-        mu[i] = prev;
+// T^T mu = rhs for an operator of FieldSolver::solve (src/FieldSolver.cpp:242-261): Dirichlet rows (row 0 unless
+// axis_regular, row N unless flux) are identity rows, the other diagonals b - chi (chi may be null).  T^T has the
+// diagonals c[i-1], diag, a[i+1].  Parallel cyclic reduction over the whole workgroup: log2(M) steps, each
+// eliminating every row's neighbours at the current stride (the unknowns' coupling doubles in reach); the
+// adjoint needs the solution, not Thomas's rounding (src/Tridiag.cpp:111-126 is the forward's serial form).
+__device__ void pcr_transposed(const double* __restrict__ a, const double* __restrict__ b,
+                               const double* __restrict__ c, const double* chi, bool axis_regular, bool flux,
+                               const double* rhs, double* mu, double* const (&buf)[8], int N, int tid, int nth) {
+    const int n = N + 1;
+    double *A0 = buf[0], *B0 = buf[1], *C0 = buf[2], *D0 = buf[3];
+    double *A1 = buf[4], *B1 = buf[5], *C1 = buf[6], *D1 = buf[7];
+    for (int i = tid; i < n; i += nth) {
+        const bool dir = (i == N && !flux) || (i == 0 && !axis_regular);
+        A0[i] = i > 0 ? c[i - 1] : 0.0;
+        B0[i] = (dir || !chi) ? b[i] : b[i] - chi[i];
+        C0[i] = i < N ? a[i + 1] : 0.0;
+        D0[i] = rhs[i];
     }
-    for (int i = N - 1; i >= 0; --i) {
-        prev = mu[i] - cp[i] * prev;
-        mu[i] = prev;
+    __syncthreads();
+    for (int st = 1; st < n; st <<= 1) {
+        for (int i = tid; i < n; i += nth) {
+            const int im = i - st, ip = i + st;
+            double an = 0.0, bn = B0[i], cn = 0.0, dn = D0[i];
+            if (im >= 0) {
+                const double k1 = A0[i] / B0[im];
+                an = -A0[im] * k1;
+                bn -= C0[im] * k1;
+                dn -= D0[im] * k1;
+            }
+            if (ip < n) {
+                const double k2 = C0[i] / B0[ip];
+                cn = -C0[ip] * k2;
+                bn -= A0[ip] * k2;
+                dn -= D0[ip] * k2;
+            }
+            A1[i] = an;
+            B1[i] = bn;
+            C1[i] = cn;
+            D1[i] = dn;
+        }
+        __syncthreads();
+        double* t;
+        t = A0; A0 = A1; A1 = t;
+        t = B0; B0 = B1; B1 = t;
+        t = C0; C0 = C1; C1 = t;
+        t = D0; D0 = D1; D1 = t;
     }
-}
-// the transposed factors of L0 (no chi): mT_i = 1 / (b_i - c_{i-1} cpT_{i-1}), cpT_i = a_{i+1} mT_i
-__device__ void l0_transposed_factors(const double* a, const double* b, const double* c, double* mT, double* cpT,
-                                      int N) {
-    mT[0] = 1.0 / b[0];
-    cpT[0] = a[1] * mT[0];
-    for (int i = 1; i <= N; ++i) {
-        mT[i] = 1.0 / (b[i] - c[i - 1] * cpT[i - 1]);
-        cpT[i] = (i < N ? a[i + 1] : 0.0) * mT[i];
-    }
-}
-__device__ void l0_transposed_solve(const double* c, const double* mT, const double* cpT, const double* rhs, double* mu,
-                                    int N) {
-    double prev = rhs[0] * mT[0];
-    mu[0] = prev;
-    for (int i = 1; i <= N; ++i) {
-        prev = (rhs[i] - c[i - 1] * prev) * mT[i];
-        mu[i] = prev;
-    }
-    for (int i = N - 1; i >= 0; --i) {
-        prev = mu[i] - cpT[i] * prev;
-        mu[i] = prev;
-    }
+    for (int i = tid; i < n; i += nth) mu[i] = D0[i] / B0[i];
+    __syncthreads();
 }
 
 // out_j of D^T l for the radial gradient stencils (FieldSolver::gradient / gradient_odd, src/FieldSolver.cpp:263-281)
@@ -260,6 +265,15 @@ __device__ __forceinline__ double over_r_T(const double* ir, const double* l, in
 __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_args A, qz_adjoint_args J) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nth = blockDim.x;
+    unsigned long long t_last = wall_clock64();
+    // phase clocks (lane 0, after the phase's closing barrier)
+    auto phase = [&](int i) {
+        if (J.ticks && tid == 0) {
+            const unsigned long long now = wall_clock64();
+            atomicAdd(&J.ticks[i], now - t_last);
+            t_last = now;
+        }
+    };
     const int N = A.grid.N, M = N + 1;
     const int Mp = (M + 1) & ~1;
     auto LA = [&](int a) { return lds + static_cast<size_t>(a) * Mp; };
@@ -280,15 +294,8 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
     for (int j = tid; j < M; j += nth) {
         LA(L_R)[j] = A.grid.r[j];
         LA(L_IH)[j] = j < N ? A.grid.inv_h[j] : 0.0;
-        SA(O_C0)[j] = A.op_c[OP_L0][j];
-        SA(O_A1D)[j] = A.op_a[OP_L1D][j];
-        SA(O_B1D)[j] = A.op_b[OP_L1D][j];
-        SA(O_C1D)[j] = A.op_c[OP_L1D][j];
-        SA(O_A1F)[j] = A.op_a[OP_L1F][j];
-        SA(O_B1F)[j] = A.op_b[OP_L1F][j];
-        SA(O_C1F)[j] = A.op_c[OP_L1F][j];
     }
-    if (tid == 0) l0_transposed_factors(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], SA(O_MT0), SA(O_CPT0), N);
+    double* const pbuf[8] = {SA(P_A0), SA(P_B0), SA(P_C0), SA(P_D0), SA(P_A1), SA(P_B1), SA(P_C1), SA(P_D1)};
     __syncthreads();
     for (int k = J.k_hi - 1; k >= J.k_lo; --k) {
         const double* st = J.traj + static_cast<size_t>(k - J.k_lo) * 6 * NP;
@@ -309,6 +316,7 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
             for (int a = A_SR; a <= S_CHI; ++a) LA(a)[j] = 0.0;
         }
         __syncthreads();
+        phase(0);
         // ---- chi of the slice (the B+ operators' diagonal) and the adjoint of the slice's forces
         const bool pushed = k < K - 1;
         for (int gi = tid; gi < NP; gi += nth) {
@@ -355,13 +363,14 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
             }
         }
         __syncthreads();
+        phase(1);
         // ---- B+ (src/Simulation.cpp:535-591, m = 0: one pass): br = L1D(chi)^-1 (-S_i1),
         //      bi = L1F(chi, jz)^-1 (grad jz + S_r1)
-        if (tid == 0)
-            thomas_transposed_chi(SA(O_A1D), SA(O_B1D), SA(O_C1D), LA(S_CHI), false, LA(A_BR), SA(T_MU0), SA(T_CP0), N);
-        else if (tid == 64)
-            thomas_transposed_chi(SA(O_A1F), SA(O_B1F), SA(O_C1F), LA(S_CHI), true, LA(A_BI), SA(T_MU1), SA(T_CP1), N);
-        __syncthreads();
+        pcr_transposed(A.op_a[OP_L1D], A.op_b[OP_L1D], A.op_c[OP_L1D], LA(S_CHI), false, false, LA(A_BR), SA(T_MU0),
+                       pbuf, N, tid, nth);
+        pcr_transposed(A.op_a[OP_L1F], A.op_b[OP_L1F], A.op_c[OP_L1F], LA(S_CHI), false, true, LA(A_BI), SA(T_MU1),
+                       pbuf, N, tid, nth);
+        phase(2);
         {
             const double* mr = SA(T_MU0);
             double* mi = SA(T_MU1);
@@ -379,6 +388,7 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
                 LA(A_JZ)[j] = gradient_T(gm, g0, gp, ir, mi, j, N, false) + (j == N ? -A.flux_coef * mi[N] : 0.0);
         }
         __syncthreads();
+        phase(3);
         // ---- deposit_S: the state and the gathered W+, Ez, Bz
         for (int gi = tid; gi < NP; gi += nth) {
             const double wi = st[5 * NP + gi];
@@ -409,17 +419,18 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
             }
         }
         __syncthreads();
+        phase(4);
         // ---- Ez, Bz and psi (src/Simulation.cpp:477-532, 797-819): bz = L0^-1 t3, t3 = -(grad_odd + over_r) jp_i1;
         //      ez = L0^-1 (-drho); wr = -grad psi, psi = L0^-1 (-rhot)
         for (int j = tid; j < M; j += nth) SA(T_RHS)[j] = -gradient_T(gm, g0, gp, ir, LA(A_WR), j, N, false);
         __syncthreads();
-        if (tid == 0)
-            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), LA(A_BZ), SA(T_MU0), N);
-        else if (tid == 64)
-            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), LA(A_EZ), SA(T_MU1), N);
-        else if (tid == 128)
-            l0_transposed_solve(SA(O_C0), SA(O_MT0), SA(O_CPT0), SA(T_RHS), SA(T_MU2), N);
-        __syncthreads();
+        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_BZ), SA(T_MU0), pbuf,
+                       N, tid, nth);
+        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_EZ), SA(T_MU1), pbuf,
+                       N, tid, nth);
+        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, SA(T_RHS), SA(T_MU2), pbuf,
+                       N, tid, nth);
+        phase(5);
         if (tid == 0) SA(T_MU0)[N] = 0.0;   // L0's row N is a Dirichlet row
         __syncthreads();
         for (int j = tid; j < M; j += nth) {
@@ -429,6 +440,7 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
             LA(A_JPI)[j] = -(gradient_T(gm, g0, gp, ir, lt, j, N, true) + over_r_T(ir, lt, j));
         }
         __syncthreads();
+        phase(6);
         // ---- the beam sources (combine_sources, src/Simulation.cpp:408-457) and the plasma deposit
         for (int j = tid; j < M; j += nth) {
             auto LB = [&](int kk, int comp) -> double& {
@@ -471,6 +483,7 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
             for (int v = 0; v < 5; ++v) J.ring[static_cast<size_t>(v) * NP + gi] = Lk[v];
         }
         __syncthreads();
+        phase(7);
     }
 }
 

@@ -47,7 +47,8 @@ def test_adjoint_equals_oracle_adjoint(qa, oracle, c4_small, regime):
     s = qa.Simulation(c4_small, ov)
     assert s.profile_size(1) == NSEG and s.profile_size(0) == 0
     J, g, ms = s.spread_grad(1, every=64)
-    print(f"{regime}: J {J:.15e} forward {ms[0]:.1f} ms reverse {ms[1]:.1f} ms")
+    print(f"{regime}: J {J:.15e} forward {ms[0]:.1f} ms reverse {ms[1]:.1f} ms, reverse-sweep phases (ms):",
+          {k: round(v / 1e5, 2) for k, v in s.adjoint_ticks().items()})
     assert J == pytest.approx(Jo, rel=1e-13)
     assert np.abs(g - go).max() <= 1e-9 * np.abs(go).max(), (g, go)
     # the step itself is the forward's: fields and the pushed beams equal the oracle's after its step
