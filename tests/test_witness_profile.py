@@ -105,7 +105,7 @@ def test_batched_objective_and_fd_gradient(qa, oracle):
     from quarz_amd import witness
     v = np.array(configs.c4_initial_profile())
     deck = configs.c4_deck(profile=v)
-    obj = witness.SpreadObjective(deck, beam=1, nseg=S)
+    obj = witness.SpreadObjective(deck, beam=1, nseg=S, method="fd")
     eps = 1e-6
     J0, g = obj.value_and_grad(v, eps)
     plain = qa.Simulation(deck)
@@ -135,10 +135,12 @@ def _oracle_J(oracle, deck, v, s, step):
 
 
 @pytest.mark.gpu
-def test_adam_lowers_the_spread(qa):
-    """A few Adam iterations on the profile, with the finite-difference gradient behind torch.autograd."""
+@pytest.mark.parametrize("method", ["adjoint", "fd"])
+def test_adam_lowers_the_spread(qa, method):
+    """A few Adam iterations on the profile, with the adjoint's or the finite-difference gradient behind
+    torch.autograd."""
     from quarz_amd import witness
-    obj = witness.SpreadObjective(configs.c4_deck(), beam=1, nseg=S)
+    obj = witness.SpreadObjective(configs.c4_deck(), beam=1, nseg=S, method=method)
     v, hist = witness.optimise(obj, configs.c4_initial_profile(), iters=4, lr=0.05)
     obj.close()
     assert hist[-1] < hist[0], hist
