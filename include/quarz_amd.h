@@ -354,8 +354,9 @@ void qz_batch_destroy(qz_batch* b);
 
 /* ---- launch shapes of the beam kernels (not in the reference) ----
  * Threads per block (a multiple of 64 in [64, 1024]) of: the beam push (Beam::advance), the deposit's
- * per-particle kernels (cell keys and scatter), its bucket sort and its per-node sums.  One table for the
- * process, 256 each until set.  Results do not depend on the shapes. */
+ * per-particle kernel (cell keys and records), a slot kept for the bucket sort (the bucket order now comes from
+ * a radix sort of fixed shape, so this entry has no effect) and its per-node sums.  One table for the process,
+ * 256 each until set.  Results do not depend on the shapes. */
 enum { QZ_K_BEAM_ADVANCE = 0, QZ_K_BEAM_PARTICLES = 1, QZ_K_BEAM_SORT = 2, QZ_K_BEAM_NODES = 3, QZ_K_COUNT = 4 };
 int qz_set_block(int kernel, int threads);
 int qz_get_block(int kernel);

@@ -179,27 +179,142 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     atomicAdd(&counts[kk], 1);
 }
 
-// deposit step 2: scatter the particle indices into their buckets (order fixed in step 3)
-__global__ void beam_scatter_kernel(int np, const int* __restrict__ key, const int* __restrict__ start,
-                                    int* __restrict__ fill, int* __restrict__ list) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const int kk = key[i];
-    if (kk < 0) return;
-    const int pos = start[kk] + atomicAdd(&fill[kk], 1);
-    list[pos] = i;
+// deposit steps 2-3: the deposited particles' indices grouped by bucket, each bucket in particle-index order --
+// the order of the serial deposit's sums.  A stable least-significant-digit radix sort of the (bucket key,
+// particle index) pairs: the pairs are first compacted in index order, then every pass is a stable counting
+// sort of one 8-bit digit of the key, so pairs with equal keys keep their index order by construction.
+// Blocks of kRThreads threads own tiles of kRTile consecutive pairs; within a tile, ranks come from wave
+// ballots over the digit's bits and a prefix over the waves of each 256-pair chunk.
+constexpr int kRThreads = 256;
+constexpr int kRTile = 4096;
+constexpr int kRWaves = kRThreads / 64;
+
+// rank of this lane among the lanes below it with the same `digit` (valid lanes only), and how many lanes of the
+// wave share it; nbits: the digit's width
+__device__ __forceinline__ void wave_match_rank(int digit, bool valid, int nbits, int& rank, int& count) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long mask = __ballot(valid);
+    for (int b = 0; b < nbits; ++b) {
+        const bool bit = valid && ((digit >> b) & 1);
+        const unsigned long long bb = __ballot(bit);
+        mask &= bit ? bb : ~bb;
+    }
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    rank = valid ? __popcll(mask & lt) : 0;
+    count = valid ? __popcll(mask) : 0;
 }
 
-// deposit step 3: order every bucket by particle index
-__global__ void beam_bucket_sort_kernel(int ncells, const int* __restrict__ start, int* __restrict__ list) {
-    const int cidx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cidx >= ncells) return;
-    const int lo = start[cidx], hi = start[cidx + 1];
-    for (int a = lo + 1; a < hi; ++a) {
-        const int v = list[a];
-        int b = a - 1;
-        while (b >= lo && list[b] > v) { list[b + 1] = list[b]; --b; }
-        list[b + 1] = v;
+// per tile: the number of deposited particles (key >= 0)
+__global__ void __launch_bounds__(kRThreads) compact_count_kernel(int np, const int* __restrict__ key,
+                                                                  int* __restrict__ tile_count) {
+    __shared__ int sh[kRWaves];
+    const int t = threadIdx.x, w = t >> 6;
+    const long base = static_cast<long>(blockIdx.x) * kRTile;
+    int c = 0;
+    for (int i = t; i < kRTile; i += kRThreads) {
+        const long e = base + i;
+        c += (e < np && key[e] >= 0) ? 1 : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o);
+    if ((t & 63) == 0) sh[w] = c;
+    __syncthreads();
+    if (t == 0) {
+        int tot = 0;
+        for (int q = 0; q < kRWaves; ++q) tot += sh[q];
+        tile_count[blockIdx.x] = tot;
+    }
+}
+
+// the (key, index) pairs of the deposited particles, in index order, at tile_off[tile] + rank in the tile
+__global__ void __launch_bounds__(kRThreads) compact_write_kernel(int np, const int* __restrict__ key,
+                                                                  const int* __restrict__ tile_off,
+                                                                  int* __restrict__ kout, int* __restrict__ vout) {
+    __shared__ int wc[kRWaves];
+    __shared__ int run;
+    const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+    const long base = static_cast<long>(blockIdx.x) * kRTile;
+    if (t == 0) run = tile_off[blockIdx.x];
+    __syncthreads();
+    for (int c0 = 0; c0 < kRTile; c0 += kRThreads) {
+        const long e = base + c0 + t;
+        const int kk = e < np ? key[e] : -1;
+        const bool v = kk >= 0;
+        const unsigned long long m = __ballot(v);
+        const int rank = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wc[w] = __popcll(m);
+        __syncthreads();
+        int pre = run;
+        for (int q = 0; q < w; ++q) pre += wc[q];
+        if (v) {
+            kout[pre + rank] = kk;
+            vout[pre + rank] = static_cast<int>(e);
+        }
+        __syncthreads();
+        if (t == 0) {
+            int tot = 0;
+            for (int q = 0; q < kRWaves; ++q) tot += wc[q];
+            run += tot;
+        }
+        __syncthreads();
+    }
+}
+
+// one radix pass, step 1: digit counts per tile, stored digit-major (hist[d * ntiles + tile])
+__global__ void __launch_bounds__(kRThreads) radix_hist_kernel(const int* __restrict__ nvalid,
+                                                               const int* __restrict__ kin, int shift, int ntiles,
+                                                               int* __restrict__ hist) {
+    __shared__ int h[256];
+    const int t = threadIdx.x;
+    h[t] = 0;
+    __syncthreads();
+    const int n = *nvalid;
+    const long base = static_cast<long>(blockIdx.x) * kRTile;
+    for (int i = t; i < kRTile; i += kRThreads) {
+        const long e = base + i;
+        if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255], 1);
+    }
+    __syncthreads();
+    hist[static_cast<size_t>(t) * ntiles + blockIdx.x] = h[t];
+}
+
+// one radix pass, step 2 (after the exclusive scan of hist into off): the stable scatter
+__global__ void __launch_bounds__(kRThreads) radix_scatter_kernel(const int* __restrict__ nvalid,
+                                                                  const int* __restrict__ kin,
+                                                                  const int* __restrict__ vin, int shift, int ntiles,
+                                                                  const int* __restrict__ off,
+                                                                  int* __restrict__ kout, int* __restrict__ vout) {
+    __shared__ int running[256];
+    __shared__ int wc[kRWaves][256];
+    const int t = threadIdx.x, w = t >> 6;
+    running[t] = off[static_cast<size_t>(t) * ntiles + blockIdx.x];
+    for (int q = 0; q < kRWaves; ++q) wc[q][t] = 0;
+    __syncthreads();
+    const int n = *nvalid;
+    const long base = static_cast<long>(blockIdx.x) * kRTile;
+    for (int c0 = 0; c0 < kRTile; c0 += kRThreads) {
+        if (base + c0 >= n) break;   // uniform over the block
+        const long e = base + c0 + t;
+        const bool v = e < n;
+        const int kk = v ? kin[e] : 0;
+        const int d = (kk >> shift) & 255;
+        int rank, cnt;
+        wave_match_rank(d, v, 8, rank, cnt);
+        if (v && rank == 0) wc[w][d] = cnt;
+        __syncthreads();
+        if (v) {
+            int pos = running[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wc[q][d];
+            kout[pos] = kk;
+            vout[pos] = vin[e];
+        }
+        __syncthreads();
+        int add = 0;
+        for (int q = 0; q < kRWaves; ++q) {
+            add += wc[q][t];
+            wc[q][t] = 0;
+        }
+        running[t] += add;
+        __syncthreads();
     }
 }
 
@@ -348,24 +463,48 @@ using namespace qz;
 
 namespace {
 struct DepWork {   // layout of the deposit workspace
-    size_t key, rec, counts, start, fill, list, bsum, total, bytes;
+    size_t key, rec, counts, start, list, bsum, total, kc, kt, vt, tcount, toff, hist, hoff, hbsum, htotal, bytes;
+    int ntiles;
 };
 DepWork dep_layout(int np, int nloc, int N) {
     auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
     const size_t nc = static_cast<size_t>(nloc + 1) * N;
     const size_t nb = (nc + 1 + 1023) / 1024;
     DepWork d;
+    d.ntiles = static_cast<int>((static_cast<size_t>(np) + kRTile - 1) / kRTile);
+    const size_t nt = static_cast<size_t>(d.ntiles > 0 ? d.ntiles : 1);
+    const size_t nh = 256 * nt + 1;
     size_t o = 0;
     d.key = o; o = al(o + 4 * static_cast<size_t>(np));
     d.rec = o; o = al(o + 8 * static_cast<size_t>(R_N) * np);
-    d.counts = o; o = al(o + 4 * (nc + 1));   // counts and fill adjacent: one fill kernel zeroes both
-    d.fill = o; o = al(o + 4 * (nc + 1));
+    d.counts = o; o = al(o + 4 * (nc + 1));
     d.start = o; o = al(o + 4 * (nc + 1));
     d.list = o; o = al(o + 4 * static_cast<size_t>(np));
     d.bsum = o; o = al(o + 4 * nb);
     d.total = o; o = al(o + 4);
+    d.kc = o; o = al(o + 4 * static_cast<size_t>(np));    // radix sort: keys, two buffers
+    d.kt = o; o = al(o + 4 * static_cast<size_t>(np));
+    d.vt = o; o = al(o + 4 * static_cast<size_t>(np));    // the second value buffer (the first is `list`)
+    d.tcount = o; o = al(o + 4 * (nt + 1));
+    d.toff = o; o = al(o + 4 * (nt + 1));
+    d.hist = o; o = al(o + 4 * nh);
+    d.hoff = o; o = al(o + 4 * nh);
+    d.hbsum = o; o = al(o + 4 * ((std::max(nh, nt + 1) + 1023) / 1024 + 1));
+    d.htotal = o; o = al(o + 4);
     d.bytes = o;
     return d;
+}
+
+// exclusive scan of n ints (one block for short arrays, three launches otherwise); bsum: (n + 1023) / 1024 ints
+void scan_exclusive(const int* in, int n, int* out, int* bsum, int* total, hipStream_t st) {
+    const int nb = (n + 1023) / 1024;
+    if (nb <= kScanOne) {
+        hipLaunchKernelGGL(scan_one_block, dim3(1), dim3(1024), 0, st, in, n, out);
+    } else {
+        hipLaunchKernelGGL(scan_block_sums, dim3(nb), dim3(1024), 0, st, in, n, bsum);
+        hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
+        hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, in, n, bsum, out);
+    }
 }
 }  // namespace
 
@@ -464,31 +603,53 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
     Real* rec = reinterpret_cast<Real*>(w + L.rec);
     int* counts = reinterpret_cast<int*>(w + L.counts);
     int* start = reinterpret_cast<int*>(w + L.start);
-    int* fill = reinterpret_cast<int*>(w + L.fill);
     int* list = reinterpret_cast<int*>(w + L.list);
     int* bsum = reinterpret_cast<int*>(w + L.bsum);
     int* total = reinterpret_cast<int*>(w + L.total);
     const int ncells = (box->nloc + 1) * grid->N;
     {
-        const int rc = qz_zero_async(counts, L.fill - L.counts + 4 * (static_cast<size_t>(ncells) + 1), st);
+        const int rc = qz_zero_async(counts, 4 * (static_cast<size_t>(ncells) + 1), st);
         if (rc) return rc;
     }
-    const int tb = g_block[QZ_K_BEAM_PARTICLES], tsort = g_block[QZ_K_BEAM_SORT], tnode = g_block[QZ_K_BEAM_NODES];
+    const int tb = g_block[QZ_K_BEAM_PARTICLES], tnode = g_block[QZ_K_BEAM_NODES];
     for (int b = 0, i0 = 0; b < nbeams; i0 += np[b], ++b)
         if (np[b] > 0)
             hipLaunchKernelGGL(beam_key_kernel, dim3((np[b] + tb - 1) / tb), dim3(tb), 0, st, *grid, *box, q[b], np[b], p[b],
                                i0, pstep ? pstep[b] : nullptr, marker, key, rec, counts, live);
     const int n1 = ncells + 1;   // counts[ncells] = 0 -> start[ncells] = total
-    const int nb = (n1 + 1023) / 1024;
-    if (nb <= kScanOne) {
-        hipLaunchKernelGGL(scan_one_block, dim3(1), dim3(1024), 0, st, counts, n1, start);
-    } else {
-        hipLaunchKernelGGL(scan_block_sums, dim3(nb), dim3(1024), 0, st, counts, n1, bsum);
-        hipLaunchKernelGGL(scan_sums_serial, dim3(1), dim3(1024), 0, st, bsum, nb, total);
-        hipLaunchKernelGGL(scan_finish, dim3(nb), dim3(1024), 0, st, counts, n1, bsum, start);
+    scan_exclusive(counts, n1, start, bsum, total, st);
+    // the bucket lists, each in particle-index order: compaction, then LSD radix passes over the key's digits
+    {
+        const int nt = L.ntiles;
+        int* kc = reinterpret_cast<int*>(w + L.kc);
+        int* kt = reinterpret_cast<int*>(w + L.kt);
+        int* vt = reinterpret_cast<int*>(w + L.vt);
+        int* tcount = reinterpret_cast<int*>(w + L.tcount);
+        int* toff = reinterpret_cast<int*>(w + L.toff);
+        int* hist = reinterpret_cast<int*>(w + L.hist);
+        int* hoff = reinterpret_cast<int*>(w + L.hoff);
+        int* hbsum = reinterpret_cast<int*>(w + L.hbsum);
+        int* htotal = reinterpret_cast<int*>(w + L.htotal);
+        int bits = 0;
+        while ((1 << bits) < ncells) ++bits;
+        const int passes = (bits + 7) / 8;
+        // the values end in `list` after the last pass
+        int* vA = passes % 2 == 0 ? list : vt;
+        int* vB = passes % 2 == 0 ? vt : list;
+        hipLaunchKernelGGL(compact_count_kernel, dim3(nt), dim3(kRThreads), 0, st, npt, key, tcount);
+        scan_exclusive(tcount, nt, toff, hbsum, htotal, st);
+        hipLaunchKernelGGL(compact_write_kernel, dim3(nt), dim3(kRThreads), 0, st, npt, key, toff, kc, vA);
+        const int* nvalid = start + ncells;
+        int *kin = kc, *kout = kt, *vin = vA, *vout = vB;
+        for (int p = 0; p < passes; ++p) {
+            hipLaunchKernelGGL(radix_hist_kernel, dim3(nt), dim3(kRThreads), 0, st, nvalid, kin, 8 * p, nt, hist);
+            scan_exclusive(hist, 256 * nt, hoff, hbsum, htotal, st);
+            hipLaunchKernelGGL(radix_scatter_kernel, dim3(nt), dim3(kRThreads), 0, st, nvalid, kin, vin, 8 * p, nt,
+                               hoff, kout, vout);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
+        }
     }
-    hipLaunchKernelGGL(beam_scatter_kernel, dim3((npt + tb - 1) / tb), dim3(tb), 0, st, npt, key, start, fill, list);
-    hipLaunchKernelGGL(beam_bucket_sort_kernel, dim3((ncells + tsort - 1) / tsort), dim3(tsort), 0, st, ncells, start, list);
     const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
     hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tnode - 1) / tnode)), dim3(tnode), 0, st, *grid, *box,
                        modes == 1 ? 1 : 0, start, list, rec, bsrc, nc);
