@@ -6,7 +6,9 @@ A step is one quasi-static time step of the workload (quarz_amd/configs.py): bea
 over every slice (one persistent kernel), the beam push.  W untimed steps one at a time (at least one: the
 first starts the leapfrog), then exactly K timed steps between device synchronisations.  The timed steps run
 pipelined (Simulation.run_pipelined: up to --depth steps in flight, each split into --chunks slice ranges,
-bit-identical to stepping one at a time); --depth 0 steps them one at a time.  Prints one JSON line:
+bit-identical to stepping one at a time); --depth 0 steps them one at a time.  --groups sets the workgroups of
+each step's sweep (the c2 default when pipelined: 4, so the 20 steps in flight hold 80 compute units; the
+eager steps then run the same kernel cooperatively).  Prints one JSON line:
   value          plasma particle-slices per second of the whole timed steps (wall clock)
   plain_value    the same run stepped one step at a time (--plain-steps steps), for comparison
   beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
@@ -38,10 +40,11 @@ HBM_PEAK = 8.0e12
 FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
 # same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
-MEASURED_TRAFFIC = {"c2": 841.9e6}
-# FP64 VALU lane operations per sweep launch, 64 x SQ_INSTS_VALU_FLOPS_FP64 (profiles/sweep_fp64_valu.md):
-# c2 one run, c4 the 33-run batched launch
-MEASURED_VALU = {"c2": 2.33e10, "c4": 1.33e11}
+# Keyed by (workload, workgroups per run); per whole sweep (every slice of one step).
+MEASURED_TRAFFIC = {("c2", 1): 841.9e6, ("c2", 4): 20163.2e6}
+# FP64 VALU lane operations per sweep (every slice of one step), 64 x SQ_INSTS_VALU_FLOPS_FP64
+# (profiles/sweep_fp64_valu.md): c2 one run, c4 the 33-run batched launch
+MEASURED_VALU = {("c2", 1): 2.33e10, ("c2", 4): 2.481e10, ("c4", 1): 1.33e11}
 
 
 def physical_cores():
@@ -345,7 +348,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=1, help="c5: steps of the one-launch-per-run comparison")
     ap.add_argument("--depth", type=int, default=-1,
                     help="time steps in flight (-1: min(steps, 32); 0: one step at a time, no pipeline)")
-    ap.add_argument("--chunks", type=int, default=128, help="slice ranges per step of the pipelined run")
+    ap.add_argument("--chunks", type=int, default=96, help="slice ranges per step of the pipelined run")
+    ap.add_argument("--groups", type=int, default=-1,
+                    help="workgroups per run of the sweep (0: the run's own choice; -1: 4 for a pipelined c2 run, "
+                         "whose 20 steps in flight then hold 80 of the 256 compute units, else the run's own)")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
@@ -380,12 +386,15 @@ def main():
     depth = min(args.steps, 32) if args.depth < 0 else args.depth
     total = warmup + args.steps + 1
     t_setup = time.perf_counter()
-    sim = quarz_amd.Simulation(deck, {"time.steps": total})
+    if args.groups < 0:
+        args.groups = 4 if (pipelined and args.config == "c2") else 0
+    over = {"sweep.groups": args.groups} if args.groups > 0 else {}
+    sim = quarz_amd.Simulation(deck, {"time.steps": total, **over})
     print(f"# {args.config} set up in {time.perf_counter() - t_setup:.1f} s ({sim.groups} workgroups per run)",
           file=sys.stderr, flush=True)
     modes = sim.m1
     group_times = None
-    if sim.groups > 1:   # workgroups per run for the sweep, timed on the first slices of the first step
+    if sim.groups > 1 and args.groups == 0:   # workgroups per run, timed on the first slices of the first step
         group_times = sim.tune_groups([8, 16, 32], nslices=64)
     shapes = sim.autotune(cache=args.tune_cache)   # launch shapes of the beam kernels for this run
     # warm-up steps, one at a time (the first starts the leapfrog); the last one's HIP-event kernel times
@@ -427,7 +436,7 @@ def main():
     # the same run stepped one step at a time, for comparison
     plain_value = None
     if pipelined and args.plain_steps > 0:
-        ps = quarz_amd.Simulation(deck, {"time.steps": args.plain_steps + 2})
+        ps = quarz_amd.Simulation(deck, {"time.steps": args.plain_steps + 2, **over})
         ps.step()
         torch.cuda.synchronize()
         tp = time.perf_counter()
@@ -443,6 +452,7 @@ def main():
     # (pipelined: flops of one step's slice ranges over the sum of their launch times)
     t_kernel = (timed_sweep_ms if timed_sweep_ms is not None else sweep_ms / args.steps) / 1e3
     achieved = flops_per_sweep / t_kernel
+    mkey = (args.config, sim.groups)   # the counter readings of this kernel and shape
     out = {
         "metric": METRIC,
         "value": psl / wall,
@@ -459,8 +469,9 @@ def main():
         "config": {"workload": f"{args.config}: {configs.DESCRIPTION[args.config]}", "M": sim.M, "slices": sim.K,
                    "plasma_particles_per_slice": int(round(psl / args.steps / max(sim.K, 1))),
                    "beam_particles": int(round(pushes / args.steps)),
-                   "parallelism": (f"one run per GPU; {depth} time steps in flight, {args.chunks} slice ranges per step"
-                                   if pipelined else "one run per GPU, one step at a time")},
+                   "parallelism": (f"one run per GPU; {depth} time steps in flight, {args.chunks} slice ranges per step, "
+                                   f"{sim.groups} workgroup(s) per step's sweep"
+                                   if pipelined else f"one run per GPU, one step at a time, {sim.groups} workgroup(s)")},
         "plain_value": plain_value,
         "beam_pushes_per_s": pushes / wall,
         "sweep_ms": sweep_ms / args.steps,
@@ -472,14 +483,16 @@ def main():
         "sweep_groups": sim.groups,
         "sweep_group_times_ms": group_times,
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK, "traffic": MEASURED_TRAFFIC.get(args.config),
+                     "frac": achieved / FP64_PEAK,
+                     "traffic": (MEASURED_TRAFFIC[mkey] / launches if mkey in MEASURED_TRAFFIC else None),
+                     "traffic_per_sweep": MEASURED_TRAFFIC.get(mkey),
                      "kernel": "sweep_kernel" if sim.groups == 1 else f"sweep_multi_kernel ({sim.groups} workgroups)",
                      "launches_per_step": launches,
                      "avg_launch_ms": t_kernel * 1e3 / launches,
                      "flops_per_launch": flops_per_sweep / launches,
-                     "valu_fp64_lane_ops": MEASURED_VALU.get(args.config),
-                     "valu_frac": (MEASURED_VALU[args.config] / t_kernel / FP64_PEAK
-                                   if args.config in MEASURED_VALU and t_kernel > 0 else None),
+                     "valu_fp64_lane_ops_per_sweep": MEASURED_VALU.get(mkey),
+                     "valu_frac": (MEASURED_VALU[mkey] / t_kernel / FP64_PEAK
+                                   if mkey in MEASURED_VALU and t_kernel > 0 else None),
                      "aggregate_achieved": flops_per_sweep * args.steps / wall / 1e12,
                      "aggregate_frac": flops_per_sweep * args.steps / wall / FP64_PEAK,
                      "note": "FP64 vector work, no dense contraction (no MFMA); MI355X's FP64 vector peak is 78.6 "

@@ -192,8 +192,9 @@ int qz_beam_sums_device(int np, const qz_beam_soa* p, double gref, double xiref,
  * call (so that call is not capturable in a graph). */
 size_t qz_sweep_workspace_doubles(int M);
 /* Workspace of a run swept by `groups` cooperating workgroups (qz_sweep_args.groups > 1): M nodes, np mobile plasma
- * particles (all species), modes 0 or 1.  The workgroups are resident together (a cooperative launch): a launch of
- * more workgroups than the device holds at once is refused with QZ_ERANGE. */
+ * particles (all species), modes 0 or 1.  The workgroups must be resident together: a launch of more workgroups
+ * than the device holds at once (one per compute unit) is refused with QZ_ERANGE, and a grid barrier that is not
+ * reached within two seconds sets the run's error flag (mws word 2) and ends the launch. */
 size_t qz_sweep_multi_workspace_doubles(int M, int np, int groups, int modes);
 int qz_sweep(const qz_sweep_args* args, void* stream);
 

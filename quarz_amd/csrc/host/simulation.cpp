@@ -1101,6 +1101,7 @@ struct StepCtx {
     hipStream_t st = nullptr;
     std::vector<std::array<DevBuf<Real>, 7>> sp;   // per mobile species: x y px py dl w hist
     DevBuf<Real> ws, bsrc;
+    DevBuf<Real> mws;                               // runs of several workgroups: their workspace
     DevBuf<char> work;
     std::vector<hipEvent_t> pushed;                 // per slice range: this context's step pushed it
     ~StepCtx() {
@@ -1118,8 +1119,6 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         throw std::runtime_error("run_pipelined: the run must be started (every beam started, the same dt, no output "
                                  "directory, steps left)");
     if (step_n + nrun - 1 > nsteps) throw std::runtime_error("run_pipelined: more steps than time.steps leaves");
-    if (groups > 1) throw std::runtime_error("run_pipelined: runs of one workgroup only (this run has " +
-                                             std::to_string(groups) + ")");
     // ranges of at least two slices: a range's deposit then needs only the previous step's pushes up to the
     // next range (a particle in the deposit window has its gather slice below the range after next)
     nchunks = std::max(1, std::min(nchunks, K / 2));
@@ -1158,6 +1157,10 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         }
         cp->ws.alloc(ws.n);
         cp->bsrc.alloc(bsrc.n);
+        if (groups > 1) {   // the barrier counter and the error flag start at zero
+            cp->mws.alloc(mws.n);
+            qz_check(qz_zero_async(cp->mws.p, 64 * sizeof(Real), stream));
+        }
         cp->work.alloc(dep_bytes);
         cp->pushed.resize(nchunks);
         for (auto& e : cp->pushed) hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
@@ -1182,6 +1185,7 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
         qz_sweep_args A = sweep_args;
         A.bsrc = C.bsrc.p;
         A.ws = C.ws.p;
+        A.mws = groups > 1 ? C.mws.p : nullptr;
         A.lost = lostv.p + static_cast<size_t>(i) * QZ_MAX_SPECIES;
         A.stats = cntv.p + static_cast<size_t>(i) * 24;
         for (size_t s = 0; s < mob.size(); ++s) {
@@ -1198,6 +1202,14 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     }
     DevBuf<qz_sweep_args> dargs(hargs.size());
     hip_check(hipMemcpy(dargs.p, hargs.data(), sizeof(qz_sweep_args) * hargs.size(), hipMemcpyHostToDevice), "arguments");
+    // a range of a run of several workgroups: its block map (one run), and before each launch the barrier counter
+    // back to zero -- the error flag is kept, so a barrier that gave up in any range of a step is reported below
+    DevBuf<int> dmap;
+    if (groups > 1) {
+        const std::vector<int> map = qz_sweep_block_map(hargs.data(), 1);
+        dmap.alloc(map.size());
+        hip_check(hipMemcpy(dmap.p, map.data(), sizeof(int) * map.size(), hipMemcpyHostToDevice), "block map");
+    }
 
     const qz_radial_grid g = grid.view();
     const Real dkick = Real(0.5) * (dt_prev + dt), ddrift = dt;
@@ -1263,12 +1275,22 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             }
             const size_t ai = static_cast<size_t>(i) * nchunks + c;
             hip_check(hipEventRecord(lev.e[2 * ai], st), "event");
-            qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
+            if (groups > 1) {
+                char* m = reinterpret_cast<char*>(C.mws.p);
+                qz_check(qz_zero_async(m, 8, st));                      // barrier counter
+                qz_check(qz_zero_async(m + 12, 64 * sizeof(Real) - 12, st));
+                // steps in flight on their own streams: ordinary launches whose workgroups all fit the device
+                qz_check(qz_sweep_multi_launch(dargs.p + ai, dmap.p, hargs.data() + ai, 1, st, depth * groups));
+            } else {
+                qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
+            }
             hip_check(hipEventRecord(lev.e[2 * ai + 1], st), "event");
             if (!last) push_range(c);
         }
     }
     for (auto& cp : ctx) hip_check(hipStreamSynchronize(cp->st), "sync");
+    if (groups > 1)
+        for (auto& cp : ctx) check_multi_sweep(cp->mws, groups, cp->st);
 
     // host side of the steps: totals, the last step's counters, the run's arrays as after the last step
     std::vector<unsigned long long> hc;

@@ -28,7 +28,7 @@ int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_O
 bool qz_sweep_fits_one_group(bool m1, int N, int np);
 // G cooperating workgroups per run (qz_sweep_args.groups > 1); dev_map: block -> (run, group) pairs
 int qz_sweep_multi_launch(const qz_sweep_args* dev_runs, const int* dev_map, const qz_sweep_args* host_runs, int nruns,
-                          hipStream_t stream);
+                          hipStream_t stream, int resident = 0);
 // the block map of a multi-workgroup launch of these runs (2 ints per block)
 std::vector<int> qz_sweep_block_map(const qz_sweep_args* runs, int nruns);
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);

@@ -23,8 +23,9 @@
 //   7. the field store (node ranges per workgroup) and the Adams-Bashforth push of the owned particles.
 // The grid barrier is a counter in the run's workspace that only grows: barrier n of the launch completes when
 // it reaches n G.  Every wait is bounded in time: a barrier not reached within two seconds marks the run's error
-// flag and ends the launch in every workgroup, and the host reports it.  The launch is cooperative, so the
-// workgroups of a run are resident together.  Arithmetic per term is that of the one-workgroup kernel
+// flag and ends the launch in every workgroup, and the host reports it.  A launch is refused unless all its
+// workgroups (and those of the launches the caller keeps in flight beside it) fit the device together, one per
+// compute unit, so every workgroup is resident.  Arithmetic per term is that of the one-workgroup kernel
 // (qz_sweep.hip), so the result is the serial oracle's bit for bit.
 #include <hip/hip_runtime.h>
 
@@ -878,8 +879,14 @@ int qz_sweep_multi_pt(int np, int groups) {
     return per <= 2 * mBS ? 2 : (per <= 4 * mBS ? 4 : 8);
 }
 
+// An ordinary launch whose workgroups all fit the device at once: the launch's own, or `resident` workgroups when
+// the caller keeps several such launches in flight on its streams (R <= the workgroups the device holds with this
+// kernel's LDS, one per compute unit).  Every workgroup then gets its compute unit as the ones before it finish,
+// and launches on different streams overlap (cooperative launches run one at a time on this runtime, and could
+// not be issued once many streams held the hardware queues).  A barrier that is not reached in time ends the
+// launch with the run's error flag set, never a hang.
 int qz_sweep_multi_launch(const qz_sweep_args* dev_runs, const int* dev_map, const qz_sweep_args* host_runs, int nruns,
-                          hipStream_t stream) {
+                          hipStream_t stream, int resident) {
     int nblocks = 0, maxN = 0, pt = 2;
     const bool m1 = host_runs[0].modes == 1;
     for (int i = 0; i < nruns; ++i) {
@@ -905,12 +912,12 @@ int qz_sweep_multi_launch(const qz_sweep_args* dev_runs, const int* dev_map, con
         if (e != hipSuccess) return qz_fail_hip(e, "occupancy");
         QZ_HIP(hipGetDevice(&dev));
         QZ_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        if (nblocks > per_cu * ncu)
-            return qz_fail(QZ_ERANGE, "sweep: " + std::to_string(nblocks) + " workgroups cannot be resident together (" +
+        const int need = std::max(resident, nblocks);
+        if (need > per_cu * ncu)
+            return qz_fail(QZ_ERANGE, "sweep: " + std::to_string(need) + " workgroups cannot be resident together (" +
                                           std::to_string(per_cu * ncu) + " fit)");
-        void* kargs[2] = {const_cast<qz_sweep_args**>(&dev_runs), const_cast<int**>(&dev_map)};
-        e = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(nblocks), dim3(mBS), kargs, mLds, stream);
-        if (e != hipSuccess) return qz_fail_hip(e, "multi-workgroup sweep launch");
+        hipLaunchKernelGGL(kern, dim3(nblocks), dim3(mBS), mLds, stream, dev_runs, dev_map);
+        QZ_HIP(hipGetLastError());
         return QZ_OK;
     };
     if (pt == 2) return m1 ? launch(sweep_multi_kernel<true, 2>) : launch(sweep_multi_kernel<false, 2>);

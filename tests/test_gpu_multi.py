@@ -90,3 +90,32 @@ def test_batched_multi_runs_equal_plain(qa, decks):
     for i, (a, b) in enumerate(zip(batch, plain)):
         for w in ("fld", "diag", "bsrc"):
             assert _same(getattr(a, w), getattr(b, w)), (i, w)
+
+
+@pytest.mark.parametrize("deck,ov,groups,steps,chunks,depth", [
+    # a slow witness (gamma 3) slipping across several ranges per step, m = 0, 2 workgroups
+    ("gain.in", {"time.steps": 4, "driver.nparticles": 20000, "xi.min": -8, "witness.gamma": 3,
+                 "witness.sigma_xi": 1.0, "witness.xi0": -4, "witness.n0": 0.001}, 2, 3, 32, 3),
+    # m = 1 mobile plasma, 3 workgroups, four steps in flight
+    ("betatron.in", {"modes": 1, "witness.x0": 0.05, "witness.nsym": 4, "time.steps": 6}, 3, 5, 6, 4),
+])
+def test_pipelined_groups_equal_plain(qa, decks, deck, ov, groups, steps, chunks, depth):
+    """Steps in flight whose sweeps run on several workgroups each (ordinary launches on the steps' streams,
+    every workgroup resident) against the same steps on one workgroup, one at a time: fields, diagnostics, beam
+    sources and every beam particle bit for bit."""
+    a = qa.Simulation(f"{decks}/{deck}", ov)
+    b = qa.Simulation(f"{decks}/{deck}", {**ov, "sweep.groups": groups})
+    assert a.groups == 1 and b.groups == groups
+    a.step()
+    b.step()
+    for _ in range(steps):
+        a.step()
+    b.run_pipelined(steps, chunks=chunks, depth=depth)
+    for w in ("fld", "diag", "bsrc"):
+        ga, gb = getattr(b, w), getattr(a, w)
+        assert _same(ga, gb), _report(f"pipelined {w}", ga, gb)
+    for bi in range(a.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            ga, gb = b.get(f"beam{bi}.{k}"), a.get(f"beam{bi}.{k}")
+            assert _same(ga, gb), _report(f"pipelined beam{bi}.{k}", ga, gb)
+    assert b.lost() == a.lost()
