@@ -22,7 +22,9 @@ int qz_fail_hip(hipError_t e, const std::string& where);
 constexpr int QZ_SWEEP_WS_ARRAYS = 56;
 constexpr int QZ_NOALLOC_MAX_NODES = 2049;
 size_t qz_sweep_workspace_doubles_impl(int M);
-int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
+// workers / queue: with a device counter `queue`, `workers` workgroups take the runs in order from it
+int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream,
+                    int workers = 0, unsigned int* queue = nullptr);
 int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_OK or a QZ_E* code
 // the one-workgroup sweep holds a run of np plasma particles on N cells (m1: modes 0 and 1)
 bool qz_sweep_fits_one_group(bool m1, int N, int np);
