@@ -131,6 +131,13 @@ def bench_batched(args):
     big = max(range(len(runs)), key=lambda i: runs[i][1]["nb"])
     shapes = sims[big].autotune(cache=args.tune_cache)
     batch = quarz_amd.Batch(sims, graph=True)
+    plan = None
+    if args.balance:   # workgroups per run, from per-shape probes of the first slices
+        tb = time.perf_counter()
+        plan = batch.balance()
+        plan["seconds"] = time.perf_counter() - tb
+        print(f"# balanced: estimate {plan['est_ms']:.0f} ms (one workgroup per run {plan['est_one_ms']:.0f} ms) in "
+              f"{plan['seconds']:.1f} s", file=sys.stderr, flush=True)
     te = time.perf_counter()
     kern_ms = batch.step()   # eager: the batched sweep launch time
     torch.cuda.synchronize()
@@ -184,13 +191,21 @@ def bench_batched(args):
         "data": "synthetic: beams sampled from per-run seeds, plasma loaded at the box head",
         "config": {"workload": f"c5: {configs.DESCRIPTION['c5']}", "runs": len(sims),
                    "cells_min_median_max": [Ns[0], Ns[len(Ns) // 2], Ns[-1]],
-                   "slices_total": sum(s.K for s in sims), "parallelism": "one workgroup per run, one launch"},
+                   "slices_total": sum(s.K for s in sims),
+                   "parallelism": ("workgroups per run from the batch autotuner: one-workgroup runs queued on the "
+                                   "free compute units, multi-workgroup runs in a launch beside them") if plan else
+                                  "one workgroup per run, one launch"},
         "beam_pushes_per_s": pushes / wall,
         "batched_sweep_ms": kern_ms,
         "graph_step_ms": graph_ms / args.steps,
         "graph_replays": replays,
         "eager_step_ms": eager_wall * 1e3,
         "launch_shapes": shapes,
+        "balance": None if plan is None else {
+            "groups_histogram": {str(g): plan["groups"].count(g) for g in sorted(set(plan["groups"]))},
+            "multi_runs": [{"N": runs[i][1]["N"], "K": runs[i][1]["K"], "groups": g}
+                           for i, g in enumerate(plan["groups"]) if g > 1],
+            "est_ms": plan["est_ms"], "est_one_group_ms": plan["est_one_ms"], "tune_s": plan["seconds"]},
         "plain_value": ppsl / pwall if ppsl > 0 else None,
         "plain_steps": args.plain_steps,
         "batched_vs_plain": (psl / wall) / (ppsl / pwall) if ppsl > 0 else None,
@@ -303,6 +318,11 @@ def bench_gradient(args):
         "adam": {"iterations": iters, "lr": 0.02, "J_initial": hist[0], "J_final": float(J_final),
                  "profile_final": [float(x) for x in v]},
         "launch_shapes": shapes,
+        "balance": None if plan is None else {
+            "groups_histogram": {str(g): plan["groups"].count(g) for g in sorted(set(plan["groups"]))},
+            "multi_runs": [{"N": runs[i][1]["N"], "K": runs[i][1]["K"], "groups": g}
+                           for i, g in enumerate(plan["groups"]) if g > 1],
+            "est_ms": plan["est_ms"], "est_one_group_ms": plan["est_one_ms"], "tune_s": plan["seconds"]},
         "fd_plain_particle_slices_per_s": ppsl / pwall,
         "fd_gradient_batched_vs_plain": pwall / (fd_ms / 1e3),
         "adjoint_vs_batched_fd": (fd_ms / 1e3) / (wall / iters),
@@ -346,6 +366,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--runs", type=int, default=256, help="c5: number of independent runs")
     ap.add_argument("--plain-steps", type=int, default=1, help="c5: steps of the one-launch-per-run comparison")
+    ap.add_argument("--no-balance", dest="balance", action="store_false",
+                    help="c5: one workgroup per run (skip the batch autotuner)")
     ap.add_argument("--depth", type=int, default=-1,
                     help="time steps in flight (-1: min(steps, 32); 0: one step at a time, no pipeline)")
     ap.add_argument("--chunks", type=int, default=96, help="slice ranges per step of the pipelined run")
@@ -480,6 +502,11 @@ def main():
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
         "launch_shapes": shapes,
+        "balance": None if plan is None else {
+            "groups_histogram": {str(g): plan["groups"].count(g) for g in sorted(set(plan["groups"]))},
+            "multi_runs": [{"N": runs[i][1]["N"], "K": runs[i][1]["K"], "groups": g}
+                           for i, g in enumerate(plan["groups"]) if g > 1],
+            "est_ms": plan["est_ms"], "est_one_group_ms": plan["est_one_ms"], "tune_s": plan["seconds"]},
         "sweep_groups": sim.groups,
         "sweep_group_times_ms": group_times,
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",

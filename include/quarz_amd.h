@@ -288,9 +288,12 @@ int qz_sim_step_fields(qz_sim* s);   /* the sweep only */
  * stream is synchronised on return) and the argument block of its sweep, for a caller that launches the sweep
  * itself (qz_sweep, qz_sweep_noalloc) on the run's arrays.  The run's step counter does not advance. */
 int qz_sim_prepare_sweep(qz_sim* s, qz_sweep_args* out);
-/* One step of n independent runs (same modes) whose sweeps share one persistent launch, one workgroup per
- * run (qz_sweep_batched); beam deposits and pushes run per run.  batch_ms (optional) receives the time of
- * the batched sweep launch.  Not in the reference: it batches what Simulation::run does for one run. */
+/* One step of n independent runs (same modes) whose sweeps are launched together; beam deposits and pushes run
+ * per run.  The runs with one workgroup (their `groups`, see qz_sim_tune_groups and qz_batch_balance) share one
+ * persistent launch whose workgroups take them from a queue, longest first; the runs with several workgroups
+ * share one multi-workgroup launch beside it, on a second stream.  The workers plus the multi-workgroup runs'
+ * workgroups never exceed the compute units.  batch_ms (optional) receives the time from the first launch to
+ * the end of both.  Not in the reference: it batches what Simulation::run does for one run. */
 int qz_sim_step_batched(qz_sim** sims, int n, double* batch_ms);
 int qz_sim_push_beams(qz_sim* s);    /* the beam push of the current step */
 /* nrun steps of Simulation::run with up to `depth` steps in flight, each step split into nchunks slice ranges:
@@ -341,7 +344,7 @@ double qz_sim_time(const qz_sim* s);
 int qz_sim_write_fields(qz_sim* s, const char* path);
 
 /* ---- a fixed set of runs stepped together (not in the reference: it batches Simulation::run) ----
- * Every step runs all sweeps in one persistent launch (as qz_sim_step_batched).  With use_graph, once a step
+ * Every step launches all sweeps together (as qz_sim_step_batched).  With use_graph, once a step
  * repeats the device work of the step before in every run (every beam started, the same dt, a beam push to
  * do, no field output), the whole step -- each run's beam deposit and plasma reload, the batched sweep, each
  * run's beam push -- is captured once as a HIP graph and replayed on later steps.  step_ms receives the
@@ -352,6 +355,18 @@ int qz_batch_create(qz_sim** sims, int n, int use_graph, qz_batch** out);
 int qz_batch_step(qz_batch* b, double* step_ms);
 long qz_batch_replays(const qz_batch* b);   /* steps run as graph replays so far */
 void qz_batch_destroy(qz_batch* b);
+/* At most max_workers workgroups (0: every compute unit the multi-workgroup runs leave) take the batch's
+ * one-workgroup runs from the queue, e.g. to leave compute units to other work.  Results do not depend on it. */
+int qz_batch_set_workers(qz_batch* b, int max_workers);
+/* Autotuner of the batch's workgroups per run, before the first step.  The runs of one shape (modes, cells,
+ * plasma particles) are timed on one of them, the one with the most slices: its first nslices slices with each
+ * candidate count, best of two launches.  Each run's sweep is estimated at (time per slice) x (its slices); a
+ * plan gives every run whose one-workgroup estimate exceeds a target the fewest workgroups that bring it under
+ * the target and list-schedules the other runs, longest first, on the remaining compute units; every estimate
+ * is tried as the target and the plan with the smallest estimated sweep is applied to the runs' `groups`.
+ * groups_out[n] receives each run's count; est_ms[2] the plan's estimate and the estimate with one workgroup
+ * per run (-1 when some run cannot take one).  Results do not depend on the counts. */
+int qz_batch_balance(qz_batch* b, const int* candidates, int ncand, int nslices, int* groups_out, double* est_ms);
 
 /* ---- launch shapes of the beam kernels (not in the reference) ----
  * Threads per block (a multiple of 64 in [64, 1024]) of: the beam push (Beam::advance), the deposit's

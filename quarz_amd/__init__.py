@@ -99,6 +99,8 @@ def lib():
         L.qz_sim_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, DP]
         L.qz_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.POINTER(C.c_void_p)]
         L.qz_batch_step.argtypes = [C.c_void_p, DP]
+        L.qz_batch_balance.argtypes = [C.c_void_p, IP, C.c_int, C.c_int, IP, DP]
+        L.qz_batch_set_workers.argtypes = [C.c_void_p, C.c_int]
         L.qz_batch_replays.argtypes = [C.c_void_p]
         L.qz_batch_replays.restype = C.c_long
         L.qz_batch_destroy.argtypes = [C.c_void_p]
@@ -407,15 +409,23 @@ def _device_name():
 
 
 class Batch:
-    """A fixed set of Simulations (same modes) stepped together: all sweeps in one persistent launch, and with
-    graph=True the steady-state step (beam deposits, plasma reloads, the batched sweep, beam pushes) replayed
-    as one captured HIP graph.  The Simulations must outlive the Batch and must not be stepped otherwise."""
+    """A fixed set of Simulations (same modes) stepped together: the sweeps of one-workgroup runs in one
+    persistent launch that takes them from a queue, longest first, those of multi-workgroup runs in one launch
+    beside it, and with graph=True the steady-state step (beam deposits, plasma reloads, the sweep launches,
+    beam pushes) replayed as one captured HIP graph.  The Simulations must outlive the Batch and must not be
+    stepped otherwise."""
 
-    def __init__(self, sims, graph=True):
+    def __init__(self, sims, graph=True, workers=0):
         self.sims = list(sims)
         arr = (C.c_void_p * len(self.sims))(*[s.h.value for s in self.sims])
         self.h = C.c_void_p()
         check(lib().qz_batch_create(arr, len(self.sims), 1 if graph else 0, C.byref(self.h)))
+        if workers:
+            self.set_workers(workers)
+
+    def set_workers(self, n):
+        """At most n workgroups take the one-workgroup runs from the queue (0: no cap)."""
+        check(lib().qz_batch_set_workers(self.h, int(n)))
 
     def step(self):
         """One step of every run; returns the batched sweep time (eager steps) or the whole step's time
@@ -423,6 +433,17 @@ class Batch:
         ms = C.c_double(0)
         check(lib().qz_batch_step(self.h, C.byref(ms)))
         return ms.value
+
+    def balance(self, candidates=(1, 2, 3, 4, 6, 8), nslices=256):
+        """Workgroups per run, before the first step: every shape of run (modes, cells, plasma particles) is
+        timed over its first `nslices` slices with each candidate count, each run's sweep is estimated from its
+        slices, and the plan with the smallest estimated batched sweep is applied (qz_batch_balance).  Returns
+        {"groups": [per run], "est_ms": plan estimate, "est_one_ms": estimate with one workgroup per run}."""
+        cand = (C.c_int * len(candidates))(*candidates)
+        g = (C.c_int * len(self.sims))()
+        est = np.zeros(2)
+        check(lib().qz_batch_balance(self.h, cand, len(candidates), int(nslices), g, est.ctypes.data_as(DP)))
+        return {"groups": list(g), "est_ms": float(est[0]), "est_one_ms": float(est[1])}
 
     @property
     def replays(self):

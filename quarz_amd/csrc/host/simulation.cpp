@@ -10,6 +10,8 @@
 #include <fstream>
 #include <functional>
 #include <iomanip>
+#include <map>
+#include <queue>
 #include <random>
 #include <sstream>
 #include <stdexcept>
@@ -97,6 +99,119 @@ static void check_multi_sweep(const DevBuf<Real>& mws, int groups, hipStream_t s
                              st), "sweep flag");
     hip_check(hipStreamSynchronize(st), "sync");
     if (flag) throw std::runtime_error("multi-workgroup sweep: a grid barrier timed out (workgroups not co-resident)");
+}
+
+static int device_cus() {
+    int dev = 0, ncu = 0;
+    hip_check(hipGetDevice(&dev), "device");
+    hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute");
+    return ncu;
+}
+
+// Workgroups of one launch go to the accelerator dies (XCDs) in turn, block b to die b mod kXcds, so a launch of
+// B workgroups puts at most ceil(B / kXcds) on any die, whichever die its first block goes to.  Two launches
+// with W and B workgroups are resident together when ceil(W / kXcds) + ceil(B / kXcds) workgroups fit one die's
+// compute units (one workgroup per compute unit).  MI355X: 8 dies of 32 compute units.
+constexpr int kXcds = 8;
+static int workers_beside(int ncu, int B) {
+    return std::max(0, kXcds * (ncu / kXcds - (B + kXcds - 1) / kXcds));
+}
+
+static int mobile_particles(const Simulation& s) {
+    int np = 0;
+    for (const auto& sp : s.species)
+        if (sp->mobile) np += sp->cap;
+    return np;
+}
+
+// The sweeps of a batched step.  The one-workgroup runs go to `workers` workgroups of one launch, which take
+// them from a device queue longest first (by Simulation::slice_ms when every run has one, else by slices x
+// (cells + particles)); the multi-workgroup runs go to one launch of their own, on a second stream beside it.
+// Every workgroup of either kernel holds a whole compute unit's LDS, and the workers are at most
+// workers_beside(the multi launch's workgroups), so the multi launch's workgroups are all resident together
+// whichever launch starts first.  Device layout of the arguments: the one-workgroup runs in queue order, the
+// multi-workgroup runs, the multi launch's block map, then one block whose first word is the queue counter.
+struct MixedSweep {
+    std::vector<qz_sweep_args> single, multi;
+    std::vector<Simulation*> multi_runs;
+    std::vector<int> map;
+    std::vector<qz_sweep_args> stage;   // host image of the device arguments (alive until the step syncs)
+    int workers = 0, multi_blocks = 0;
+    size_t map_offset() const { return single.size() + multi.size(); }
+    size_t map_blocks() const { return (sizeof(int) * map.size() + sizeof(qz_sweep_args) - 1) / sizeof(qz_sweep_args); }
+    size_t blocks() const { return map_offset() + map_blocks() + 1; }
+};
+
+static MixedSweep plan_mixed(Simulation* const* runs, int n, int max_workers) {
+    MixedSweep P;
+    bool est = true;
+    for (int i = 0; i < n; ++i) est = est && runs[i]->slice_ms > 0;
+    std::vector<std::pair<double, int>> cost;
+    for (int i = 0; i < n; ++i) {
+        Simulation& s = *runs[i];
+        if (qz_validate_sweep(s.sweep_args) != QZ_OK) throw std::runtime_error(qz_last_error());
+        if (s.groups > 1) {
+            if (!s.mws.p) throw std::runtime_error("a multi-workgroup run needs its workspace");
+            P.multi.push_back(s.sweep_args);
+            P.multi_runs.push_back(&s);
+            P.multi_blocks += s.groups;
+        } else {
+            cost.emplace_back(est ? s.slice_ms * s.K : static_cast<double>(s.K) * (s.grid.N + mobile_particles(s)), i);
+        }
+    }
+    std::stable_sort(cost.begin(), cost.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    for (const auto& c : cost) P.single.push_back(runs[c.second]->sweep_args);
+    const int ncu = device_cus(), ns = static_cast<int>(P.single.size());
+    if (!P.multi.empty()) {
+        if (P.multi_blocks > ncu)
+            throw std::runtime_error("batched sweep: the multi-workgroup runs need " + std::to_string(P.multi_blocks) +
+                                     " workgroups, more than the " + std::to_string(ncu) + " compute units");
+        P.map = qz_sweep_block_map(P.multi.data(), static_cast<int>(P.multi.size()));
+        if (ns > 0 && workers_beside(ncu, P.multi_blocks) < 1)
+            throw std::runtime_error("batched sweep: the multi-workgroup runs take every compute unit and leave "
+                                     "none to the one-workgroup runs; give them fewer workgroups");
+        P.workers = std::min(ns, workers_beside(ncu, P.multi_blocks));
+    } else {
+        P.workers = std::min(ns, ncu);
+    }
+    if (max_workers > 0) P.workers = std::min(P.workers, max_workers);
+    P.stage.assign(P.blocks(), qz_sweep_args{});
+    std::copy(P.single.begin(), P.single.end(), P.stage.begin());
+    std::copy(P.multi.begin(), P.multi.end(), P.stage.begin() + static_cast<long>(P.single.size()));
+    if (!P.map.empty()) std::memcpy(P.stage.data() + P.map_offset(), P.map.data(), sizeof(int) * P.map.size());
+    return P;
+}
+
+// the arguments into `buf` (grown, after a wait for `st`, when too small), stream-ordered on st or at once
+static void upload_mixed(const MixedSweep& P, DevBuf<qz_sweep_args>& buf, hipStream_t st, bool now) {
+    if (buf.n < P.blocks()) {
+        hip_check(hipStreamSynchronize(st), "sync");
+        buf.alloc(P.blocks());
+    }
+    const size_t bytes = sizeof(qz_sweep_args) * (P.blocks() - 1);
+    if (now) hip_check(hipMemcpy(buf.p, P.stage.data(), bytes, hipMemcpyHostToDevice), "sweep arguments");
+    else hip_check(hipMemcpyAsync(buf.p, P.stage.data(), bytes, hipMemcpyHostToDevice, st), "sweep arguments");
+}
+
+// the two launches (kernels only, so that a capture can hold them): the multi launch forks to `aux` and joins
+// back to `main`
+static void enqueue_mixed(const MixedSweep& P, qz_sweep_args* dbuf, hipStream_t main, hipStream_t aux,
+                          hipEvent_t fork, hipEvent_t join) {
+    const int ns = static_cast<int>(P.single.size()), nm = static_cast<int>(P.multi.size());
+    if (nm > 0) {
+        hip_check(hipEventRecord(fork, main), "fork");
+        hip_check(hipStreamWaitEvent(aux, fork, 0), "fork");
+        for (Simulation* s : P.multi_runs) qz_check(qz_zero_async(s->mws.p, 64 * sizeof(double), aux));
+        const int* dmap = reinterpret_cast<const int*>(dbuf + P.map_offset());
+        qz_check(qz_sweep_multi_launch(dbuf + ns, dmap, P.multi.data(), nm, aux, P.workers + P.multi_blocks));
+        hip_check(hipEventRecord(join, aux), "join");
+    }
+    if (ns > 0) {
+        const bool queued = ns > P.workers;
+        unsigned int* queue = reinterpret_cast<unsigned int*>(dbuf + (P.blocks() - 1));
+        qz_check(qz_sweep_launch(dbuf, P.single.data(), ns, main, queued ? P.workers : 0, queued ? queue : nullptr));
+    }
+    if (nm > 0) hip_check(hipStreamWaitEvent(main, join, 0), "join");
 }
 
 // ---------------------------------------------------------------------------- RadialGrid (src/RadialGrid.cpp)
@@ -863,8 +978,12 @@ Simulation::Simulation(const std::string& deck, const std::string& overrides) { 
 
 Simulation::~Simulation() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (aux) (void)hipStreamSynchronize(aux);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : aux_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1123,6 +1242,13 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     // next range (a particle in the deposit window has its gather slice below the range after next)
     nchunks = std::max(1, std::min(nchunks, K / 2));
     depth = std::max(1, std::min(depth, nrun));
+    if (groups > 1) {   // every step's launch of `groups` workgroups resident beside the others (workers_beside)
+        const int ncu = device_cus();
+        if (depth * ((groups + kXcds - 1) / kXcds) > ncu / kXcds)
+            throw std::runtime_error("run_pipelined: " + std::to_string(depth) + " steps in flight of " +
+                                     std::to_string(groups) + " workgroups each cannot all be resident; use at most " +
+                                     std::to_string((ncu / kXcds) / ((groups + kXcds - 1) / kXcds)) + " in flight");
+    }
     std::vector<int> cb(nchunks + 1);
     for (int c = 0; c <= nchunks; ++c) cb[c] = static_cast<int>((static_cast<long long>(K) * c) / nchunks);
 
@@ -1693,29 +1819,37 @@ extern "C" int qz_sim_step(qz_sim* h) {
     QZ_SIM_TRY(h->s->step())
 }
 
-// One step of n independent runs with their sweeps in one persistent launch (one workgroup per run).
-// Every run's preparation is enqueued on its own stream; the launch waits for all of them on the first
-// run's stream; every run then finishes and pushes its beams on its own stream.
-extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
+// Ensures the run's second launch stream and its fork/join events exist.
+static void ensure_aux(qz::Simulation& s) {
+    if (s.aux) return;
+    qz::hip_check(hipStreamCreate(&s.aux), "stream");
+    for (auto& e : s.aux_ev) qz::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+}
+
+// One step of n independent runs with their sweeps launched together: the one-workgroup runs in one persistent
+// launch whose workgroups take them from a queue, longest first, and the multi-workgroup runs (groups > 1) in
+// one launch beside it (qz::MixedSweep).  Every run's preparation is enqueued on its own stream; the launches
+// wait for all of them on the first run's stream; every run then finishes and pushes its beams on its own
+// stream.
+static int step_batched(qz_sim** hs, int n, double* batch_ms, int max_workers) {
     if (!hs || n <= 0) return qz_fail(QZ_EINVAL, "qz_sim_step_batched: no runs");
     QZ_SIM_TRY({
+        std::vector<qz::Simulation*> runs(n);
         for (int i = 0; i < n; ++i) {
             if (!hs[i]) throw std::runtime_error("qz_sim_step_batched: null run");
             if (hs[i]->s->done) throw std::runtime_error("qz_sim_step_batched: a run has ended (time.steps reached)");
             if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_sim_step_batched: runs differ in modes");
-            if ((hs[i]->s->groups > 1) != (hs[0]->s->groups > 1))
-                throw std::runtime_error("qz_sim_step_batched: the runs of one step have one workgroup each, or all "
-                                         "several");
+            runs[i] = hs[i]->s.get();
         }
-        std::vector<qz_sweep_args> args(n);
         qz::Simulation& s0 = *hs[0]->s;
         for (int i = 0; i < n; ++i) {
             qz::Simulation& s = *hs[i]->s;
             s.prepare_sweep();
-            args[i] = s.sweep_args;
             qz::hip_check(hipEventRecord(s.ev[0], s.stream), "event");
             if (i > 0) qz::hip_check(hipStreamWaitEvent(s0.stream, s.ev[0], 0), "stream wait");
         }
+        const qz::MixedSweep P = qz::plan_mixed(runs.data(), n, max_workers);
+        if (!P.multi.empty()) ensure_aux(s0);
         struct Events {   // released on every exit path
             hipEvent_t e[2] = {nullptr, nullptr};
             ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
@@ -1723,10 +1857,12 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
         qz::hip_check(hipEventCreate(&ev.e[0]), "event");
         qz::hip_check(hipEventCreate(&ev.e[1]), "event");
         hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
+        qz::upload_mixed(P, s0.d_args, s0.stream, false);
         qz::hip_check(hipEventRecord(e0, s0.stream), "event");
-        qz::launch_sweeps(s0.d_args, args.data(), n, s0.stream);
+        qz::enqueue_mixed(P, s0.d_args.p, s0.stream, s0.aux, s0.aux_ev[0], s0.aux_ev[1]);
         qz::hip_check(hipEventRecord(e1, s0.stream), "event");
         qz::hip_check(hipEventSynchronize(e1), "sync");
+        for (qz::Simulation* s : P.multi_runs) qz::check_multi_sweep(s->mws, s->groups, s->stream);
         float ms = 0;
         qz::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
         if (batch_ms) *batch_ms = ms;
@@ -1755,30 +1891,39 @@ extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) {
     })
 }
 
-// A fixed set of runs stepped together.  Every step runs all sweeps in one persistent launch; once a step
-// repeats the device work of the one before in every run (Simulation::capturable), the whole step (each
-// run's beam deposit and plasma reload, the batched sweep, each run's beam push) is captured once as a HIP
+extern "C" int qz_sim_step_batched(qz_sim** hs, int n, double* batch_ms) { return step_batched(hs, n, batch_ms, 0); }
+
+// A fixed set of runs stepped together.  Every step launches all sweeps together (qz_sim_step_batched); once a
+// step repeats the device work of the one before in every run (Simulation::capturable), the whole step (each
+// run's beam deposit and plasma reload, the sweep launches, each run's beam push) is captured once as a HIP
 // graph on the batch's stream and replayed.  Not in the reference.
 struct qz_batch {
     static constexpr int kBranches = 8;   // streams the per-run work of a captured step is spread over
     std::vector<qz_sim*> sims;
     bool use_graph = true;
-    hipStream_t st = nullptr;
+    int max_workers = 0;                       // cap on the one-workgroup launch's workgroups (0: none)
+    hipStream_t st = nullptr, aux = nullptr;   // aux: the multi-workgroup launch of a captured step
     hipStream_t br[kBranches] = {};
-    hipEvent_t fork[2] = {}, join[2][kBranches] = {};
+    hipEvent_t fork[2] = {}, join[2][kBranches] = {}, mfork = nullptr, mjoin = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    qz_sweep_args* dargs = nullptr;   // device copy of the runs' sweep arguments, read by the graph's launch
-    std::vector<qz_sweep_args> hargs;
+    qz::MixedSweep plan;                     // the captured step's launches
+    qz::DevBuf<qz_sweep_args> dargs;         // their device arguments, read by the graph's launches
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     long replays = 0;
-    ~qz_batch() {
+    void drop_graph() {   // the next replayable step captures again
         if (st) (void)hipStreamSynchronize(st);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        if (dargs) (void)hipFree(dargs);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~qz_batch() {
+        drop_graph();
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (mfork) (void)hipEventDestroy(mfork);
+        if (mjoin) (void)hipEventDestroy(mjoin);
         for (int f = 0; f < 2; ++f) {
             if (fork[f]) (void)hipEventDestroy(fork[f]);
             for (auto e : join[f])
@@ -1786,6 +1931,7 @@ struct qz_batch {
         }
         for (auto b : br)
             if (b) (void)hipStreamDestroy(b);
+        if (aux) (void)hipStreamDestroy(aux);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -1797,16 +1943,20 @@ extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out
         for (int i = 0; i < n; ++i) {
             if (!hs[i]) throw std::runtime_error("qz_batch_create: null run");
             if (hs[i]->s->m1 != hs[0]->s->m1) throw std::runtime_error("qz_batch_create: runs differ in modes");
-            if (hs[i]->s->groups > 1) throw std::runtime_error("qz_batch_create: runs of one workgroup only");
+            for (int j = 0; j < i; ++j)
+                if (hs[j] == hs[i]) throw std::runtime_error("qz_batch_create: a run appears twice");
             b->sims.push_back(hs[i]);
         }
         b->use_graph = use_graph != 0;
         qz::hip_check(hipStreamCreate(&b->st), "stream");
+        qz::hip_check(hipStreamCreate(&b->aux), "stream");
         for (auto& x : b->br) qz::hip_check(hipStreamCreate(&x), "stream");
         for (int f = 0; f < 2; ++f) {
             qz::hip_check(hipEventCreateWithFlags(&b->fork[f], hipEventDisableTiming), "event");
             for (auto& e : b->join[f]) qz::hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
         }
+        qz::hip_check(hipEventCreateWithFlags(&b->mfork, hipEventDisableTiming), "event");
+        qz::hip_check(hipEventCreateWithFlags(&b->mjoin, hipEventDisableTiming), "event");
         qz::hip_check(hipEventCreate(&b->e0), "event");
         qz::hip_check(hipEventCreate(&b->e1), "event");
         *out = b.release();
@@ -1815,15 +1965,23 @@ extern "C" int qz_batch_create(qz_sim** hs, int n, int use_graph, qz_batch** out
 
 extern "C" void qz_batch_destroy(qz_batch* b) { delete b; }
 
+extern "C" int qz_batch_set_workers(qz_batch* b, int max_workers) {
+    if (!b || max_workers < 0) return qz_fail(QZ_EINVAL, "qz_batch_set_workers: null batch or negative count");
+    QZ_SIM_TRY({
+        b->drop_graph();
+        b->max_workers = max_workers;
+    })
+}
+
 // capture of one step of every run: the per-run work (preparation, then beam push) on branch streams forked
-// from and joined back to the batch stream around the batched sweep, so that runs overlap in the graph
+// from and joined back to the batch stream around the sweep launches, so that runs overlap in the graph
 // (each run's stream is switched to its branch meanwhile)
 static void capture_batch(qz_batch* b) {
     const int n = static_cast<int>(b->sims.size());
-    b->hargs.resize(n);
-    for (int i = 0; i < n; ++i) b->hargs[i] = b->sims[i]->s->sweep_args;   // the same every step
-    qz::hip_check(hipMalloc(reinterpret_cast<void**>(&b->dargs), sizeof(qz_sweep_args) * n), "batch arguments");
-    qz::hip_check(hipMemcpy(b->dargs, b->hargs.data(), sizeof(qz_sweep_args) * n, hipMemcpyHostToDevice), "batch arguments");
+    std::vector<qz::Simulation*> runs(n);
+    for (int i = 0; i < n; ++i) runs[i] = b->sims[i]->s.get();
+    b->plan = qz::plan_mixed(runs.data(), n, b->max_workers);   // the arguments are the same every step
+    qz::upload_mixed(b->plan, b->dargs, b->st, true);
     std::vector<hipStream_t> saved(n);
     for (int i = 0; i < n; ++i) saved[i] = b->sims[i]->s->stream;
     const int nb = std::min(n, qz_batch::kBranches);
@@ -1843,7 +2001,7 @@ static void capture_batch(qz_batch* b) {
     qz::hip_check(hipStreamBeginCapture(b->st, hipStreamCaptureModeThreadLocal), "begin capture");
     try {
         per_run(0, [](qz::Simulation& s) { s.prepare_sweep(false); });
-        if (qz_sweep_launch(b->dargs, b->hargs.data(), n, b->st) != QZ_OK) throw std::runtime_error(qz_last_error());
+        qz::enqueue_mixed(b->plan, b->dargs.p, b->st, b->aux, b->mfork, b->mjoin);
         per_run(1, [](qz::Simulation& s) { s.enqueue_push(false); });
     } catch (...) {
         for (int i = 0; i < n; ++i) b->sims[i]->s->stream = saved[i];
@@ -1867,7 +2025,7 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
             replay = replay && s.capturable();
         }
         if (!replay) {   // start-up, last or output steps: the eager batched step
-            const int rc = qz_sim_step_batched(b->sims.data(), n, step_ms);
+            const int rc = step_batched(b->sims.data(), n, step_ms, b->max_workers);
             if (rc != QZ_OK) throw std::runtime_error(qz_last_error());
             return QZ_OK;
         }
@@ -1876,6 +2034,7 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
         qz::hip_check(hipGraphLaunch(b->exec, b->st), "graph launch");
         qz::hip_check(hipEventRecord(b->e1, b->st), "event");
         qz::hip_check(hipEventSynchronize(b->e1), "sync");
+        for (qz::Simulation* s : b->plan.multi_runs) qz::check_multi_sweep(s->mws, s->groups, s->stream);
         float ms = 0;
         qz::hip_check(hipEventElapsedTime(&ms, b->e0, b->e1), "elapsed");
         ++b->replays;
@@ -1895,35 +2054,44 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
 }
 extern "C" long qz_batch_replays(const qz_batch* b) { return b ? b->replays : -1; }
 
-// Autotuner of the sweep's workgroups per run (quarz_amd.h): every candidate count sweeps the first nslices
-// slices of the coming step; the fastest is kept.  Before the first step only: the samples overwrite the plasma
-// arrays and the first rows of fld and diag, which the step reloads and rewrites.
-extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nslices, double* ms_out) {
-    if (!h || !cand || ncand < 1 || nslices < 1 || !ms_out) return qz_fail(QZ_EINVAL, "qz_sim_tune_groups: bad arguments");
-    QZ_SIM_TRY({
-        qz::Simulation& s = *h->s;
-        if (s.swept) throw std::runtime_error("qz_sim_tune_groups: tune before the first step");
-        int np = 0;
-        for (const auto& sp : s.species)
-            if (sp->mobile) np += sp->cap;
-        struct Events {
-            hipEvent_t e[2] = {nullptr, nullptr};
-            ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
-        } ev;
-        qz::hip_check(hipEventCreate(&ev.e[0]), "event");
-        qz::hip_check(hipEventCreate(&ev.e[1]), "event");
-        int best = -1;
-        for (int i = 0; i < ncand; ++i) {
-            ms_out[i] = -1;   // a count the run cannot use
-            const int G = cand[i];
-            if (G < 1 || (G == 1 && !qz_sweep_fits_one_group(s.m1, s.grid.N, np)) || np > G * 4096) continue;
-            s.groups = G;
-            s.mws.release();
-            if (G > 1) s.mws.alloc(qz_sweep_multi_workspace_doubles(s.M, np, G, s.m1 ? 1 : 0));
-            s.prepare_sweep(false);
-            qz_sweep_args a = s.sweep_args;
-            a.k_end = std::min(nslices, s.K);
-            a.stats = nullptr;
+namespace {
+// The run's sweep takes G workgroups from now on (a multi-workgroup workspace for G > 1).
+void set_groups(qz::Simulation& s, int G) {
+    if (G == s.groups && (G == 1 || s.mws.p)) return;
+    s.groups = G;
+    s.mws.release();
+    if (G > 1) s.mws.alloc(qz_sweep_multi_workspace_doubles(s.M, qz::mobile_particles(s), G, s.m1 ? 1 : 0));
+}
+
+// Whether the run's sweep can take G workgroups.
+bool groups_usable(const qz::Simulation& s, int G, int ncu) {
+    const int np = qz::mobile_particles(s);
+    if (G < 1 || G > ncu || np > G * 4096) return false;
+    return G > 1 || qz_sweep_fits_one_group(s.m1, s.grid.N, np);
+}
+
+// Times the first nslices slices of the coming step's sweep with each candidate workgroup count, the best of
+// `reps` launches each (ms_out[i] = -1: a count the run cannot use); the run keeps its workgroup count.
+// Before the first step only: the samples overwrite the plasma arrays and the first rows of fld and diag,
+// which the step reloads and rewrites.
+void probe_groups(qz::Simulation& s, const int* cand, int ncand, int nslices, int reps, double* ms_out) {
+    if (s.swept) throw std::runtime_error("sweep probe: probe before the first step");
+    struct Events {
+        hipEvent_t e[2] = {nullptr, nullptr};
+        ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
+    } ev;
+    qz::hip_check(hipEventCreate(&ev.e[0]), "event");
+    qz::hip_check(hipEventCreate(&ev.e[1]), "event");
+    const int keep = s.groups, ncu = qz::device_cus();
+    for (int i = 0; i < ncand; ++i) {
+        ms_out[i] = -1;
+        if (!groups_usable(s, cand[i], ncu)) continue;
+        set_groups(s, cand[i]);
+        s.prepare_sweep(false);
+        qz_sweep_args a = s.sweep_args;
+        a.k_end = std::min(nslices, s.K);
+        a.stats = nullptr;
+        for (int r = 0; r < reps; ++r) {
             qz::hip_check(hipEventRecord(ev.e[0], s.stream), "event");
             qz::launch_sweeps(s.d_args, &a, 1, s.stream);
             qz::hip_check(hipEventRecord(ev.e[1], s.stream), "event");
@@ -1931,14 +2099,140 @@ extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nsl
             qz::check_multi_sweep(s.mws, s.groups, s.stream);
             float ms = 0;
             qz::hip_check(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]), "elapsed");
-            ms_out[i] = ms;
-            if (best < 0 || ms < ms_out[best]) best = i;
+            if (ms_out[i] < 0 || ms < ms_out[i]) ms_out[i] = ms;
         }
+    }
+    set_groups(s, keep);
+    qz::hip_check(hipStreamSynchronize(s.stream), "sync");
+}
+
+// Makespan of list scheduling: jobs (longest first) each to the earliest free of w workers.
+double list_makespan(const std::vector<double>& jobs_desc, int w) {
+    if (jobs_desc.empty()) return 0;
+    if (w < 1) return 1e300;
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+    for (int i = 0; i < w; ++i) free_at.push(0.0);
+    double end = 0;
+    for (double t : jobs_desc) {
+        const double s = free_at.top();
+        free_at.pop();
+        free_at.push(s + t);
+        end = std::max(end, s + t);
+    }
+    return end;
+}
+}  // namespace
+
+// Autotuner of the sweep's workgroups per run (quarz_amd.h): every candidate count sweeps the first nslices
+// slices of the coming step; the fastest is kept.
+extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nslices, double* ms_out) {
+    if (!h || !cand || ncand < 1 || nslices < 1 || !ms_out) return qz_fail(QZ_EINVAL, "qz_sim_tune_groups: bad arguments");
+    QZ_SIM_TRY({
+        qz::Simulation& s = *h->s;
+        probe_groups(s, cand, ncand, nslices, 1, ms_out);
+        int best = -1;
+        for (int i = 0; i < ncand; ++i)
+            if (ms_out[i] >= 0 && (best < 0 || ms_out[i] < ms_out[best])) best = i;
         if (best < 0) throw std::runtime_error("qz_sim_tune_groups: no candidate fits this run");
-        s.groups = cand[best];
-        s.mws.release();
-        if (s.groups > 1) s.mws.alloc(qz_sweep_multi_workspace_doubles(s.M, np, s.groups, s.m1 ? 1 : 0));
-        qz::hip_check(hipStreamSynchronize(s.stream), "sync");
+        set_groups(s, cand[best]);
+    })
+}
+
+// Autotuner of a batch's workgroups per run (quarz_amd.h).  Runs of one shape (modes, cells, plasma particles)
+// share a time per slice; the shape's run with the most slices is probed over its first nslices slices with
+// every candidate count (best of two launches).  A run's sweep is then estimated at (time per slice) x (its
+// slices).  A plan gives every run whose one-workgroup estimate exceeds a target T the fewest workgroups that
+// bring it under T, and the other runs to the remaining compute units, list-scheduled longest first; its cost
+// is the larger of T and that schedule's makespan.  Every estimate of every run and count is tried as T, and
+// the cheapest plan is applied.  est_ms[0] = its cost, est_ms[1] = the cost with one workgroup per run.
+extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nslices, int* groups_out,
+                                double* est_ms) {
+    if (!b || !cand || ncand < 1 || nslices < 1 || !groups_out || !est_ms)
+        return qz_fail(QZ_EINVAL, "qz_batch_balance: bad arguments");
+    QZ_SIM_TRY({
+        const int n = static_cast<int>(b->sims.size()), ncu = qz::device_cus();
+        std::map<std::array<int, 3>, std::vector<int>> shapes;   // (modes, cells, particles) -> runs
+        for (int i = 0; i < n; ++i) {
+            const qz::Simulation& s = *b->sims[i]->s;
+            if (s.swept) throw std::runtime_error("qz_batch_balance: balance before the first step");
+            shapes[{s.m1 ? 1 : 0, s.grid.N, qz::mobile_particles(s)}].push_back(i);
+        }
+        // est[i][c]: run i's sweep with cand[c] workgroups (ms), < 0 where unusable
+        std::vector<std::vector<double>> est(n, std::vector<double>(ncand, -1));
+        std::vector<double> one(n, -1), per(ncand);
+        int c1 = -1;
+        for (int c = 0; c < ncand; ++c)
+            if (cand[c] == 1) c1 = c;
+        for (const auto& kv : shapes) {
+            int rep = kv.second[0];
+            for (int i : kv.second)
+                if (b->sims[i]->s->K > b->sims[rep]->s->K) rep = i;
+            qz::Simulation& r = *b->sims[rep]->s;
+            probe_groups(r, cand, ncand, nslices, 2, per.data());
+            const double sampled = std::min(nslices, r.K);
+            for (int i : kv.second) {
+                qz::Simulation& s = *b->sims[i]->s;
+                for (int c = 0; c < ncand; ++c) est[i][c] = per[c] < 0 ? -1 : per[c] / sampled * s.K;
+                s.slice_ms = c1 >= 0 && per[c1] > 0 ? per[c1] / sampled : 0;
+                one[i] = c1 >= 0 ? est[i][c1] : -1;
+            }
+        }
+        // a plan for target T: groups per run (0: none fits), and its cost
+        std::vector<int> G(n);
+        auto plan_cost = [&](double T, std::vector<int>& g) -> double {
+            int used = 0;
+            double top = 0;
+            std::vector<double> singles;
+            for (int i = 0; i < n; ++i) {
+                if (one[i] >= 0 && one[i] <= T) {
+                    g[i] = 1;
+                    singles.push_back(one[i]);
+                    continue;
+                }
+                int pick = -1;
+                for (int c = 0; c < ncand; ++c)
+                    if (cand[c] > 1 && est[i][c] >= 0 && est[i][c] <= T && (pick < 0 || cand[c] < cand[pick])) pick = c;
+                if (pick < 0) return 1e300;
+                g[i] = cand[pick];
+                used += cand[pick];
+                top = std::max(top, est[i][pick]);
+            }
+            const int free_cus = used > 0 ? qz::workers_beside(ncu, used) : ncu;
+            if (used > ncu || (!singles.empty() && free_cus < 1)) return 1e300;
+            std::sort(singles.begin(), singles.end(), std::greater<double>());
+            const int w = std::min(static_cast<int>(singles.size()), free_cus);
+            return std::max(top, list_makespan(singles, w));
+        };
+        std::vector<double> targets;
+        for (int i = 0; i < n; ++i)
+            for (int c = 0; c < ncand; ++c)
+                if (est[i][c] >= 0) targets.push_back(est[i][c]);
+        std::sort(targets.begin(), targets.end());
+        targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
+        double best = 1e300;
+        std::vector<int> g(n);
+        for (double T : targets) {
+            const double c = plan_cost(T, g);
+            if (c < best * (1 - 1e-9)) { best = c; G = g; }
+        }
+        if (best >= 1e299) throw std::runtime_error("qz_batch_balance: no plan fits the compute units");
+        double single_cost = 1e300;
+        {
+            std::vector<double> all;
+            bool ok = true;
+            for (int i = 0; i < n; ++i) { ok = ok && one[i] >= 0; all.push_back(one[i]); }
+            if (ok) {
+                std::sort(all.begin(), all.end(), std::greater<double>());
+                single_cost = list_makespan(all, std::min(n, ncu));
+            }
+        }
+        b->drop_graph();
+        for (int i = 0; i < n; ++i) {
+            set_groups(*b->sims[i]->s, G[i]);
+            groups_out[i] = G[i];
+        }
+        est_ms[0] = best;
+        est_ms[1] = single_cost < 1e299 ? single_cost : -1;
     })
 }
 

@@ -210,6 +210,9 @@ public:
     int sweep_launches = 1;      // sweep kernel launches of the last step (pipelined: its slice ranges)
     int groups = 1;              // workgroups of the sweep (1: the one-workgroup kernel)
     DevBuf<Real> mws;            // groups > 1: the multi-workgroup workspace
+    double slice_ms = 0;         // a batch's estimate of the one-workgroup sweep's time per slice (0: none)
+    hipStream_t aux = nullptr;   // a batched step's launch stream for the multi-workgroup runs (made on first use)
+    hipEvent_t aux_ev[2] = {};   // fork to and join from `aux`
     double particle_slices = 0, beam_pushed = 0, beam_deposited = 0;
     double phase_ticks[QZ_NPHASE] = {};
     hipStream_t stream = nullptr;

@@ -119,3 +119,75 @@ def test_pipelined_groups_equal_plain(qa, decks, deck, ov, groups, steps, chunks
             ga, gb = b.get(f"beam{bi}.{k}"), a.get(f"beam{bi}.{k}")
             assert _same(ga, gb), _report(f"pipelined beam{bi}.{k}", ga, gb)
     assert b.lost() == a.lost()
+
+
+# ------------------------------------------------------------------------------------------ mixed batches
+GAIN_RUNS = [{"xi.min": -5, "witness.xi0": -4},
+             {"xi.min": -6, "witness.xi0": -4.5, "grid.type": "uniform", "grid.dr": 0.02, "electrons.ppc": 2},
+             {"xi.min": -4.5, "witness.xi0": -4, "grid.type": "uniform", "grid.dr": 0.08, "electrons.ppc": 8},
+             {"xi.min": -5.5, "witness.xi0": -4.5, "electrons.ppc": 2},
+             {"xi.min": -4, "witness.xi0": -3.5, "grid.type": "uniform", "grid.dr": 0.04}]
+GAIN_BASE = {"time.steps": 5, "driver.nparticles": 20000, "witness.nparticles": 5000}
+
+
+def _batch_vs_plain(qa, deck, batch_ovs, plain_ovs, steps, workers=0, balance=None):
+    """A graph Batch against the same decks stepped one run at a time on one workgroup, bit for bit after every
+    step (the eager start-up step, the replays, the first of which captures, and the eager last step)."""
+    plain = [qa.Simulation(deck, ov) for ov in plain_ovs]
+    runs = [qa.Simulation(deck, ov) for ov in batch_ovs]
+    assert all(p.groups == 1 for p in plain)
+    batch = qa.Batch(runs, graph=True, workers=workers)
+    try:
+        plan = batch.balance(**balance) if balance else None
+        for st in range(steps):
+            for s in plain:
+                s.step()
+            batch.step()
+            for i, (a, b) in enumerate(zip(runs, plain)):
+                for w in ("fld", "diag", "bsrc"):
+                    ga, gb = getattr(a, w), getattr(b, w)
+                    assert _same(ga, gb), _report(f"step {st} run {i} (groups {a.groups}) {w}", ga, gb)
+                for bi in range(a.nbeams):
+                    for k in qa.BEAM_ARRAYS:
+                        ga, gb = a.get(f"beam{bi}.{k}"), b.get(f"beam{bi}.{k}")
+                        assert _same(ga, gb), _report(f"step {st} run {i} beam{bi}.{k}", ga, gb)
+                assert a.lost() == b.lost()
+                assert a.t == b.t
+        replays = batch.replays
+    finally:
+        batch.close()
+    return runs, plan, replays
+
+
+def test_mixed_batch_queue_equals_plain(qa, decks):
+    """Five runs of different grids and slice counts in one graph Batch: two on several workgroups (2 and 3, one
+    launch beside the other) and three on one workgroup each, taken from the queue by two workers.  Every run
+    equals its own one-workgroup plain run after every step."""
+    ovs = [dict(GAIN_BASE, **o) for o in GAIN_RUNS]
+    forced = [dict(o) for o in ovs]
+    forced[0]["sweep.groups"] = 2
+    forced[3]["sweep.groups"] = 3
+    runs, _, replays = _batch_vs_plain(qa, f"{decks}/gain.in", forced, ovs, 6, workers=2)
+    assert [r.groups for r in runs] == [2, 1, 1, 3, 1]
+    assert replays == 4
+
+
+def test_one_group_batch_queue_equals_plain(qa, decks):
+    """One-workgroup runs only, one worker: the queue runs them one after another, longest first."""
+    ovs = [dict(GAIN_BASE, **o) for o in GAIN_RUNS[:3]]
+    _, _, replays = _batch_vs_plain(qa, f"{decks}/gain.in", ovs, ovs, 6, workers=1)
+    assert replays == 4
+
+
+def test_balanced_batch_equals_plain(qa, decks):
+    """The batch autotuner: every shape probed with one to four workgroups, the plan applied to the runs, then
+    the steps; the plan is no worse than one workgroup per run by its own estimate, and every run equals its
+    plain run."""
+    ovs = [dict(GAIN_BASE, **o) for o in GAIN_RUNS]
+    runs, plan, replays = _batch_vs_plain(qa, f"{decks}/gain.in", ovs, ovs, 6,
+                                          balance={"candidates": (1, 2, 3, 4), "nslices": 64})
+    print("plan:", plan)
+    assert plan["groups"] == [r.groups for r in runs]
+    assert all(g in (1, 2, 3, 4) for g in plan["groups"])
+    assert 0 < plan["est_ms"] <= plan["est_one_ms"] * (1 + 1e-12)
+    assert replays == 4

@@ -457,14 +457,19 @@ def test_c5_runs_batched_bit_exact(qa, oracle, tmp_path):
 
 
 def test_c5_bench_batch_extremes_equal_plain(qa, tmp_path):
-    """The bench's c5 batch: all 256 runs stepped as a Batch with graph replays (the eager start-up step, then
-    four replays, the first of which captures the graph) for the workload's 5 steps.  The runs at the extremes (fewest and most cells,
-    fewest and most slices) equal the same decks stepped one run at a time, bit for bit."""
+    """The bench's c5 batch: all 256 runs balanced by the batch autotuner (workgroups per run), then stepped as a
+    Batch with graph replays (the eager start-up step, then four replays, the first of which captures the graph)
+    for the workload's 5 steps.  The runs at the extremes (fewest and most cells, fewest and most slices) and the
+    two shortest and two longest of the runs the plan gave several workgroups equal the same decks stepped one
+    run at a time on one workgroup, bit for bit."""
     from quarz_amd import configs
     runs = configs.c5_runs(256, steps=7, workdir=str(tmp_path))
     sims = [qa.Simulation(deck) for deck, _ in runs]
     batch = qa.Batch(sims, graph=True)
     try:
+        plan = batch.balance()
+        print("plan: est", plan["est_ms"], "ms, one workgroup per run", plan["est_one_ms"], "ms; groups",
+              {g: plan["groups"].count(g) for g in sorted(set(plan["groups"]))})
         for _ in range(5):
             batch.step()
         replays = batch.replays
@@ -476,6 +481,8 @@ def test_c5_bench_batch_extremes_equal_plain(qa, tmp_path):
                    max(range(256), key=lambda i: (info[i]["N"], info[i]["K"])),
                    min(range(256), key=lambda i: (info[i]["K"], info[i]["N"])),
                    max(range(256), key=lambda i: (info[i]["K"], info[i]["N"]))})
+    multi = sorted((i for i in range(256) if plan["groups"][i] > 1), key=lambda i: (info[i]["K"], info[i]["N"]))
+    pick = sorted(set(pick) | set(multi[:2] + multi[-2:]))
     for i in pick:
         p = qa.Simulation(runs[i][0])
         for _ in range(5):
