@@ -360,7 +360,8 @@ void qz_batch_destroy(qz_batch* b);
 int qz_batch_set_workers(qz_batch* b, int max_workers);
 /* Autotuner of the batch's workgroups per run, before the first step.  The runs of one shape (modes, cells,
  * plasma particles) are timed on one of them, the one with the most slices: its first nslices slices with each
- * candidate count, best of two launches.  Each run's sweep is estimated at (time per slice) x (its slices); a
+ * candidate count that leaves at most 1024 plasma particles per workgroup (the multi-workgroup runs share one
+ * launch, which sizes every run's per-workgroup share by the largest), best of two launches.  Each run's sweep is estimated at (time per slice) x (its slices); a
  * plan gives every run whose one-workgroup estimate exceeds a target the fewest workgroups that bring it under
  * the target and list-schedules the other runs, longest first, on the remaining compute units; every estimate
  * is tried as the target and the plan with the smallest estimated sweep is applied to the runs' `groups`.

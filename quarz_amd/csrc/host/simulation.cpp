@@ -124,19 +124,24 @@ static int mobile_particles(const Simulation& s) {
     return np;
 }
 
-// The sweeps of a batched step.  The one-workgroup runs go to `workers` workgroups of one launch, which take
-// them from a device queue longest first (by Simulation::slice_ms when every run has one, else by slices x
-// (cells + particles)); the multi-workgroup runs go to one launch of their own, on a second stream beside it.
-// Every workgroup of either kernel holds a whole compute unit's LDS, and the workers are at most
-// workers_beside(the multi launch's workgroups), so the multi launch's workgroups are all resident together
-// whichever launch starts first.  Device layout of the arguments: the one-workgroup runs in queue order, the
-// multi-workgroup runs, the multi launch's block map, then one block whose first word is the queue counter.
+// The sweeps of a batched step.  The one-workgroup runs are taken from a device queue, longest first (by
+// Simulation::slice_ms when every run has one, else by slices x (cells + particles)), by workgroups that each
+// hold a whole compute unit's LDS.  With multi-workgroup runs, when every one of them needs 2 particles per
+// thread (the batch autotuner's choices always do) all runs share one launch (qz_sweep_mixed_launch): its first
+// `slots` blocks are the multi-workgroup runs' workgroups, packed so that each run's workgroups share a residue
+// class mod kXcds and so one die and its L2, and the blocks the packing leaves empty, and the `workers` blocks
+// after them, work the queue; at most one block per compute unit, so all are resident together.  Otherwise the
+// multi-workgroup runs go to a launch of their own on a second stream, with the queue's workers kept to
+// workers_beside(its workgroups) so that both launches are resident together whichever starts first.  Device
+// layout of the arguments: the one-workgroup runs in queue order, the multi-workgroup runs, the block map, then
+// one block whose first word is the queue counter.
 struct MixedSweep {
     std::vector<qz_sweep_args> single, multi;
     std::vector<Simulation*> multi_runs;
     std::vector<int> map;
     std::vector<qz_sweep_args> stage;   // host image of the device arguments (alive until the step syncs)
-    int workers = 0, multi_blocks = 0;
+    bool fused = false;                 // one launch for all runs
+    int workers = 0, multi_blocks = 0, slots = 0;
     size_t map_offset() const { return single.size() + multi.size(); }
     size_t map_blocks() const { return (sizeof(int) * map.size() + sizeof(qz_sweep_args) - 1) / sizeof(qz_sweep_args); }
     size_t blocks() const { return map_offset() + map_blocks() + 1; }
@@ -161,20 +166,62 @@ static MixedSweep plan_mixed(Simulation* const* runs, int n, int max_workers) {
     }
     std::stable_sort(cost.begin(), cost.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
     for (const auto& c : cost) P.single.push_back(runs[c.second]->sweep_args);
-    const int ncu = device_cus(), ns = static_cast<int>(P.single.size());
-    if (!P.multi.empty()) {
+    const int ncu = device_cus(), ns = static_cast<int>(P.single.size()), nm = static_cast<int>(P.multi.size());
+    const int cap = max_workers > 0 ? max_workers : ncu;   // blocks that work the queue
+    if (nm == 0) {
+        P.workers = std::min({ns, ncu, cap});
+    } else {
         if (P.multi_blocks > ncu)
             throw std::runtime_error("batched sweep: the multi-workgroup runs need " + std::to_string(P.multi_blocks) +
                                      " workgroups, more than the " + std::to_string(ncu) + " compute units");
-        P.map = qz_sweep_block_map(P.multi.data(), static_cast<int>(P.multi.size()));
-        if (ns > 0 && workers_beside(ncu, P.multi_blocks) < 1)
-            throw std::runtime_error("batched sweep: the multi-workgroup runs take every compute unit and leave "
-                                     "none to the one-workgroup runs; give them fewer workgroups");
-        P.workers = std::min(ns, workers_beside(ncu, P.multi_blocks));
-    } else {
-        P.workers = std::min(ns, ncu);
+        P.fused = true;
+        for (Simulation* r : P.multi_runs) P.fused = P.fused && qz_sweep_multi_pt(mobile_particles(*r), r->groups) == 2;
+        if (P.fused) {
+            std::vector<int> order(nm), load(kXcds, 0);
+            for (int r = 0; r < nm; ++r) order[r] = r;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int x, int y) { return P.multi[x].groups > P.multi[y].groups; });
+            std::vector<std::vector<std::pair<int, int>>> die(kXcds);
+            for (int r : order) {
+                const int x = static_cast<int>(std::min_element(load.begin(), load.end()) - load.begin());
+                for (int g = 0; g < P.multi[r].groups; ++g) die[x].emplace_back(r, g);
+                load[x] += P.multi[r].groups;
+            }
+            const int lmax = *std::max_element(load.begin(), load.end());
+            std::vector<std::pair<int, int>> slot;   // (run, group); run -1: no run
+            if (kXcds * lmax <= ncu) {
+                slot.assign(kXcds * lmax, {-1, 0});
+                for (int x = 0; x < kXcds; ++x)
+                    for (size_t j = 0; j < die[x].size(); ++j) slot[x + kXcds * j] = die[x][j];
+            } else {   // no room to pack: the runs' workgroups one after another
+                for (int r = 0; r < nm; ++r)
+                    for (int g = 0; g < P.multi[r].groups; ++g) slot.emplace_back(r, g);
+            }
+            P.slots = static_cast<int>(slot.size());
+            int queue_blocks = 0;
+            for (auto& sl : slot)
+                if (sl.first < 0) {   // an empty slot works the queue (-1) or ends at once (-2)
+                    const bool work = queue_blocks < std::min(ns, cap);
+                    sl.first = work ? -1 : -2;
+                    queue_blocks += work ? 1 : 0;
+                }
+            P.workers = std::max(0, std::min(std::min(ns, cap) - queue_blocks, ncu - P.slots));
+            if (ns > 0 && queue_blocks + P.workers < 1)
+                throw std::runtime_error("batched sweep: the multi-workgroup runs take every compute unit and leave "
+                                         "none to the one-workgroup runs; give them fewer workgroups");
+            P.map.resize(2 * slot.size());
+            for (size_t b = 0; b < slot.size(); ++b) {
+                P.map[2 * b] = slot[b].first;
+                P.map[2 * b + 1] = slot[b].second;
+            }
+        } else {
+            P.map = qz_sweep_block_map(P.multi.data(), nm);
+            if (ns > 0 && workers_beside(ncu, P.multi_blocks) < 1)
+                throw std::runtime_error("batched sweep: the multi-workgroup runs take every compute unit and leave "
+                                         "none to the one-workgroup runs; give them fewer workgroups");
+            P.workers = std::min({ns, workers_beside(ncu, P.multi_blocks), cap});
+        }
     }
-    if (max_workers > 0) P.workers = std::min(P.workers, max_workers);
     P.stage.assign(P.blocks(), qz_sweep_args{});
     std::copy(P.single.begin(), P.single.end(), P.stage.begin());
     std::copy(P.multi.begin(), P.multi.end(), P.stage.begin() + static_cast<long>(P.single.size()));
@@ -193,11 +240,19 @@ static void upload_mixed(const MixedSweep& P, DevBuf<qz_sweep_args>& buf, hipStr
     else hip_check(hipMemcpyAsync(buf.p, P.stage.data(), bytes, hipMemcpyHostToDevice, st), "sweep arguments");
 }
 
-// the two launches (kernels only, so that a capture can hold them): the multi launch forks to `aux` and joins
-// back to `main`
+// the launches (kernels only, so that a capture can hold them): one on `main`, or the multi launch forked to
+// `aux` and joined back to `main` beside the queue's launch
 static void enqueue_mixed(const MixedSweep& P, qz_sweep_args* dbuf, hipStream_t main, hipStream_t aux,
                           hipEvent_t fork, hipEvent_t join) {
     const int ns = static_cast<int>(P.single.size()), nm = static_cast<int>(P.multi.size());
+    unsigned int* queue = reinterpret_cast<unsigned int*>(dbuf + (P.blocks() - 1));
+    if (P.fused) {
+        for (Simulation* s : P.multi_runs) qz_check(qz_zero_async(s->mws.p, 64 * sizeof(double), main));
+        const int* dmap = reinterpret_cast<const int*>(dbuf + P.map_offset());
+        qz_check(qz_sweep_mixed_launch(dbuf, P.single.data(), ns, ns > 0 ? queue : nullptr, dbuf + ns, dmap,
+                                       P.multi.data(), nm, P.slots, P.workers, main));
+        return;
+    }
     if (nm > 0) {
         hip_check(hipEventRecord(fork, main), "fork");
         hip_check(hipStreamWaitEvent(aux, fork, 0), "fork");
@@ -208,7 +263,6 @@ static void enqueue_mixed(const MixedSweep& P, qz_sweep_args* dbuf, hipStream_t 
     }
     if (ns > 0) {
         const bool queued = ns > P.workers;
-        unsigned int* queue = reinterpret_cast<unsigned int*>(dbuf + (P.blocks() - 1));
         qz_check(qz_sweep_launch(dbuf, P.single.data(), ns, main, queued ? P.workers : 0, queued ? queue : nullptr));
     }
     if (nm > 0) hip_check(hipStreamWaitEvent(main, join, 0), "join");
@@ -1849,7 +1903,7 @@ static int step_batched(qz_sim** hs, int n, double* batch_ms, int max_workers) {
             if (i > 0) qz::hip_check(hipStreamWaitEvent(s0.stream, s.ev[0], 0), "stream wait");
         }
         const qz::MixedSweep P = qz::plan_mixed(runs.data(), n, max_workers);
-        if (!P.multi.empty()) ensure_aux(s0);
+        if (!P.multi.empty() && !P.fused) ensure_aux(s0);
         struct Events {   // released on every exit path
             hipEvent_t e[2] = {nullptr, nullptr};
             ~Events() { for (auto x : e) if (x) (void)hipEventDestroy(x); }
@@ -1859,11 +1913,38 @@ static int step_batched(qz_sim** hs, int n, double* batch_ms, int max_workers) {
         hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
         qz::upload_mixed(P, s0.d_args, s0.stream, false);
         qz::hip_check(hipEventRecord(e0, s0.stream), "event");
-        qz::enqueue_mixed(P, s0.d_args.p, s0.stream, s0.aux, s0.aux_ev[0], s0.aux_ev[1]);
+        if (P.multi.empty() || P.fused) {
+            qz::enqueue_mixed(P, s0.d_args.p, s0.stream, s0.aux, s0.aux_ev[0], s0.aux_ev[1]);
+        } else {
+            // two streams of one process can share a hardware queue, and then run one launch after the other;
+            // a graph's independent branches run on streams of their own, so the two launches go through a
+            // one-off graph, as in a captured step
+            struct OneOff {
+                hipGraph_t g = nullptr;
+                hipGraphExec_t x = nullptr;
+                ~OneOff() {
+                    if (x) (void)hipGraphExecDestroy(x);
+                    if (g) (void)hipGraphDestroy(g);
+                }
+            } oo;
+            qz::hip_check(hipStreamBeginCapture(s0.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                qz::enqueue_mixed(P, s0.d_args.p, s0.stream, s0.aux, s0.aux_ev[0], s0.aux_ev[1]);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s0.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            qz::hip_check(hipStreamEndCapture(s0.stream, &oo.g), "end capture");
+            qz::hip_check(hipGraphInstantiate(&oo.x, oo.g, nullptr, nullptr, 0), "graph instantiate");
+            qz::hip_check(hipGraphLaunch(oo.x, s0.stream), "graph launch");
+            qz::hip_check(hipEventRecord(e1, s0.stream), "event");
+            qz::hip_check(hipEventSynchronize(e1), "sync");   // before the one-off graph is destroyed
+        }
         qz::hip_check(hipEventRecord(e1, s0.stream), "event");
         qz::hip_check(hipEventSynchronize(e1), "sync");
-        for (qz::Simulation* s : P.multi_runs) qz::check_multi_sweep(s->mws, s->groups, s->stream);
-        float ms = 0;
+        float ms = 0;   // every multi-workgroup run's error flag is checked by its finish_sweep
         qz::hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
         if (batch_ms) *batch_ms = ms;
         // the rest of every run's step (counters, output, beam push) touches only that run's stream and
@@ -2034,8 +2115,7 @@ extern "C" int qz_batch_step(qz_batch* b, double* step_ms) {
         qz::hip_check(hipGraphLaunch(b->exec, b->st), "graph launch");
         qz::hip_check(hipEventRecord(b->e1, b->st), "event");
         qz::hip_check(hipEventSynchronize(b->e1), "sync");
-        for (qz::Simulation* s : b->plan.multi_runs) qz::check_multi_sweep(s->mws, s->groups, s->stream);
-        float ms = 0;
+        float ms = 0;   // every multi-workgroup run's error flag is checked by its finish_sweep
         qz::hip_check(hipEventElapsedTime(&ms, b->e0, b->e1), "elapsed");
         ++b->replays;
         if (step_ms) *step_ms = ms;
@@ -2140,7 +2220,7 @@ extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nsl
 
 // Autotuner of a batch's workgroups per run (quarz_amd.h).  Runs of one shape (modes, cells, plasma particles)
 // share a time per slice; the shape's run with the most slices is probed over its first nslices slices with
-// every candidate count (best of two launches).  A run's sweep is then estimated at (time per slice) x (its
+// every candidate count that keeps the smallest particle share (best of two launches).  A run's sweep is then estimated at (time per slice) x (its
 // slices).  A plan gives every run whose one-workgroup estimate exceeds a target T the fewest workgroups that
 // bring it under T, and the other runs to the remaining compute units, list-scheduled longest first; its cost
 // is the larger of T and that schedule's makespan.  Every estimate of every run and count is tried as T, and
@@ -2168,7 +2248,21 @@ extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nsl
             for (int i : kv.second)
                 if (b->sims[i]->s->K > b->sims[rep]->s->K) rep = i;
             qz::Simulation& r = *b->sims[rep]->s;
-            probe_groups(r, cand, ncand, nslices, 2, per.data());
+            // the multi-workgroup runs share one launch, whose particle share per workgroup follows the largest
+            // particles-per-thread of its runs: only counts at the smallest (1024 particles or fewer per
+            // workgroup) are offered, so that every run keeps its own share
+            std::vector<int> ok_cand;
+            std::vector<int> at;
+            for (int c = 0; c < ncand; ++c)
+                if (cand[c] == 1 || qz_sweep_multi_pt(kv.first[2], cand[c]) == 2) {
+                    ok_cand.push_back(cand[c]);
+                    at.push_back(c);
+                }
+            std::vector<double> got(ok_cand.size(), -1);
+            if (!ok_cand.empty())
+                probe_groups(r, ok_cand.data(), static_cast<int>(ok_cand.size()), nslices, 2, got.data());
+            std::fill(per.begin(), per.end(), -1.0);
+            for (size_t j = 0; j < at.size(); ++j) per[at[j]] = got[j];
             const double sampled = std::min(nslices, r.K);
             for (int i : kv.second) {
                 qz::Simulation& s = *b->sims[i]->s;
@@ -2197,7 +2291,7 @@ extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nsl
                 used += cand[pick];
                 top = std::max(top, est[i][pick]);
             }
-            const int free_cus = used > 0 ? qz::workers_beside(ncu, used) : ncu;
+            const int free_cus = ncu - used;   // the one launch of qz::plan_mixed
             if (used > ncu || (!singles.empty() && free_cus < 1)) return 1e300;
             std::sort(singles.begin(), singles.end(), std::greater<double>());
             const int w = std::min(static_cast<int>(singles.size()), free_cus);

@@ -33,6 +33,21 @@ int qz_sweep_multi_launch(const qz_sweep_args* dev_runs, const int* dev_map, con
                           hipStream_t stream, int resident = 0);
 // the block map of a multi-workgroup launch of these runs (2 ints per block)
 std::vector<int> qz_sweep_block_map(const qz_sweep_args* runs, int nruns);
+// particles per thread (2, 4 or 8) the multi-workgroup kernel needs for np particles on `groups` workgroups; one
+// launch takes the largest of its runs', and workgroup g of every run owns particles [g P, (g + 1) P),
+// P = 512 x that value, so runs of one launch are best given workgroup counts of one value
+int qz_sweep_multi_pt(int np, int groups);
+namespace qz {
+long sweep_lds_needed(bool m1, int N, int pt, int L);   // LDS bytes of the one-workgroup kernel
+long sweep_multi_lds_needed(bool m1, int N, int pt);    // LDS bytes of the multi-workgroup kernel
+}  // namespace qz
+// One launch of a batch with runs of one and of several workgroups (qz_sweep_mixed.hip): `slots` blocks for the
+// multi-workgroup runs (dev_map: 2 ints per slot; run -1: a slot that works the queue, -2: one that ends), then
+// `workers` blocks; every block without a run takes the one-workgroup runs from the device counter `queue`, in
+// order.  Every multi-workgroup run must need 2 particles per thread (qz_sweep_multi_pt).
+int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* host_single, int nsingle,
+                          unsigned int* queue, const qz_sweep_args* dev_multi, const int* dev_map,
+                          const qz_sweep_args* host_multi, int nmulti, int slots, int workers, hipStream_t stream);
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);
 // zero `bytes` (a multiple of 4) at p with a kernel: unlike a memset node it is safe inside a captured graph
 int qz_zero_async(void* p, size_t bytes, hipStream_t st);

@@ -131,11 +131,11 @@ GAIN_BASE = {"time.steps": 5, "driver.nparticles": 20000, "witness.nparticles": 
 
 
 def _batch_vs_plain(qa, deck, batch_ovs, plain_ovs, steps, workers=0, balance=None):
-    """A graph Batch against the same decks stepped one run at a time on one workgroup, bit for bit after every
-    step (the eager start-up step, the replays, the first of which captures, and the eager last step)."""
+    """A graph Batch against the same decks stepped one run at a time (on one workgroup where the run fits one),
+    bit for bit after every step (the eager start-up step, the replays, the first of which captures, and the
+    eager last step)."""
     plain = [qa.Simulation(deck, ov) for ov in plain_ovs]
     runs = [qa.Simulation(deck, ov) for ov in batch_ovs]
-    assert all(p.groups == 1 for p in plain)
     batch = qa.Batch(runs, graph=True, workers=workers)
     try:
         plan = batch.balance(**balance) if balance else None
@@ -190,4 +190,17 @@ def test_balanced_batch_equals_plain(qa, decks):
     assert plan["groups"] == [r.groups for r in runs]
     assert all(g in (1, 2, 3, 4) for g in plan["groups"])
     assert 0 < plan["est_ms"] <= plan["est_one_ms"] * (1 + 1e-12)
+    assert replays == 4
+
+
+def test_mixed_batch_two_launches_equals_plain(qa, decks):
+    """A multi-workgroup run with more than 1024 plasma particles per workgroup (8 per cell on the gain grid, over
+    4096, on 2 workgroups: 4 particles per thread) cannot share the one mixed launch, so its sweep goes to a
+    launch of its own beside the queue's: every run still equals its plain run after every step."""
+    ovs = [dict(GAIN_BASE, **o) for o in GAIN_RUNS[:3]]
+    ovs[0]["electrons.ppc"] = 8
+    forced = [dict(o) for o in ovs]
+    forced[0]["sweep.groups"] = 2
+    runs, _, replays = _batch_vs_plain(qa, f"{decks}/gain.in", forced, ovs, 6, workers=1)
+    assert [r.groups for r in runs] == [2, 1, 1]
     assert replays == 4
