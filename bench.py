@@ -318,11 +318,6 @@ def bench_gradient(args):
         "adam": {"iterations": iters, "lr": 0.02, "J_initial": hist[0], "J_final": float(J_final),
                  "profile_final": [float(x) for x in v]},
         "launch_shapes": shapes,
-        "balance": None if plan is None else {
-            "groups_histogram": {str(g): plan["groups"].count(g) for g in sorted(set(plan["groups"]))},
-            "multi_runs": [{"N": runs[i][1]["N"], "K": runs[i][1]["K"], "groups": g}
-                           for i, g in enumerate(plan["groups"]) if g > 1],
-            "est_ms": plan["est_ms"], "est_one_group_ms": plan["est_one_ms"], "tune_s": plan["seconds"]},
         "fd_plain_particle_slices_per_s": ppsl / pwall,
         "fd_gradient_batched_vs_plain": pwall / (fd_ms / 1e3),
         "adjoint_vs_batched_fd": (fd_ms / 1e3) / (wall / iters),
@@ -502,11 +497,6 @@ def main():
         "beam_push_ms": beam_ms / args.steps,
         "sweep_particle_slices_per_s": psl / (sweep_ms / 1e3),
         "launch_shapes": shapes,
-        "balance": None if plan is None else {
-            "groups_histogram": {str(g): plan["groups"].count(g) for g in sorted(set(plan["groups"]))},
-            "multi_runs": [{"N": runs[i][1]["N"], "K": runs[i][1]["K"], "groups": g}
-                           for i, g in enumerate(plan["groups"]) if g > 1],
-            "est_ms": plan["est_ms"], "est_one_group_ms": plan["est_one_ms"], "tune_s": plan["seconds"]},
         "sweep_groups": sim.groups,
         "sweep_group_times_ms": group_times,
         "roofline": {"bound": "fp64-valu", "achieved": achieved / 1e12, "peak": FP64_PEAK / 1e12, "unit": "TFLOP/s",
