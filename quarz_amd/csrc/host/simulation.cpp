@@ -126,8 +126,9 @@ static int mobile_particles(const Simulation& s) {
 
 // The sweeps of a batched step.  The one-workgroup runs are taken from a device queue, longest first (by
 // Simulation::slice_ms when every run has one, else by slices x (cells + particles)), by workgroups that each
-// hold a whole compute unit's LDS.  With multi-workgroup runs, when every one of them needs 2 particles per
-// thread (the batch autotuner's choices always do) all runs share one launch (qz_sweep_mixed_launch): its first
+// hold a whole compute unit's LDS, each run swept with its own particles per thread (qz_sweep_mixed_launch).
+// With multi-workgroup runs, when every one of them needs 2 particles per thread (the batch autotuner's choices
+// always do) all runs share that one launch: its first
 // `slots` blocks are the multi-workgroup runs' workgroups, packed so that each run's workgroups share a residue
 // class mod kXcds and so one die and its L2, and the blocks the packing leaves empty, and the `workers` blocks
 // after them, work the queue; at most one block per compute unit, so all are resident together.  Otherwise the
@@ -169,6 +170,7 @@ static MixedSweep plan_mixed(Simulation* const* runs, int n, int max_workers) {
     const int ncu = device_cus(), ns = static_cast<int>(P.single.size()), nm = static_cast<int>(P.multi.size());
     const int cap = max_workers > 0 ? max_workers : ncu;   // blocks that work the queue
     if (nm == 0) {
+        P.fused = true;
         P.workers = std::min({ns, ncu, cap});
     } else {
         if (P.multi_blocks > ncu)
@@ -261,10 +263,9 @@ static void enqueue_mixed(const MixedSweep& P, qz_sweep_args* dbuf, hipStream_t 
         qz_check(qz_sweep_multi_launch(dbuf + ns, dmap, P.multi.data(), nm, aux, P.workers + P.multi_blocks));
         hip_check(hipEventRecord(join, aux), "join");
     }
-    if (ns > 0) {
-        const bool queued = ns > P.workers;
-        qz_check(qz_sweep_launch(dbuf, P.single.data(), ns, main, queued ? P.workers : 0, queued ? queue : nullptr));
-    }
+    if (ns > 0)
+        qz_check(qz_sweep_mixed_launch(dbuf, P.single.data(), ns, queue, nullptr, nullptr, nullptr, 0, 0, P.workers,
+                                       main));
     if (nm > 0) hip_check(hipStreamWaitEvent(main, join, 0), "join");
 }
 

@@ -22,9 +22,8 @@ int qz_fail_hip(hipError_t e, const std::string& where);
 constexpr int QZ_SWEEP_WS_ARRAYS = 56;
 constexpr int QZ_NOALLOC_MAX_NODES = 2049;
 size_t qz_sweep_workspace_doubles_impl(int M);
-// workers / queue: with a device counter `queue`, `workers` workgroups take the runs in order from it
-int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream,
-                    int workers = 0, unsigned int* queue = nullptr);
+// one workgroup per run, run b on workgroup b
+int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream);
 int qz_validate_sweep(const qz_sweep_args& a);   // the checks of qz_sweep, QZ_OK or a QZ_E* code
 // the one-workgroup sweep holds a run of np plasma particles on N cells (m1: modes 0 and 1)
 bool qz_sweep_fits_one_group(bool m1, int N, int np);
@@ -44,7 +43,8 @@ long sweep_multi_lds_needed(bool m1, int N, int pt);    // LDS bytes of the mult
 // One launch of a batch with runs of one and of several workgroups (qz_sweep_mixed.hip): `slots` blocks for the
 // multi-workgroup runs (dev_map: 2 ints per slot; run -1: a slot that works the queue, -2: one that ends), then
 // `workers` blocks; every block without a run takes the one-workgroup runs from the device counter `queue`, in
-// order.  Every multi-workgroup run must need 2 particles per thread (qz_sweep_multi_pt).
+// order, each swept with its own particles per thread.  Every multi-workgroup run must need 2 particles per
+// thread (qz_sweep_multi_pt).  nmulti = 0, slots = 0: a queue of one-workgroup runs only.
 int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* host_single, int nsingle,
                           unsigned int* queue, const qz_sweep_args* dev_multi, const int* dev_map,
                           const qz_sweep_args* host_multi, int nmulti, int slots, int workers, hipStream_t stream);

@@ -6,26 +6,10 @@
 
 namespace qz {
 
-// The sweep kernel: workgroup b sweeps run b, or (queue != nullptr) every workgroup takes runs from a shared counter
-// until none is left -- a batch whose runs are ordered longest first then fills fewer compute units evenly
-template <bool M1, int PT, bool QUEUE>
-__global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restrict__ runs, int nruns,
-                                                    unsigned int* __restrict__ queue) {
-    if (!QUEUE) {
-        sweep_run<M1, PT>(runs[blockIdx.x]);
-        return;
-    }
-    extern __shared__ double smem_d[];
-    int* s_next = reinterpret_cast<int*>(smem_d);   // free between runs
-    auto take = [&]() {
-        __syncthreads();
-        if (threadIdx.x == 0) *s_next = static_cast<int>(atomicAdd(queue, 1u));
-        __syncthreads();
-        const int r = *s_next;
-        __syncthreads();
-        return r;
-    };
-    for (int r = take(); r < nruns; r = take()) sweep_run<M1, PT>(runs[r]);
+// The sweep kernel: workgroup b sweeps run b
+template <bool M1, int PT>
+__global__ __launch_bounds__(kBS) void sweep_kernel(const qz_sweep_args* __restrict__ runs) {
+    sweep_run<M1, PT>(runs[blockIdx.x]);
 }
 
 // LDS bytes the kernel needs for N cells, PT particles per thread and L live particles
@@ -54,8 +38,7 @@ bool qz_sweep_fits_one_group(bool m1, int N, int np) {
     return sweep_lds_needed(m1, N, pt, np) <= kLdsBytes;
 }
 
-int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream,
-                    int workers, unsigned int* queue) {
+int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_runs, int nruns, hipStream_t stream) {
     int maxnp = 0, maxN = 0;
     const bool m1 = host_runs[0].modes == 1;
     for (int i = 0; i < nruns; ++i) {
@@ -74,22 +57,12 @@ int qz_sweep_launch(const qz_sweep_args* dev_runs, const qz_sweep_args* host_run
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsBytes);
         if (e != hipSuccess) return qz_fail_hip(e, "hipFuncSetAttribute");
-        const int nb = queue ? std::max(1, std::min(workers, nruns)) : nruns;
-        if (queue) {
-            const int rc = qz_zero_async(queue, sizeof(unsigned int), stream);
-            if (rc) return rc;
-        }
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(kBS), kLdsBytes, stream, dev_runs, nruns, queue);
+        hipLaunchKernelGGL(kern, dim3(nruns), dim3(kBS), kLdsBytes, stream, dev_runs);
         e = hipGetLastError();
         if (e != hipSuccess) return qz_fail_hip(e, "sweep launch");
         return QZ_OK;
     };
-    if (queue) {
-        if (pt == 2) return m1 ? launch(sweep_kernel<true, 2, true>) : launch(sweep_kernel<false, 2, true>);
-        if (pt == 4) return m1 ? launch(sweep_kernel<true, 4, true>) : launch(sweep_kernel<false, 4, true>);
-        return m1 ? launch(sweep_kernel<true, 8, true>) : launch(sweep_kernel<false, 8, true>);
-    }
-    if (pt == 2) return m1 ? launch(sweep_kernel<true, 2, false>) : launch(sweep_kernel<false, 2, false>);
-    if (pt == 4) return m1 ? launch(sweep_kernel<true, 4, false>) : launch(sweep_kernel<false, 4, false>);
-    return m1 ? launch(sweep_kernel<true, 8, false>) : launch(sweep_kernel<false, 8, false>);
+    if (pt == 2) return m1 ? launch(sweep_kernel<true, 2>) : launch(sweep_kernel<false, 2>);
+    if (pt == 4) return m1 ? launch(sweep_kernel<true, 4>) : launch(sweep_kernel<false, 4>);
+    return m1 ? launch(sweep_kernel<true, 8>) : launch(sweep_kernel<false, 8>);
 }

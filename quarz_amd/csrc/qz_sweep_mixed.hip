@@ -4,8 +4,9 @@
 // host's order (longest first).
 // Every block of the launch holds a whole compute unit's LDS and the launch has at most one block per compute
 // unit, so all its blocks are resident together; blocks are dispatched in order, so the multi-workgroup runs'
-// workgroups never wait behind the queue's.  Each run's arithmetic is that of its own kernel (sweep_run,
-// sweep_multi_run), so every run equals its plain launch bit for bit.
+// workgroups never wait behind the queue's.  A one-workgroup run is swept with the fewest particles per thread
+// it needs (2, 4 or 8: the three instantiations of sweep_run), not the batch's largest.  Each run's arithmetic
+// is that of its own kernel (sweep_run, sweep_multi_run), so every run equals its plain launch bit for bit.
 #include "qz_sweep_single.hpp"
 #include "qz_sweep_multi_body.hpp"
 
@@ -15,7 +16,7 @@ namespace qz {
 
 static_assert(kBS == mBS && kLdsBytes == mLds, "both sweep bodies run in blocks of one shape");
 
-template <bool M1, int PTS>
+template <bool M1>
 __global__ __launch_bounds__(kBS) void sweep_mixed_kernel(const qz_sweep_args* __restrict__ single, int nsingle,
                                                           unsigned int* __restrict__ queue,
                                                           const qz_sweep_args* __restrict__ multi,
@@ -39,7 +40,14 @@ __global__ __launch_bounds__(kBS) void sweep_mixed_kernel(const qz_sweep_args* _
         __syncthreads();
         return r;
     };
-    for (int r = take(); r < nsingle; r = take()) sweep_run<M1, PTS>(single[r]);
+    for (int r = take(); r < nsingle; r = take()) {   // each run with the fewest particles per thread it needs
+        const qz_sweep_args& A = single[r];
+        int np = 0;
+        for (int sp = 0; sp < A.nspecies; ++sp) np += A.sp[sp].np;
+        if (np <= 2 * kBS) sweep_run<M1, 2>(A);
+        else if (np <= 4 * kBS) sweep_run<M1, 4>(A);
+        else sweep_run<M1, 8>(A);
+    }
 }
 
 }  // namespace qz
@@ -54,14 +62,17 @@ int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* 
     const qz_sweep_args* first = nsingle > 0 ? host_single : host_multi;
     if (!first) return qz_fail(QZ_EINVAL, "mixed sweep: no runs");
     const bool m1 = first->modes == 1;
-    int maxnp = 0, maxN = 0, maxNm = 0, need_slots = 0;
+    int maxNm = 0, need_slots = 0;
     for (int i = 0; i < nsingle; ++i) {
         const qz_sweep_args& a = host_single[i];
         if ((a.modes == 1) != m1) return qz_fail(QZ_EINVAL, "mixed sweep: all runs need the same modes");
         int np = 0;
         for (int s = 0; s < a.nspecies; ++s) np += a.sp[s].np;
-        maxnp = std::max(maxnp, np);
-        maxN = std::max(maxN, a.grid.N);
+        if (np > kBS * 8)
+            return qz_fail(QZ_ERANGE, "mixed sweep: more than 4096 plasma particles in a one-workgroup run");
+        const int pt = np <= 2 * kBS ? 2 : (np <= 4 * kBS ? 4 : 8);   // as the kernel picks it
+        if (sweep_lds_needed(m1, a.grid.N, pt, np) > kLdsBytes)
+            return qz_fail(QZ_ERANGE, "mixed sweep: a one-workgroup run does not fit the LDS of one workgroup");
     }
     for (int i = 0; i < nmulti; ++i) {
         const qz_sweep_args& a = host_multi[i];
@@ -77,11 +88,6 @@ int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* 
     }
     if (need_slots > slots) return qz_fail(QZ_EINVAL, "mixed sweep: fewer slots than multi-workgroup workgroups");
     if (nsingle > 0 && workers + slots - need_slots < 1) return qz_fail(QZ_EINVAL, "mixed sweep: no worker");
-    if (maxnp > kBS * 8)
-        return qz_fail(QZ_ERANGE, "mixed sweep: more than 4096 plasma particles in a one-workgroup run");
-    const int pt = maxnp <= 2 * kBS ? 2 : (maxnp <= 4 * kBS ? 4 : 8);
-    if (nsingle > 0 && sweep_lds_needed(m1, maxN, pt, maxnp) > kLdsBytes)
-        return qz_fail(QZ_ERANGE, "mixed sweep: a one-workgroup run does not fit the LDS of one workgroup");
     if (nmulti > 0 && sweep_multi_lds_needed(m1, maxNm, 2) > mLds)
         return qz_fail(QZ_ERANGE, "mixed sweep: a grid does not fit the LDS of the multi-workgroup sweep");
     auto launch = [&](auto kern) -> int {
@@ -106,7 +112,5 @@ int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* 
         QZ_HIP(hipGetLastError());
         return QZ_OK;
     };
-    if (pt == 2) return m1 ? launch(sweep_mixed_kernel<true, 2>) : launch(sweep_mixed_kernel<false, 2>);
-    if (pt == 4) return m1 ? launch(sweep_mixed_kernel<true, 4>) : launch(sweep_mixed_kernel<false, 4>);
-    return m1 ? launch(sweep_mixed_kernel<true, 8>) : launch(sweep_mixed_kernel<false, 8>);
+    return m1 ? launch(sweep_mixed_kernel<true>) : launch(sweep_mixed_kernel<false>);
 }
