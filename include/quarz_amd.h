@@ -338,6 +338,13 @@ int qz_sim_stats(const qz_sim* s, double* out7);
  * contributions, deposit node sums, combine, B_z rhs, psi/E_z/B_z solves, W+, S+ gather, S+ contributions,
  * S+ node sums, Picard staging, Picard rhs, Picard solves, store, push */
 int qz_sim_phase_ticks(const qz_sim* s, double* out16);
+/* The schedule of the last qz_sim_run_pipelined, per range launch (step-major: launch i = step i / nchunks,
+ * range i % nchunks), four times in ms from HIP events on the steps' streams, relative to the first range's
+ * deposit start: the range's beam deposit starts (its stream's earlier work and the previous step's push it
+ * waits for are done), its sweep starts, its sweep ends, its beam push ends.  Writes min(count, cap) quadruples
+ * to out (optional) and the ranges per step to *nchunks (optional); returns the count (0 before any pipelined
+ * run), negative on error.  Not in the reference. */
+long qz_sim_pipeline_timeline(const qz_sim* s, double* out, long cap, int* nchunks);
 long qz_sim_lost(const qz_sim* s);
 int qz_sim_groups(const qz_sim* s);   /* workgroups per run of the sweep (1: the one-workgroup kernel) */
 double qz_sim_time(const qz_sim* s);

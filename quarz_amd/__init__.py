@@ -118,6 +118,8 @@ def lib():
         L.qz_sim_timing.argtypes = [C.c_void_p, DP]
         L.qz_sim_stats.argtypes = [C.c_void_p, DP]
         L.qz_sim_phase_ticks.argtypes = [C.c_void_p, DP]
+        L.qz_sim_pipeline_timeline.argtypes = [C.c_void_p, DP, C.c_long, C.POINTER(C.c_int)]
+        L.qz_sim_pipeline_timeline.restype = C.c_long
         L.qz_sim_groups.argtypes = [C.c_void_p]
         L.qz_sim_lost.restype = C.c_long
         L.qz_sim_lost.argtypes = [C.c_void_p]
@@ -279,6 +281,16 @@ class Simulation:
         out = np.zeros(2)
         check(lib().qz_sim_run_pipelined(self.h, int(steps), int(chunks), int(depth), out.ctypes.data_as(DP)))
         return float(out[0]), float(out[1])
+
+    def pipeline_timeline(self):
+        """The last run_pipelined's schedule: an array (steps, ranges, 4) of ms per range launch -- deposit start,
+        sweep start, sweep end, push end -- relative to the first range's deposit start."""
+        nc = C.c_int(0)
+        n = lib().qz_sim_pipeline_timeline(self.h, None, 0, C.byref(nc))
+        check(int(min(n, 0)))
+        out = np.zeros(4 * n)
+        lib().qz_sim_pipeline_timeline(self.h, out.ctypes.data_as(DP), n, None)
+        return out.reshape(-1, max(nc.value, 1), 4) if n else out.reshape(0, 0, 4)
 
     def phase_ms(self):
         """Milliseconds of each sweep phase of the last sweep (one thread's wall clock, 100 MHz ticks)."""

@@ -1383,8 +1383,8 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     }
     DevBuf<qz_sweep_args> dargs(hargs.size());
     hip_check(hipMemcpy(dargs.p, hargs.data(), sizeof(qz_sweep_args) * hargs.size(), hipMemcpyHostToDevice), "arguments");
-    // a range of a run of several workgroups: its block map (one run), and before each launch the barrier counter
-    // back to zero -- the error flag is kept, so a barrier that gave up in any range of a step is reported below
+    // a range of a run of several workgroups: its block map (one run); the error flag is never cleared between
+    // the ranges, so a barrier that gave up in any range of a step is reported below
     DevBuf<int> dmap;
     if (groups > 1) {
         const std::vector<int> map = qz_sweep_block_map(hargs.data(), 1);
@@ -1404,7 +1404,7 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
                 if (x) (void)hipEventDestroy(x);
         }
     } lev;
-    lev.e.assign(2 * static_cast<size_t>(nrun) * nchunks, nullptr);
+    lev.e.assign(4 * static_cast<size_t>(nrun) * nchunks, nullptr);   // sweep start, end; deposit start; push end
     for (auto& x : lev.e) hip_check(hipEventCreate(&x), "event");
     for (int i = 0; i < nrun; ++i) {
         StepCtx& C = *ctx[i % depth];
@@ -1440,33 +1440,37 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
             const int c0 = cb[c], c1 = cb[c + 1];
             // beam sources of rows [d0, d1]: up to row c1, which the range's last slice differentiates
             if (P) hip_check(hipStreamWaitEvent(st, P->pushed[std::min(c + 1, nchunks - 1)], 0), "wait");
+            const size_t ai = static_cast<size_t>(i) * nchunks + c;
+            const size_t ne = 2 * static_cast<size_t>(nrun) * nchunks;   // the deposit and push marks follow
+            hip_check(hipEventRecord(lev.e[ne + 2 * ai], st), "event");
             const int d0 = c == 0 ? 0 : c0 + 1, d1 = std::min(c1, K - 1);
             // the counts of Simulation::step (live beam particles before the deposit and before the push) are
             // taken by the kernels themselves: each window counts the live particles whose lower slice it holds,
             // each range's push the particles it marks
-            if (d0 <= d1) {
-                qz_check(qz_zero_async(C.bsrc.p + d0 * row, sizeof(Real) * row * static_cast<size_t>(d1 - d0 + 1), st));
+            if (d0 <= d1) {   // the window's rows are overwritten: no zero fill first
                 qz_beam_grid bx = box;
                 bx.k0 = box.k0 + d0;
                 bx.nloc = d1 - d0 + 1;
                 if (!dq.empty())
-                    qz_check(qz_beam_deposit_beams(&g, &bx, mm, static_cast<int>(dq.size()), dq.data(), dnp.data(),
-                                                   dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, dpstep.data(),
-                                                   marker, cnt + 20, st));
+                    qz_check(qz_beam_deposit_window(&g, &bx, mm, static_cast<int>(dq.size()), dq.data(), dnp.data(),
+                                                    dsoa.data(), C.bsrc.p + d0 * row, NCB, C.work.p, dpstep.data(),
+                                                    marker, cnt + 20, st));
+                else
+                    qz_check(qz_zero_async(C.bsrc.p + d0 * row, sizeof(Real) * row * static_cast<size_t>(d1 - d0 + 1),
+                                           st));
             }
-            const size_t ai = static_cast<size_t>(i) * nchunks + c;
             hip_check(hipEventRecord(lev.e[2 * ai], st), "event");
             if (groups > 1) {
-                char* m = reinterpret_cast<char*>(C.mws.p);
-                qz_check(qz_zero_async(m, 8, st));                      // barrier counter
-                qz_check(qz_zero_async(m + 12, 64 * sizeof(Real) - 12, st));
-                // steps in flight on their own streams: ordinary launches whose workgroups all fit the device
+                // steps in flight on their own streams: ordinary launches whose workgroups all fit the device.  The
+                // barrier counter is zero: zeroed with the workspace, and put back to zero by each launch's last
+                // workgroup out
                 qz_check(qz_sweep_multi_launch(dargs.p + ai, dmap.p, hargs.data() + ai, 1, st, depth * groups));
             } else {
                 qz_check(qz_sweep_launch(dargs.p + ai, hargs.data() + ai, 1, st));
             }
             hip_check(hipEventRecord(lev.e[2 * ai + 1], st), "event");
             if (!last) push_range(c);
+            hip_check(hipEventRecord(lev.e[ne + 2 * ai + 1], st), "event");
         }
     }
     for (auto& cp : ctx) hip_check(hipStreamSynchronize(cp->st), "sync");
@@ -1500,11 +1504,17 @@ void Simulation::run_pipelined(int nrun, int nchunks, int depth, double& psl_tot
     hip_check(hipMemcpyAsync(bsrc.p, Lc.bsrc.p, sizeof(Real) * bsrc.n, hipMemcpyDeviceToDevice, stream), "bsrc copy");
     hip_check(hipStreamSynchronize(stream), "sync");
     double launch_ms = 0;
-    for (size_t ai = 0; ai < static_cast<size_t>(nrun) * nchunks; ++ai) {
-        float ms = 0;
+    const size_t nl = static_cast<size_t>(nrun) * nchunks;
+    pipe_timeline.assign(4 * nl, 0.0);
+    for (size_t ai = 0; ai < nl; ++ai) {
+        float ms = 0, t[4] = {0, 0, 0, 0};
         hip_check(hipEventElapsedTime(&ms, lev.e[2 * ai], lev.e[2 * ai + 1]), "elapsed");
+        const hipEvent_t marks[4] = {lev.e[2 * nl + 2 * ai], lev.e[2 * ai], lev.e[2 * ai + 1], lev.e[2 * nl + 2 * ai + 1]};
+        for (int m = 0; m < 4; ++m) hip_check(hipEventElapsedTime(&t[m], lev.e[2 * nl], marks[m]), "elapsed");
+        for (int m = 0; m < 4; ++m) pipe_timeline[4 * ai + m] = t[m];
         launch_ms += ms;
     }
+    pipe_chunks = nchunks;
     ms_sweep = static_cast<float>(launch_ms / nrun);   // per step: the sum of its range launches' kernel times
     sweep_launches = nchunks;
     ms_beam = ms_beam_dep = 0;
@@ -2541,6 +2551,17 @@ extern "C" int qz_sim_stats(const qz_sim* h, double* out7) {
     out7[6] = s.sweep_launches;
     return QZ_OK;
 }
+extern "C" long qz_sim_pipeline_timeline(const qz_sim* h, double* out, long cap, int* nchunks) {
+    if (!h) return qz_fail(QZ_EINVAL, "qz_sim_pipeline_timeline: null run");
+    const std::vector<double>& t = h->s->pipe_timeline;
+    const long n = static_cast<long>(t.size() / 4);
+    if (out)
+        for (long i = 0; i < std::min(n, cap); ++i)
+            for (int m = 0; m < 4; ++m) out[4 * i + m] = t[4 * i + m];
+    if (nchunks) *nchunks = h->s->pipe_chunks;
+    return n;
+}
+
 extern "C" int qz_sim_phase_ticks(const qz_sim* h, double* out) {
     if (!h || !out) return qz_fail(QZ_EINVAL, "qz_sim_phase_ticks: null argument");
     for (int i = 0; i < QZ_NPHASE; ++i) out[i] = h->s->phase_ticks[i];

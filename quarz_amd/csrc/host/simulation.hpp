@@ -208,6 +208,8 @@ public:
     long lost_last = 0;
     float ms_sweep = 0, ms_beam = 0, ms_beam_dep = 0;
     int sweep_launches = 1;      // sweep kernel launches of the last step (pipelined: its slice ranges)
+    std::vector<double> pipe_timeline;   // last run_pipelined: 4 marks (ms) per range launch, step-major
+    int pipe_chunks = 0;
     int groups = 1;              // workgroups of the sweep (1: the one-workgroup kernel)
     DevBuf<Real> mws;            // groups > 1: the multi-workgroup workspace
     double slice_ms = 0;         // a batch's estimate of the one-workgroup sweep's time per slice (0: none)

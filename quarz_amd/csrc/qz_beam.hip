@@ -320,7 +320,8 @@ __global__ void __launch_bounds__(kRThreads) radix_scatter_kernel(const int* __r
 
 // deposit step 4: one thread per (local slice, node): the four buckets touching the node, merged by index
 __global__ void beam_node_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, const int* __restrict__ start,
-                                 const int* __restrict__ list, const Real* __restrict__ rec, Real* __restrict__ src, int nc) {
+                                 const int* __restrict__ list, const Real* __restrict__ rec, Real* __restrict__ src, int nc,
+                                 int overwrite) {
     const int M = g.N + 1, N = g.N;
     const long node = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (node >= static_cast<long>(bg.nloc) * M) return;
@@ -338,10 +339,14 @@ __global__ void beam_node_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, cons
             lo[nb] = start[key]; hi[nb] = start[key + 1]; ca[nb] = a; cb[nb] = b;
             if (lo[nb] < hi[nb]) ++nb;
         }
-    if (nb == 0) return;
     Real* S = src + static_cast<size_t>(node) * nc;
-    Real acc[15];
-    for (int c = 0; c < nc; ++c) acc[c] = S[c];
+    if (nb == 0) {   // no particle: the node keeps what src holds, or is zero when the deposit overwrites
+        if (overwrite)
+            for (int c = 0; c < nc; ++c) S[c] = Real(0);
+        return;
+    }
+    Real acc[15];   // from zero when overwriting: +0 + x is x, as the zero-filled source plus x
+    for (int c = 0; c < nc; ++c) acc[c] = overwrite ? Real(0) : S[c];
     const Real iv = g.inv_V[jj];
     while (true) {
         // next particle in index order among the buckets
@@ -433,7 +438,7 @@ __global__ void scan_finish(const int* __restrict__ in, int n, const int* __rest
 
 // ---- the same exclusive scan in one block when n <= 1024 * kScanOne (a deposit restricted to a few rows): each
 // thread sums a contiguous chunk, the block scans the chunk sums, each thread writes its chunk's prefixes
-constexpr int kScanOne = 64;
+constexpr int kScanOne = 128;   // up to 131072 ints: a radix histogram of 1.5e6 particles stays one launch
 __global__ void __launch_bounds__(1024) scan_one_block(const int* __restrict__ in, int n, int* __restrict__ out) {
     __shared__ int sh[1024];
     const int t = threadIdx.x;
@@ -585,7 +590,8 @@ extern "C" size_t qz_beam_deposit_workspace_bytes(int np, int nloc, int N) {
 // the first beam's contributions in particle order, then the next beam's, as nbeams deposits one after another do.
 static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams, const double* q,
                          const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
-                         const int* const* pstep, int marker, unsigned long long* live, hipStream_t st) {
+                         const int* const* pstep, int marker, unsigned long long* live, hipStream_t st,
+                         bool overwrite = false) {
     long ntot = 0;
     for (int b = 0; b < nbeams; ++b) {
         if (np[b] < 0) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
@@ -595,7 +601,9 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
     if (!grid || !box || !bsrc || (ntot > 0 && !work)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: null argument");
     if (grid->N < 2 || box->nloc < 1) return qz_fail(QZ_EINVAL, "qz_beam_deposit: bad extent");
     if (nc != (modes == 1 ? B_NC1 : B_NC0)) return qz_fail(QZ_EINVAL, "qz_beam_deposit: nc must be 5 (m = 0) or 15 (m = 1)");
-    if (ntot == 0) return QZ_OK;
+    if (ntot == 0) return overwrite ? qz_zero_async(bsrc, sizeof(Real) * static_cast<size_t>(box->nloc) *
+                                                          (grid->N + 1) * nc, st)
+                                    : QZ_OK;
     const int npt = static_cast<int>(ntot);
     const DepWork L = dep_layout(npt, box->nloc, grid->N);
     char* w = static_cast<char*>(work);
@@ -652,9 +660,16 @@ static int deposit_beams(const qz_radial_grid* grid, const qz_beam_grid* box, in
     }
     const long nodes = static_cast<long>(box->nloc) * (grid->N + 1);
     hipLaunchKernelGGL(beam_node_kernel, dim3(static_cast<unsigned>((nodes + tnode - 1) / tnode)), dim3(tnode), 0, st, *grid, *box,
-                       modes == 1 ? 1 : 0, start, list, rec, bsrc, nc);
+                       modes == 1 ? 1 : 0, start, list, rec, bsrc, nc, overwrite ? 1 : 0);
     QZ_HIP(hipGetLastError());
     return QZ_OK;
+}
+
+int qz_beam_deposit_window(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams,
+                           const double* q, const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
+                           const int* const* pstep, int marker, unsigned long long* live_count, hipStream_t st) {
+    if (nbeams < 1 || !q || !np || !p) return qz_fail(QZ_EINVAL, "qz_beam_deposit_window: null argument or no beam");
+    return deposit_beams(grid, box, modes, nbeams, q, np, p, bsrc, nc, work, pstep, marker, live_count, st, true);
 }
 
 extern "C" int qz_beam_deposit(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, double q, int np,

@@ -49,6 +49,11 @@ int qz_sweep_mixed_launch(const qz_sweep_args* dev_single, const qz_sweep_args* 
                           unsigned int* queue, const qz_sweep_args* dev_multi, const int* dev_map,
                           const qz_sweep_args* host_multi, int nmulti, int slots, int workers, hipStream_t stream);
 int qz_beam_count_live(int np, const double* w, unsigned long long* count, hipStream_t st);
+// qz_beam_deposit_beams into the rows [box->k0, box->k0 + box->nloc) at bsrc, overwriting them (every node of
+// the window written, nodes without particles zero) instead of adding to what they hold: no zero fill first
+int qz_beam_deposit_window(const qz_radial_grid* grid, const qz_beam_grid* box, int modes, int nbeams,
+                           const double* q, const int* np, const qz_beam_soa* p, double* bsrc, int nc, void* work,
+                           const int* const* pstep, int marker, unsigned long long* live_count, hipStream_t st);
 // zero `bytes` (a multiple of 4) at p with a kernel: unlike a memset node it is safe inside a captured graph
 int qz_zero_async(void* p, size_t bytes, hipStream_t st);
 // out[r][j] = s0 a[K-1-r][j][c0] + s1 a[K-1-r][j][c1] (s1 = 0: one term), a of K x M x nc doubles, on the device

@@ -21,8 +21,8 @@
 //      previous iterate of the coupled modes from global memory and writing the next one to the other of two
 //      buffers (one grid barrier per iteration);
 //   7. the field store (node ranges per workgroup) and the Adams-Bashforth push of the owned particles.
-// The grid barrier is a counter in the run's workspace that only grows: barrier n of the launch completes when
-// it reaches n G.  Every wait is bounded in time: a barrier not reached within two seconds marks the run's error
+// The grid barrier is a counter in the run's workspace that only grows during a launch: barrier n of the launch
+// completes when it reaches n G; the last workgroup to leave the launch puts it back to zero.  Every wait is bounded in time: a barrier not reached within two seconds marks the run's error
 // flag and ends the launch in every workgroup, and the host reports it.  A launch is refused unless all its
 // workgroups (and those of the launches the caller keeps in flight beside it) fit the device together, one per
 // compute unit, so every workgroup is resident.  Arithmetic per term is that of the one-workgroup kernel
@@ -51,7 +51,7 @@ constexpr unsigned long long kBarrierTicks = 200000000ull;   // 2 s of the 100 M
 struct MwsLayout {
     size_t misc = 0, cnt, lists, bpalt, total;
     __host__ __device__ MwsLayout(int M, int N, int np, int G, bool m1) {
-        cnt = 64;                                                         // [0] barrier, [1] error, [8..12) bw0
+        cnt = 64;   // words 0 barrier, 1 exit count, 2 error; doubles [8..12) bw0
         lists = cnt + (static_cast<size_t>(G) * (N + 1) + 1) / 2 + 1;     // G x N ints
         bpalt = lists + static_cast<size_t>(kDepComps) * 2 * (np > 1 ? np : 1);
         total = bpalt + static_cast<size_t>(m1 ? 6 : 2) * M;
@@ -852,6 +852,16 @@ __device__ __forceinline__ void sweep_multi_run(const qz_sweep_args* __restrict_
             }
         }
         __syncthreads();
+    }
+    // every workgroup has passed its last barrier: the last one out puts the barrier counter (word 0) and the
+    // exit counter (word 1) back to zero, so the next launch on this workspace needs no reset of its own (the
+    // error flag, word 2, stays)
+    if (threadIdx.x == 0) {
+        const unsigned int out = __hip_atomic_fetch_add(bar + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (out + 1 == static_cast<unsigned int>(G)) {
+            __hip_atomic_store(bar, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(bar + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
