@@ -41,10 +41,10 @@ FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-s
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
 # same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
 # Keyed by (workload, workgroups per run); per whole sweep (every slice of one step).
-MEASURED_TRAFFIC = {("c2", 1): 841.9e6, ("c2", 4): 20163.2e6}
+MEASURED_TRAFFIC = {("c2", 1): 841.9e6, ("c2", 4): 20163.2e6, ("c2", 6): 21787.9e6}
 # FP64 VALU lane operations per sweep (every slice of one step), 64 x SQ_INSTS_VALU_FLOPS_FP64
 # (profiles/sweep_fp64_valu.md): c2 one run, c4 the 33-run batched launch
-MEASURED_VALU = {("c2", 1): 2.33e10, ("c2", 4): 2.481e10, ("c4", 1): 1.33e11}
+MEASURED_VALU = {("c2", 1): 2.33e10, ("c2", 4): 2.481e10, ("c2", 6): 2.497e10, ("c4", 1): 1.33e11}
 
 
 def physical_cores():
@@ -367,8 +367,8 @@ def main():
                     help="time steps in flight (-1: min(steps, 32); 0: one step at a time, no pipeline)")
     ap.add_argument("--chunks", type=int, default=96, help="slice ranges per step of the pipelined run")
     ap.add_argument("--groups", type=int, default=-1,
-                    help="workgroups per run of the sweep (0: the run's own choice; -1: 4 for a pipelined c2 run, "
-                         "whose 20 steps in flight then hold 80 of the 256 compute units, else the run's own)")
+                    help="workgroups per run of the sweep (0: the run's own choice; -1: 6 for a pipelined c2 run, "
+                         "whose 20 steps in flight then hold 120 of the 256 compute units, else the run's own)")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
@@ -404,7 +404,7 @@ def main():
     total = warmup + args.steps + 1
     t_setup = time.perf_counter()
     if args.groups < 0:
-        args.groups = 4 if (pipelined and args.config == "c2") else 0
+        args.groups = 6 if (pipelined and args.config == "c2") else 0
     over = {"sweep.groups": args.groups} if args.groups > 0 else {}
     sim = quarz_amd.Simulation(deck, {"time.steps": total, **over})
     print(f"# {args.config} set up in {time.perf_counter() - t_setup:.1f} s ({sim.groups} workgroups per run)",

@@ -438,7 +438,7 @@ __global__ void scan_finish(const int* __restrict__ in, int n, const int* __rest
 
 // ---- the same exclusive scan in one block when n <= 1024 * kScanOne (a deposit restricted to a few rows): each
 // thread sums a contiguous chunk, the block scans the chunk sums, each thread writes its chunk's prefixes
-constexpr int kScanOne = 128;   // up to 131072 ints: a radix histogram of 1.5e6 particles stays one launch
+constexpr int kScanOne = 64;
 __global__ void __launch_bounds__(1024) scan_one_block(const int* __restrict__ in, int n, int* __restrict__ out) {
     __shared__ int sh[1024];
     const int t = threadIdx.x;
