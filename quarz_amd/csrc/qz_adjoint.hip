@@ -242,6 +242,92 @@ __device__ void pcr_transposed(const double* __restrict__ a, const double* __res
     __syncthreads();
 }
 
+// pcr_transposed for an operator that is the same every slice (no chi): the reduction of the matrix is done
+// once, keeping each level's two elimination factors and the last level's diagonal (plan: kPcrLevels x 2 rows of
+// n factors, then n diagonals), so that a slice's solves only apply them to their right-hand sides -- the same
+// operations in the same order as pcr_transposed, so the same results.
+constexpr int kPcrLevels = 11;   // ceil(log2(n)) for n <= 2048 rows
+__device__ void pcr_plan(const double* __restrict__ a, const double* __restrict__ b, const double* __restrict__ c,
+                         double* __restrict__ plan, double* const (&buf)[8], int N, int tid, int nth) {
+    const int n = N + 1;
+    double *A0 = buf[0], *B0 = buf[1], *C0 = buf[2];
+    double *A1 = buf[4], *B1 = buf[5], *C1 = buf[6];
+    for (int i = tid; i < n; i += nth) {
+        A0[i] = i > 0 ? c[i - 1] : 0.0;
+        B0[i] = b[i];
+        C0[i] = i < N ? a[i + 1] : 0.0;
+    }
+    __syncthreads();   // a Dirichlet row's diagonal is b itself, as every row's without chi
+    int lev = 0;
+    for (int st = 1; st < n; st <<= 1, ++lev) {
+        double* k1v = plan + static_cast<size_t>(2 * lev) * n;
+        double* k2v = k1v + n;
+        for (int i = tid; i < n; i += nth) {
+            const int im = i - st, ip = i + st;
+            double an = 0.0, bn = B0[i], cn = 0.0, k1 = 0.0, k2 = 0.0;
+            if (im >= 0) {
+                k1 = A0[i] / B0[im];
+                an = -A0[im] * k1;
+                bn -= C0[im] * k1;
+            }
+            if (ip < n) {
+                k2 = C0[i] / B0[ip];
+                cn = -C0[ip] * k2;
+                bn -= A0[ip] * k2;
+            }
+            k1v[i] = k1;
+            k2v[i] = k2;
+            A1[i] = an;
+            B1[i] = bn;
+            C1[i] = cn;
+        }
+        __syncthreads();
+        double* t;
+        t = A0; A0 = A1; A1 = t;
+        t = B0; B0 = B1; B1 = t;
+        t = C0; C0 = C1; C1 = t;
+    }
+    double* bl = plan + static_cast<size_t>(2 * kPcrLevels) * n;
+    for (int i = tid; i < n; i += nth) bl[i] = B0[i];
+    __syncthreads();
+}
+
+// three right-hand sides through one plan: mu_r = T^-T rhs_r; buf holds two n-row buffers per side
+__device__ void pcr_apply3(const double* __restrict__ plan, const double* const (&rhs)[3], double* const (&mu)[3],
+                           double* const (&buf)[8], int N, int tid, int nth) {
+    const int n = N + 1;
+    double* D0[3] = {buf[0], buf[1], buf[2]};
+    double* D1[3] = {buf[3], buf[4], buf[5]};
+    for (int i = tid; i < n; i += nth)
+        for (int r = 0; r < 3; ++r) D0[r][i] = rhs[r][i];
+    __syncthreads();
+    int lev = 0;
+    for (int st = 1; st < n; st <<= 1, ++lev) {
+        const double* k1v = plan + static_cast<size_t>(2 * lev) * n;
+        const double* k2v = k1v + n;
+        for (int i = tid; i < n; i += nth) {
+            const int im = i - st, ip = i + st;
+            const double k1 = k1v[i], k2 = k2v[i];
+            for (int r = 0; r < 3; ++r) {
+                double dn = D0[r][i];
+                if (im >= 0) dn -= D0[r][im] * k1;
+                if (ip < n) dn -= D0[r][ip] * k2;
+                D1[r][i] = dn;
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < 3; ++r) {
+            double* t = D0[r];
+            D0[r] = D1[r];
+            D1[r] = t;
+        }
+    }
+    const double* bl = plan + static_cast<size_t>(2 * kPcrLevels) * n;
+    for (int i = tid; i < n; i += nth)
+        for (int r = 0; r < 3; ++r) mu[r][i] = D0[r][i] / bl[i];
+    __syncthreads();
+}
+
 // out_j of D^T l for the radial gradient stencils (FieldSolver::gradient / gradient_odd, src/FieldSolver.cpp:263-281)
 __device__ __forceinline__ double gradient_T(const double* gm, const double* g0, const double* gp, const double* ir,
                                              const double* l, int j, int N, bool odd) {
@@ -297,6 +383,9 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
     }
     double* const pbuf[8] = {SA(P_A0), SA(P_B0), SA(P_C0), SA(P_D0), SA(P_A1), SA(P_B1), SA(P_C1), SA(P_D1)};
     __syncthreads();
+    // L0 is the same every slice: its reduction once per launch
+    double* l0plan = part + 5 * static_cast<size_t>(NP);
+    pcr_plan(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], l0plan, pbuf, N, tid, nth);
     for (int k = J.k_hi - 1; k >= J.k_lo; --k) {
         const double* st = J.traj + static_cast<size_t>(k - J.k_lo) * 6 * NP;
         // ---- the slice's fields; the objective's field adjoints through store_slice (src/Simulation.cpp:594-633)
@@ -424,12 +513,11 @@ __global__ void __launch_bounds__(kAdjThreads) sweep_adjoint_kernel(qz_sweep_arg
         //      ez = L0^-1 (-drho); wr = -grad psi, psi = L0^-1 (-rhot)
         for (int j = tid; j < M; j += nth) SA(T_RHS)[j] = -gradient_T(gm, g0, gp, ir, LA(A_WR), j, N, false);
         __syncthreads();
-        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_BZ), SA(T_MU0), pbuf,
-                       N, tid, nth);
-        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, LA(A_EZ), SA(T_MU1), pbuf,
-                       N, tid, nth);
-        pcr_transposed(A.op_a[OP_L0], A.op_b[OP_L0], A.op_c[OP_L0], nullptr, true, false, SA(T_RHS), SA(T_MU2), pbuf,
-                       N, tid, nth);
+        {
+            const double* const rhs[3] = {LA(A_BZ), LA(A_EZ), SA(T_RHS)};
+            double* const mu[3] = {SA(T_MU0), SA(T_MU1), SA(T_MU2)};
+            pcr_apply3(l0plan, rhs, mu, pbuf, N, tid, nth);
+        }
         phase(5);
         if (tid == 0) SA(T_MU0)[N] = 0.0;   // L0's row N is a Dirichlet row
         __syncthreads();
@@ -684,7 +772,9 @@ size_t adjoint_lds_bytes(int M) {
 using namespace qz;
 
 extern "C" size_t qz_sweep_adjoint_workspace_doubles(int M, int np) {
-    return static_cast<size_t>(S_NARR) * (M > 0 ? M : 0) + 5 * static_cast<size_t>(np > 0 ? np : 0);
+    const size_t m = M > 0 ? static_cast<size_t>(M) : 0;
+    return static_cast<size_t>(S_NARR) * m + 5 * static_cast<size_t>(np > 0 ? np : 0) +
+           static_cast<size_t>(2 * kPcrLevels + 1) * m;   // the solve arrays, the particles' adjoints, L0's plan
 }
 
 extern "C" int qz_sweep_adjoint(const qz_sweep_args* a, const qz_adjoint_args* j, void* stream) {
