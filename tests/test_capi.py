@@ -148,3 +148,22 @@ def test_message_entry_points_reject_bad_arguments():
     assert L.qz_pack_message(args, 1, 0, 0.0, 0, 0, dummy, None) == -1 and "ws" in quarz_amd.last_error()
     assert L.qz_unpack_message(args, 1, dummy, None, None) == -1 and "ws" in quarz_amd.last_error()
     assert 0 < C.sizeof(quarz_amd.SweepArgsHeadC) < L.qz_sweep_args_bytes() <= 4096
+
+
+def test_batch_entry_points_reject_bad_arguments():
+    """The batch's autotuner, worker cap, step and pipeline schedule answer null handles and bad counts with
+    QZ_EINVAL and a message, before any device work."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    cand = (C.c_int * 3)(1, 2, 4)
+    groups = (C.c_int * 4)()
+    est = (C.c_double * 2)()
+    assert L.qz_batch_balance(None, cand, 3, 64, groups, est) == -1
+    assert "bad arguments" in quarz_amd.last_error()
+    assert L.qz_batch_set_workers(None, 4) == -1 and "null batch" in quarz_amd.last_error()
+    assert L.qz_batch_step(None, None) == -1 and "null batch" in quarz_amd.last_error()
+    assert L.qz_sim_step_batched(None, 0, None) == -1 and "no runs" in quarz_amd.last_error()
+    assert L.qz_batch_create(None, 0, 1, None) == -1 and "no runs" in quarz_amd.last_error()
+    nc = C.c_int(0)
+    assert L.qz_sim_pipeline_timeline(None, None, 0, C.byref(nc)) == -1 and "null" in quarz_amd.last_error()
