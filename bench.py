@@ -7,8 +7,8 @@ over every slice (one persistent kernel), the beam push.  W untimed steps one at
 first starts the leapfrog), then exactly K timed steps between device synchronisations.  The timed steps run
 pipelined (Simulation.run_pipelined: up to --depth steps in flight, each split into --chunks slice ranges,
 bit-identical to stepping one at a time); --depth 0 steps them one at a time.  --groups sets the workgroups of
-each step's sweep (the c2 default when pipelined: 4, so the 20 steps in flight hold 80 compute units; the
-eager steps then run the same kernel cooperatively).  Prints one JSON line:
+each step's sweep (the defaults when pipelined: c2 6, so the 20 steps in flight hold 120 compute units; c3 16,
+10 steps x 16; the eager steps run the same kernel).  Prints one JSON line:
   value          plasma particle-slices per second of the whole timed steps (wall clock)
   plain_value    the same run stepped one step at a time (--plain-steps steps), for comparison
   beam_pushes_per_s   live beam particles pushed per second of the whole timed steps
@@ -22,8 +22,8 @@ eager steps then run the same kernel cooperatively).  Prints one JSON line:
                  events around the deposit (bsrc reset, live count, deposit) and the push launches
   cpu_baseline   the oracle's OpenMP build of the same step (the reference's loop structure) timed on this
                  host's cores on one full sweep of the same deck
-The inputs (beam particles, grid, operators) are resident in HBM before the timed region; the plasma is
-reloaded at the head of the box on the host every step, as in the reference.
+The inputs (beam particles, grid, operators) are resident in HBM before the timed region; a cold plasma is
+formed at the head of the box by the sweep kernel every step.
 """
 import argparse
 import json
@@ -368,15 +368,17 @@ def main():
     ap.add_argument("--chunks", type=int, default=96, help="slice ranges per step of the pipelined run")
     ap.add_argument("--groups", type=int, default=-1,
                     help="workgroups per run of the sweep (0: the run's own choice; -1: 6 for a pipelined c2 run, "
-                         "whose 20 steps in flight then hold 120 of the 256 compute units, else the run's own)")
+                         "whose 20 steps in flight then hold 120 of the 256 compute units, 16 for a pipelined c3 "
+                         "run (10 x 16), else the run's own)")
     ap.add_argument("--tune-cache", default=None,
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"c3": 10, "c4": 50, "c5": 3}.get(args.config, 20)
     if args.config == "c3":
-        args.depth = 0         # runs of several workgroups step one step at a time
         args.warmup = min(args.warmup, 1)
+        if args.groups < 0 and args.depth != 0:   # the measured best shape of the pipelined long box (96 ranges)
+            args.groups = 16   # 10 steps x 16 workgroups in flight: 160 of the 256 compute units
     # The pipelined run wants one hardware queue per step in flight; HIP's default is 4 per process, read when
     # HIP starts (so set before torch is imported), never above 32.  The batched launches (c4, c5) keep the
     # default: with 32 queues their long persistent launch ran about 1.5x slower (c4: 412 against 276 ms).

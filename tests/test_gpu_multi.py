@@ -204,3 +204,29 @@ def test_mixed_batch_two_launches_equals_plain(qa, decks):
     runs, _, replays = _batch_vs_plain(qa, f"{decks}/gain.in", forced, ovs, 6, workers=1)
     assert [r.groups for r in runs] == [2, 1, 1]
     assert replays == 4
+
+
+def test_c3_shape_pipelined_equals_plain(qa, tmp_path):
+    """The long-box workload's plasma and grid (1024 stretched cells, electrons and mobile ions, 32768 particles
+    per slice, 8 workgroups) in a short box: four steps with three in flight over six ranges each equal the same
+    steps one at a time, bit for bit (fields, diagnostics, beam sources, every beam particle)."""
+    from quarz_amd import configs
+    deck = configs.c3_deck(workdir=str(tmp_path), xi_min=-1.5, steps=5, nd=100000, nw=20000, driver_xi0=-0.6,
+                           witness_xi0=-1.2)
+    ov = {"driver.sigma_xi": 0.2, "driver.x0": 0.05, "driver.n0": 2, "witness.sigma_xi": 0.05}
+    a = qa.Simulation(deck, ov)
+    b = qa.Simulation(deck, ov)
+    assert a.groups == 8 and b.groups == 8
+    a.step()
+    b.step()
+    for _ in range(4):
+        a.step()
+    b.run_pipelined(4, chunks=6, depth=3)
+    for w in ("fld", "diag", "bsrc"):
+        ga, gb = getattr(b, w), getattr(a, w)
+        assert _same(ga, gb), _report(f"pipelined {w}", ga, gb)
+    for bi in range(a.nbeams):
+        for k in qa.BEAM_ARRAYS:
+            ga, gb = b.get(f"beam{bi}.{k}"), a.get(f"beam{bi}.{k}")
+            assert _same(ga, gb), _report(f"pipelined beam{bi}.{k}", ga, gb)
+    assert b.lost() == a.lost()
