@@ -328,40 +328,50 @@ __global__ void beam_node_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, cons
     const int kk = static_cast<int>(node / M), jj = static_cast<int>(node % M);
     const bool ngp = bg.ngp != 0;
     const Real idxi = Real(1) / bg.dxi;
-    // buckets (a = slice offset, b = node offset of the particle's corner): cell (kk - a, jj - b)
-    int lo[4], hi[4], ca[4], cb[4];
-    int nb = 0;
-    for (int a = 0; a < (ngp ? 1 : 2); ++a)
-        for (int b = 0; b < 2; ++b) {
-            const int kc = kk - a, jc = jj - b;
-            if (jc < 0 || jc >= N) continue;
-            const int key = (kc + 1) * N + jc;   // kc >= -1
-            lo[nb] = start[key]; hi[nb] = start[key + 1]; ca[nb] = a; cb[nb] = b;
-            if (lo[nb] < hi[nb]) ++nb;
-        }
+    // buckets q = 2 a + b (a = slice offset, b = node offset of the particle's corner): cell (kk - a, jj - b).  A
+    // particle sits in one of them at most, so the merge by index has no ties.  The head of each bucket is kept
+    // in a register, and the advanced bucket's next head is loaded while the chosen particle's record is read.
+    constexpr int kEnd = 0x7fffffff;
+    int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0, lo2 = 0, hi2 = 0, lo3 = 0, hi3 = 0;
+    auto bucket = [&](int a, int b, int& lo, int& hi) {
+        const int kc = kk - a, jc = jj - b;
+        if ((a == 1 && ngp) || jc < 0 || jc >= N) return;
+        const int key = (kc + 1) * N + jc;   // kc >= -1
+        lo = start[key];
+        hi = start[key + 1];
+    };
+    bucket(0, 0, lo0, hi0);
+    bucket(0, 1, lo1, hi1);
+    bucket(1, 0, lo2, hi2);
+    bucket(1, 1, lo3, hi3);
     Real* S = src + static_cast<size_t>(node) * nc;
-    if (nb == 0) {   // no particle: the node keeps what src holds, or is zero when the deposit overwrites
-        if (overwrite)
+    if (lo0 >= hi0 && lo1 >= hi1 && lo2 >= hi2 && lo3 >= hi3) {   // no particle: the node keeps what src holds,
+        if (overwrite)                                               // or is zero when the deposit overwrites
             for (int c = 0; c < nc; ++c) S[c] = Real(0);
         return;
     }
+    int h0 = lo0 < hi0 ? list[lo0] : kEnd, h1 = lo1 < hi1 ? list[lo1] : kEnd;
+    int h2 = lo2 < hi2 ? list[lo2] : kEnd, h3 = lo3 < hi3 ? list[lo3] : kEnd;
     Real acc[15];   // from zero when overwriting: +0 + x is x, as the zero-filled source plus x
     for (int c = 0; c < nc; ++c) acc[c] = overwrite ? Real(0) : S[c];
     const Real iv = g.inv_V[jj];
     while (true) {
         // next particle in index order among the buckets
-        int best = -1, bi = 0x7fffffff;
-        for (int q = 0; q < nb; ++q)
-            if (lo[q] < hi[q]) {
-                const int v = list[lo[q]];
-                if (v < bi) { bi = v; best = q; }
-            }
-        if (best < 0) break;
-        ++lo[best];
+        int best = 0, bi = h0;
+        if (h1 < bi) { bi = h1; best = 1; }
+        if (h2 < bi) { bi = h2; best = 2; }
+        if (h3 < bi) { bi = h3; best = 3; }
+        if (bi == kEnd) break;
+        switch (best) {
+            case 0: ++lo0; h0 = lo0 < hi0 ? list[lo0] : kEnd; break;
+            case 1: ++lo1; h1 = lo1 < hi1 ? list[lo1] : kEnd; break;
+            case 2: ++lo2; h2 = lo2 < hi2 ? list[lo2] : kEnd; break;
+            default: ++lo3; h3 = lo3 < hi3 ? list[lo3] : kEnd; break;
+        }
         const Real* rr = rec + static_cast<size_t>(bi) * R_N;
         const Real tk = rr[R_TK], t = rr[R_T];
-        const Real Wk = ca[best] == 0 ? Real(1) - tk : tk;
-        const Real Wr = cb[best] == 0 ? Real(1) - t : t;
+        const Real Wk = (best >> 1) == 0 ? Real(1) - tk : tk;
+        const Real Wr = (best & 1) == 0 ? Real(1) - t : t;
         const Real d = rr[R_WQ] * Wk * Wr * iv * idxi;
         const Real omvz = rr[R_OMVZ], vz = rr[R_VZ];
         acc[B_RT0] += d * omvz;
