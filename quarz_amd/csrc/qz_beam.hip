@@ -19,6 +19,14 @@ namespace qz {
 // per-particle record of the deposit (the node-independent factors of src/Beam.cpp:738-757)
 enum BRec : int { R_WQ = 0, R_TK, R_T, R_OMVZ, R_VZ, R_VR, R_VT, R_C, R_S, R_VX, R_VY, R_N };
 
+// count: every active lane adds `add` (0 or 1) to *ctr, with one atomic per wave instead of one per lane
+__device__ __forceinline__ void wave_count(unsigned long long* __restrict__ ctr, bool add) {
+    const unsigned long long m = __ballot(add ? 1 : 0);
+    const unsigned long long act = __ballot(1);
+    if (static_cast<int>(__lane_id()) == __ffsll(static_cast<long long>(act)) - 1 && m)
+        atomicAdd(ctr, static_cast<unsigned long long>(__popcll(m)));
+}
+
 // With pstep, only the particles whose gather slice k (after clamping) lies in [s_lo, s_hi) and that are not
 // yet marked with `step` are pushed, and they are marked: a step pushed slice range by slice range pushes
 // every particle once, also those that slip into a later range.
@@ -57,7 +65,7 @@ __global__ void beam_advance_kernel(qz_radial_grid g, qz_beam_grid bg, int m1, R
         if (k < s_lo || k >= s_hi || pstep[i] >= step) return;
         pstep[i] = step;
         // the live particles this step pushes, each once: Simulation::step's count before the push
-        if (nmarked) atomicAdd(nmarked, 1ull);
+        if (nmarked) wave_count(nmarked, true);
     }
     const int k1 = ngp ? k : k + 1;
     int j; Real t;
@@ -150,8 +158,8 @@ __global__ void beam_key_kernel(qz_radial_grid g, qz_beam_grid bg, Real q, int n
     const int k = ngp ? static_cast<int>(floor(sx + Real(0.5))) : static_cast<int>(floor(sx));
     if (live) {   // the window's share of the live count: particles before (after) the box go to the first (last)
         const bool first = k0 == 0, last = k0 + nloc >= bg.nxi;
-        if ((k >= k0 || first) && (k < k0 + nloc || last) && !(pstep && pstep[i] >= marker) && P.w[i] != Real(0))
-            atomicAdd(live, 1ull);
+        wave_count(live, (k >= k0 || first) && (k < k0 + nloc || last) && !(pstep && pstep[i] >= marker) &&
+                             P.w[i] != Real(0));
     }
     if (k < k0 - 1 || k >= k0 + nloc) return;
     if (pstep && pstep[i] >= marker) return;
