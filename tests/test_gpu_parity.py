@@ -354,7 +354,7 @@ def test_c1_workload_bit_exact(qa, oracle):
     for k in da:
         assert da[k] == db[k], (k, da[k], db[k])
     # the reference's index-order moments after the 10 steps, bit for bit; the device order differs from them
-    # only in the last bits (the north star's moment gate is 1e-9)
+    # only in the last bits (a 1e-9 relative gate on beam moments after 10 steps is the parity bar)
     sg, so = g.beam_sums(0)[0], o.beam_sums(0)[0]
     assert np.array_equal(sg, so)
     di = o.beam_diag(0)
