@@ -374,7 +374,7 @@ def main():
                     help="JSON file of launch shapes: reused for the same device and shapes instead of timing")
     args = ap.parse_args()
     if args.steps is None:
-        args.steps = {"c3": 10, "c4": 50, "c5": 3}.get(args.config, 20)
+        args.steps = {"c1": 10, "c3": 10, "c4": 50, "c5": 3}.get(args.config, 20)
     if args.config == "c3":
         args.warmup = min(args.warmup, 1)
         if args.groups < 0 and args.depth != 0:   # the measured best shape of the pipelined long box (96 ranges)
