@@ -39,12 +39,13 @@ FP64_PEAK = 78.6e12
 HBM_PEAK = 8.0e12
 FLOPS = {0: (361.0, 84.0), 1: (679.0, 508.0)}   # per particle-slice, per node-slice
 # HBM bytes of one sweep launch (read + written), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over the
-# same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md)
+# same command, scaled by the counters' reading of a 1 GiB copy (profiles/sweep_c2_hbm_traffic.md,
+# profiles/sweep_c3_counters.md)
 # Keyed by (workload, workgroups per run); per whole sweep (every slice of one step).
-MEASURED_TRAFFIC = {("c2", 1): 841.9e6, ("c2", 4): 20163.2e6, ("c2", 6): 21787.9e6}
+MEASURED_TRAFFIC = {("c2", 1): 841.9e6, ("c2", 4): 20163.2e6, ("c2", 6): 21787.9e6, ("c3", 16): 699.5e9}
 # FP64 VALU lane operations per sweep (every slice of one step), 64 x SQ_INSTS_VALU_FLOPS_FP64
 # (profiles/sweep_fp64_valu.md): c2 one run, c4 the 33-run batched launch
-MEASURED_VALU = {("c2", 1): 2.33e10, ("c2", 4): 2.481e10, ("c2", 6): 2.497e10, ("c4", 1): 1.33e11}
+MEASURED_VALU = {("c2", 1): 2.33e10, ("c2", 4): 2.481e10, ("c2", 6): 2.497e10, ("c3", 16): 5.066e11, ("c4", 1): 1.33e11}
 
 
 def physical_cores():
