@@ -375,6 +375,11 @@ int qz_batch_set_workers(qz_batch* b, int max_workers);
  * groups_out[n] receives each run's count; est_ms[2] the plan's estimate and the estimate with one workgroup
  * per run (-1 when some run cannot take one).  Results do not depend on the counts. */
 int qz_batch_balance(qz_batch* b, const int* candidates, int ncand, int nslices, int* groups_out, double* est_ms);
+/* The autotuner's planning step alone, host only: est[i * ncand + c] is run i's estimated sweep (ms) with
+ * candidates[c] workgroups (< 0: unusable), ncu the compute units.  Writes groups_out[n] and est_ms[2] as
+ * qz_batch_balance does; an error (-1, with a message) when no plan fits the compute units. */
+int qz_batch_plan(int n, const int* candidates, int ncand, const double* est, int ncu, int* groups_out,
+                  double* est_ms);
 
 /* ---- launch shapes of the beam kernels (not in the reference) ----
  * Threads per block (a multiple of 64 in [64, 1024]) of: the beam push (Beam::advance), the deposit's

@@ -101,6 +101,7 @@ def lib():
         L.qz_batch_step.argtypes = [C.c_void_p, DP]
         L.qz_batch_balance.argtypes = [C.c_void_p, IP, C.c_int, C.c_int, IP, DP]
         L.qz_batch_set_workers.argtypes = [C.c_void_p, C.c_int]
+        L.qz_batch_plan.argtypes = [C.c_int, IP, C.c_int, DP, C.c_int, IP, DP]
         L.qz_batch_replays.argtypes = [C.c_void_p]
         L.qz_batch_replays.restype = C.c_long
         L.qz_batch_destroy.argtypes = [C.c_void_p]
@@ -471,6 +472,19 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_plan(est_ms, candidates, ncu=256):
+    """The batch autotuner's planning step on given estimates (host only): est_ms[i][c] is run i's sweep time with
+    candidates[c] workgroups (< 0: unusable).  Returns {"groups", "est_ms", "est_one_ms"} as Batch.balance."""
+    est = np.ascontiguousarray(est_ms, dtype=np.float64)
+    n, ncand = est.shape
+    assert ncand == len(candidates)
+    cand = (C.c_int * ncand)(*candidates)
+    g = (C.c_int * n)()
+    out = np.zeros(2)
+    check(lib().qz_batch_plan(n, cand, ncand, est.ctypes.data_as(DP), int(ncu), g, out.ctypes.data_as(DP)))
+    return {"groups": list(g), "est_ms": float(out[0]), "est_one_ms": float(out[1])}
 
 
 # ---------------------------------------------------------------------------- kernel-level entry points

@@ -2214,6 +2214,77 @@ double list_makespan(const std::vector<double>& jobs_desc, int w) {
 }
 }  // namespace
 
+namespace {
+// The batch autotuner's plan (qz_batch_balance, qz_batch_plan): est[i * ncand + c] is run i's estimated sweep
+// with cand[c] workgroups (< 0: unusable).  A plan for a target T gives every run whose one-workgroup estimate
+// exceeds T the fewest workgroups that bring it under T and list-schedules the other runs, longest first, on the
+// compute units left; its cost is the larger of T and that makespan.  Every estimate is tried as T; the cheapest
+// plan goes to G, its cost to cost[0] and the cost of one workgroup per run to cost[1] (-1: some run cannot).
+void plan_groups(int n, const int* cand, int ncand, const double* est, int ncu, int* G, double* cost) {
+    int c1 = -1;
+    for (int c = 0; c < ncand; ++c)
+        if (cand[c] == 1) c1 = c;
+    auto E = [&](int i, int c) { return est[static_cast<size_t>(i) * ncand + c]; };
+    auto one = [&](int i) { return c1 >= 0 ? E(i, c1) : -1.0; };
+    auto plan_cost = [&](double T, std::vector<int>& g) -> double {
+        int used = 0;
+        double top = 0;
+        std::vector<double> singles;
+        for (int i = 0; i < n; ++i) {
+            if (one(i) >= 0 && one(i) <= T) {
+                g[i] = 1;
+                singles.push_back(one(i));
+                continue;
+            }
+            int pick = -1;
+            for (int c = 0; c < ncand; ++c)
+                if (cand[c] > 1 && E(i, c) >= 0 && E(i, c) <= T && (pick < 0 || cand[c] < cand[pick])) pick = c;
+            if (pick < 0) return 1e300;
+            g[i] = cand[pick];
+            used += cand[pick];
+            top = std::max(top, E(i, pick));
+        }
+        const int free_cus = ncu - used;   // the one launch of qz::plan_mixed
+        if (used > ncu || (!singles.empty() && free_cus < 1)) return 1e300;
+        std::sort(singles.begin(), singles.end(), std::greater<double>());
+        const int w = std::min(static_cast<int>(singles.size()), free_cus);
+        return std::max(top, list_makespan(singles, w));
+    };
+    std::vector<double> targets;
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < ncand; ++c)
+            if (E(i, c) >= 0) targets.push_back(E(i, c));
+    std::sort(targets.begin(), targets.end());
+    targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
+    double best = 1e300;
+    std::vector<int> g(n), bestg(n);
+    for (double T : targets) {
+        const double c = plan_cost(T, g);
+        if (c < best * (1 - 1e-9)) {
+            best = c;
+            bestg = g;
+        }
+    }
+    if (best >= 1e299) throw std::runtime_error("batch plan: no plan fits the compute units");
+    double single_cost = -1;
+    {
+        std::vector<double> all;
+        bool ok = true;
+        for (int i = 0; i < n; ++i) {
+            ok = ok && one(i) >= 0;
+            all.push_back(one(i));
+        }
+        if (ok) {
+            std::sort(all.begin(), all.end(), std::greater<double>());
+            single_cost = list_makespan(all, std::min(n, ncu));
+        }
+    }
+    for (int i = 0; i < n; ++i) G[i] = bestg[i];
+    cost[0] = best;
+    cost[1] = single_cost;
+}
+}  // namespace
+
 // Autotuner of the sweep's workgroups per run (quarz_amd.h): every candidate count sweeps the first nslices
 // slices of the coming step; the fastest is kept.
 extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nslices, double* ms_out) {
@@ -2231,11 +2302,8 @@ extern "C" int qz_sim_tune_groups(qz_sim* h, const int* cand, int ncand, int nsl
 
 // Autotuner of a batch's workgroups per run (quarz_amd.h).  Runs of one shape (modes, cells, plasma particles)
 // share a time per slice; the shape's run with the most slices is probed over its first nslices slices with
-// every candidate count that keeps the smallest particle share (best of two launches).  A run's sweep is then estimated at (time per slice) x (its
-// slices).  A plan gives every run whose one-workgroup estimate exceeds a target T the fewest workgroups that
-// bring it under T, and the other runs to the remaining compute units, list-scheduled longest first; its cost
-// is the larger of T and that schedule's makespan.  Every estimate of every run and count is tried as T, and
-// the cheapest plan is applied.  est_ms[0] = its cost, est_ms[1] = the cost with one workgroup per run.
+// every candidate count that keeps the smallest particle share (best of two launches).  A run's sweep is then
+// estimated at (time per slice) x (its slices), and plan_groups picks the counts, which the runs then take.
 extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nslices, int* groups_out,
                                 double* est_ms) {
     if (!b || !cand || ncand < 1 || nslices < 1 || !groups_out || !est_ms)
@@ -2248,9 +2316,8 @@ extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nsl
             if (s.swept) throw std::runtime_error("qz_batch_balance: balance before the first step");
             shapes[{s.m1 ? 1 : 0, s.grid.N, qz::mobile_particles(s)}].push_back(i);
         }
-        // est[i][c]: run i's sweep with cand[c] workgroups (ms), < 0 where unusable
-        std::vector<std::vector<double>> est(n, std::vector<double>(ncand, -1));
-        std::vector<double> one(n, -1), per(ncand);
+        // est[i * ncand + c]: run i's sweep with cand[c] workgroups (ms), < 0 where unusable
+        std::vector<double> est(static_cast<size_t>(n) * ncand, -1.0), per(ncand);
         int c1 = -1;
         for (int c = 0; c < ncand; ++c)
             if (cand[c] == 1) c1 = c;
@@ -2277,68 +2344,29 @@ extern "C" int qz_batch_balance(qz_batch* b, const int* cand, int ncand, int nsl
             const double sampled = std::min(nslices, r.K);
             for (int i : kv.second) {
                 qz::Simulation& s = *b->sims[i]->s;
-                for (int c = 0; c < ncand; ++c) est[i][c] = per[c] < 0 ? -1 : per[c] / sampled * s.K;
-                s.slice_ms = c1 >= 0 && per[c1] > 0 ? per[c1] / sampled : 0;
-                one[i] = c1 >= 0 ? est[i][c1] : -1;
-            }
-        }
-        // a plan for target T: groups per run (0: none fits), and its cost
-        std::vector<int> G(n);
-        auto plan_cost = [&](double T, std::vector<int>& g) -> double {
-            int used = 0;
-            double top = 0;
-            std::vector<double> singles;
-            for (int i = 0; i < n; ++i) {
-                if (one[i] >= 0 && one[i] <= T) {
-                    g[i] = 1;
-                    singles.push_back(one[i]);
-                    continue;
-                }
-                int pick = -1;
                 for (int c = 0; c < ncand; ++c)
-                    if (cand[c] > 1 && est[i][c] >= 0 && est[i][c] <= T && (pick < 0 || cand[c] < cand[pick])) pick = c;
-                if (pick < 0) return 1e300;
-                g[i] = cand[pick];
-                used += cand[pick];
-                top = std::max(top, est[i][pick]);
-            }
-            const int free_cus = ncu - used;   // the one launch of qz::plan_mixed
-            if (used > ncu || (!singles.empty() && free_cus < 1)) return 1e300;
-            std::sort(singles.begin(), singles.end(), std::greater<double>());
-            const int w = std::min(static_cast<int>(singles.size()), free_cus);
-            return std::max(top, list_makespan(singles, w));
-        };
-        std::vector<double> targets;
-        for (int i = 0; i < n; ++i)
-            for (int c = 0; c < ncand; ++c)
-                if (est[i][c] >= 0) targets.push_back(est[i][c]);
-        std::sort(targets.begin(), targets.end());
-        targets.erase(std::unique(targets.begin(), targets.end()), targets.end());
-        double best = 1e300;
-        std::vector<int> g(n);
-        for (double T : targets) {
-            const double c = plan_cost(T, g);
-            if (c < best * (1 - 1e-9)) { best = c; G = g; }
-        }
-        if (best >= 1e299) throw std::runtime_error("qz_batch_balance: no plan fits the compute units");
-        double single_cost = 1e300;
-        {
-            std::vector<double> all;
-            bool ok = true;
-            for (int i = 0; i < n; ++i) { ok = ok && one[i] >= 0; all.push_back(one[i]); }
-            if (ok) {
-                std::sort(all.begin(), all.end(), std::greater<double>());
-                single_cost = list_makespan(all, std::min(n, ncu));
+                    est[static_cast<size_t>(i) * ncand + c] = per[c] < 0 ? -1 : per[c] / sampled * s.K;
+                s.slice_ms = c1 >= 0 && per[c1] > 0 ? per[c1] / sampled : 0;
             }
         }
+        std::vector<int> G(n);
+        double cost[2];
+        plan_groups(n, cand, ncand, est.data(), ncu, G.data(), cost);
         b->drop_graph();
         for (int i = 0; i < n; ++i) {
             set_groups(*b->sims[i]->s, G[i]);
             groups_out[i] = G[i];
         }
-        est_ms[0] = best;
-        est_ms[1] = single_cost < 1e299 ? single_cost : -1;
+        est_ms[0] = cost[0];
+        est_ms[1] = cost[1];
     })
+}
+
+extern "C" int qz_batch_plan(int n, const int* cand, int ncand, const double* est, int ncu, int* groups_out,
+                             double* est_ms) {
+    if (n < 1 || !cand || ncand < 1 || !est || ncu < 1 || !groups_out || !est_ms)
+        return qz_fail(QZ_EINVAL, "qz_batch_plan: bad arguments");
+    QZ_SIM_TRY(plan_groups(n, cand, ncand, est, ncu, groups_out, est_ms))
 }
 
 // Autotuner step for one beam kernel (quarz_amd.h): candidates timed on scratch copies of the largest beam

@@ -167,3 +167,30 @@ def test_batch_entry_points_reject_bad_arguments():
     assert L.qz_batch_create(None, 0, 1, None) == -1 and "no runs" in quarz_amd.last_error()
     nc = C.c_int(0)
     assert L.qz_sim_pipeline_timeline(None, None, 0, C.byref(nc)) == -1 and "null" in quarz_amd.last_error()
+
+
+def test_batch_plan_on_known_estimates():
+    """The batch autotuner's planning step on hand-made estimates (host only).  Four runs on 8 compute units:
+    the longest (80 ms alone, 50 ms on 2 workgroups, 30 ms on 4) sets the one-workgroup-per-run makespan; the
+    plan gives it 4 workgroups and list-schedules the other three on the 4 units left, for 30 ms; when extra
+    workgroups do not shorten a run the plan keeps one per run; when no plan fits the units it is an error."""
+    import numpy as np
+    import quarz_amd
+    est = np.array([[80.0, 50.0, 30.0], [30.0, 25.0, 20.0], [20.0, 15.0, 12.0], [10.0, 8.0, 7.0]])
+    p = quarz_amd.batch_plan(est, (1, 2, 4), ncu=8)
+    assert p["groups"] == [4, 1, 1, 1]
+    assert p["est_ms"] == 30.0 and p["est_one_ms"] == 80.0
+    # no speed-up from more workgroups: one each
+    flat = np.array([[40.0, 45.0, 50.0]] * 3)
+    p = quarz_amd.batch_plan(flat, (1, 2, 4), ncu=8)
+    assert p["groups"] == [1, 1, 1] and p["est_ms"] == 40.0
+    # two units for three runs: list scheduling, longest first
+    p = quarz_amd.batch_plan(np.array([[30.0], [20.0], [15.0]]), (1,), ncu=2)
+    assert p["groups"] == [1, 1, 1] and p["est_ms"] == 35.0
+    # a run that cannot take one workgroup and no count that fits: an error
+    try:
+        quarz_amd.batch_plan(np.array([[-1.0, -1.0]]), (1, 2), ncu=8)
+    except quarz_amd.QuarzError as e:
+        assert "no plan" in str(e)
+    else:
+        raise AssertionError("an unplannable batch was accepted")
