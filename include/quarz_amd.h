@@ -401,6 +401,39 @@ int qz_sim_tune_groups(qz_sim* s, const int* candidates, int ncand, int nslices,
 /* "<gfx arch name>|<compute units>CU" of the current device, the key under which launch shapes are cached */
 int qz_device_tag(char* out, int n);
 
+/* ---- laser envelope (src/Laser.hpp:47-108, src/Laser.cpp) ----
+ * The complex envelope a^ of one box: K slices (slice 0 at the head of the box, internal xi = xi_min + k dxi) of
+ * M radial nodes, (K, M, 2) doubles (Re, Im) in device memory.  laser.l = 0, no ionization losses, one box on one
+ * GPU (the head of the box is the first slice: no guard rows).  scheme: 0 trapezoid, 1 BDF2 (the reference's
+ * default), 2 backward Euler in xi (laser.xi_scheme); circular: 1 for circular polarisation (<a^2> = |a^|^2). */
+typedef struct qz_laser qz_laser;
+/* Laser::Laser without the pulse (src/Laser.cpp:24-84): the grid of M nodes (nodes[0] = 0, increasing), the box,
+ * the time step dt of the beam push (dt <= 0: the envelope is never advanced, src/Laser.cpp:318) and
+ * k0 = omega0 / omega_p > 1.  The envelope starts at zero.  Grids up to 1861 nodes (QZ_ERANGE beyond). */
+int qz_laser_create(const double* nodes, int M, int K, double xi_min, double dxi, double dt, double k0, int scheme,
+                    int circular, qz_laser** out);
+void qz_laser_destroy(qz_laser* L);
+/* the Gaussian pulse of Laser::Laser (src/Laser.cpp:195-236, laser.l = 0): laser.a0, w0, L0, xi0 (internal xi)
+ * and focus (the distance to the focal plane), formed on the host and uploaded */
+int qz_laser_gaussian(qz_laser* L, double a0, double w0, double L0, double xi0, double focus);
+/* the current envelope from / to host memory: (K, M, 2) doubles */
+int qz_laser_set_envelope(qz_laser* L, const double* a_host);
+/* what = 0: the current envelope (K, M, 2) (Laser::current); 1: the ponderomotive arrays of the last step
+ * (K, M, 3): <a^2>, d<a^2>/dr, d<a^2>/dxi at the level the step started from (Laser::ponderomotive) */
+int qz_laser_get(qz_laser* L, int what, double* out_host);
+/* the device address of the same arrays (valid until the next qz_laser_step, which exchanges the envelope's
+ * two buffers) */
+int qz_laser_device(qz_laser* L, int what, double** ptr);
+/* One time step (src/Simulation.cpp:731-735 and 784, over every slice): Laser::prepare_slice of every slice, then
+ * Laser::advance_slice from the head of the box with the plasma susceptibility chi of each slice, (K, M) doubles
+ * in device memory (NULL: vacuum); the advanced envelope becomes the current one (Laser::begin_step).
+ * stream: NULL runs on the laser's own stream and waits for it; otherwise the call is asynchronous on `stream`,
+ * and the caller synchronizes `stream` before the next qz_laser_* call. */
+int qz_laser_step(qz_laser* L, const double* chi, void* stream);
+/* Laser::sums (src/Laser.cpp:421-438) of the current envelope, summed on the host in slice and node order:
+ * {sum |a|^2 dV, sum |a|^2 xi dV (the user's xi = -xi), sum |a|^2 r^2 dV, max |a|, sum |a|^2 xi^2 dV} */
+int qz_laser_sums(qz_laser* L, double* out5);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2348,3 +2348,4 @@ int or_sim_write_fields(void* h, const char* path) { OR_TRY(write_fields(*static
 int or_sim_write_axis(void* h, const char* path) { OR_TRY(write_axis(*static_cast<Sim*>(h), path)) }
 
 }  // extern "C"
+#include "laser_oracle.inc"

@@ -133,6 +133,16 @@ def lib():
         L.qz_sim_profile_size.argtypes = [C.c_void_p, C.c_int]
         L.qz_sim_adjoint_ticks.argtypes = [C.c_void_p, DP]
         L.qz_sweep_adjoint_workspace_doubles.restype = C.c_size_t
+        L.qz_laser_create.argtypes = [DP, C.c_int, C.c_int] + [C.c_double] * 4 + [C.c_int, C.c_int,
+                                                                                 C.POINTER(C.c_void_p)]
+        L.qz_laser_destroy.argtypes = [C.c_void_p]
+        L.qz_laser_destroy.restype = None
+        L.qz_laser_gaussian.argtypes = [C.c_void_p] + [C.c_double] * 5
+        L.qz_laser_set_envelope.argtypes = [C.c_void_p, DP]
+        L.qz_laser_get.argtypes = [C.c_void_p, C.c_int, DP]
+        L.qz_laser_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.qz_laser_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.qz_laser_sums.argtypes = [C.c_void_p, DP]
         L.qz_sweep_adjoint_workspace_doubles.argtypes = [C.c_int, C.c_int]
         L.qz_sweep_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.qz_beam_spread_adjoint.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, C.c_double,
@@ -485,6 +495,66 @@ def batch_plan(est_ms, candidates, ncu=256):
     out = np.zeros(2)
     check(lib().qz_batch_plan(n, cand, ncand, est.ctypes.data_as(DP), int(ncu), g, out.ctypes.data_as(DP)))
     return {"groups": list(g), "est_ms": float(out[0]), "est_one_ms": float(out[1])}
+
+
+class Laser:
+    """The laser envelope of one box (src/Laser.hpp:47-108) on the GPU: K slices of the radial grid `nodes`, slice 0
+    at the head of the box (internal xi = xi_min + k dxi).  scheme: "trapezoid", "bdf2" (the reference's default)
+    or "euler" in xi; laser.l = 0, no ionization losses.  step(chi) advances the envelope by one time step dt with
+    the plasma susceptibility chi (K, M) of every slice (None: vacuum)."""
+
+    SCHEMES = {"trapezoid": 0, "bdf2": 1, "euler": 2}
+
+    def __init__(self, nodes, K, xi_min, dxi, dt, k0, scheme="bdf2", circular=False):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        self.M, self.K = len(self.nodes), int(K)
+        self.xi_min, self.dxi = float(xi_min), float(dxi)
+        h = C.c_void_p()
+        check(lib().qz_laser_create(self.nodes.ctypes.data_as(DP), self.M, self.K, self.xi_min, self.dxi, float(dt),
+                                    float(k0), self.SCHEMES[scheme], int(bool(circular)), C.byref(h)))
+        self.h = h
+
+    def gaussian(self, a0, w0, L0, xi0, focus=0.0):
+        """The Gaussian pulse of Laser::Laser (xi0 internal)."""
+        check(lib().qz_laser_gaussian(self.h, a0, w0, L0, xi0, focus))
+
+    def set_envelope(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == (self.K, self.M, 2)
+        check(lib().qz_laser_set_envelope(self.h, a.ctypes.data_as(DP)))
+
+    def envelope(self):
+        out = np.zeros((self.K, self.M, 2))
+        check(lib().qz_laser_get(self.h, 0, out.ctypes.data_as(DP)))
+        return out
+
+    def ponderomotive(self):
+        """(K, M, 3) of the last step: <a^2>, d<a^2>/dr, d<a^2>/dxi at the level the step started from."""
+        out = np.zeros((self.K, self.M, 3))
+        check(lib().qz_laser_get(self.h, 1, out.ctypes.data_as(DP)))
+        return out
+
+    def step(self, chi=None, stream=None):
+        """One time step; chi: a float64 device tensor (K, M) or None."""
+        if chi is not None:
+            assert chi.dtype.itemsize == 8 and tuple(chi.shape) == (self.K, self.M) and chi.is_contiguous()
+        check(lib().qz_laser_step(self.h, C.c_void_p(chi.data_ptr()) if chi is not None else None, stream))
+
+    def sums(self):
+        out = np.zeros(5)
+        check(lib().qz_laser_sums(self.h, out.ctypes.data_as(DP)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qz_laser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------------------- kernel-level entry points

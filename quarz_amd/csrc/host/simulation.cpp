@@ -926,6 +926,11 @@ Simulation::Simulation(const std::string& deck, const std::string& overrides) { 
     Config cfg;
     cfg.parse(deck);
     cfg.overrides(overrides);
+    // a laser (src/Simulation.cpp:272) pushes the plasma and the beams ponderomotively; this sweep and these pushes
+    // have no such term, so such a deck is refused here rather than run without it (the envelope alone: qz_laser_*)
+    if (cfg.has("laser.a0") || cfg.has("laser.file") || cfg.has("laser.comb_a"))
+        throw std::runtime_error("laser: this step has no ponderomotive coupling, so decks with a laser are not run "
+                                 "(the laser envelope alone is the qz_laser_* component)");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
     if (!cfg.has("xi.convention"))

@@ -40,6 +40,10 @@ def lib():
         L.or_thomas.argtypes = [DP, DP, DP, DP, DP, C.c_int]
         L.or_make_uniform.argtypes = [C.c_double, C.c_double, DP, C.c_int]
         L.or_make_regions.argtypes = [C.c_double, DP, DP, C.c_int, C.c_double, DP, C.c_int]
+        L.or_laser_gaussian.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 8 + [DP]
+        L.or_laser_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                    DP, DP, DP, DP]
+        L.or_laser_sums.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, DP, DP]
         L.or_solver_create.restype = C.c_void_p
         L.or_solver_create.argtypes = [DP, C.c_int]
         L.or_solver_destroy.argtypes = [C.c_void_p]
@@ -174,6 +178,31 @@ class Solver:
         t = np.zeros(len(xs))
         check(lib().or_grid_locate(self.h, len(xs), _p(xs), _p(j), _p(t)))
         return j, t
+
+    def laser_gaussian(self, K, xi_min, dxi, a0, w0, L0, xi0, focus, k0):
+        """The Gaussian pulse (l = 0) on this grid, (K, M, 2); xi, xi0 internal."""
+        a = np.zeros((K, self.M, 2))
+        check(lib().or_laser_gaussian(self.h, K, xi_min, dxi, a0, w0, L0, xi0, focus, k0, _p(a)))
+        return a
+
+    def laser_step(self, a, dxi, dt, k0, scheme, circular=False, chi=None):
+        """One envelope step over the whole box: (a^(n+1), pond (K, M, 3)).  scheme 0 trapezoid, 1 bdf2, 2 euler."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        K = a.shape[0]
+        assert a.shape == (K, self.M, 2)
+        if chi is not None:
+            chi = np.ascontiguousarray(chi, dtype=np.float64)
+            assert chi.shape == (K, self.M)
+        an = np.zeros_like(a)
+        pond = np.zeros((K, self.M, 3))
+        check(lib().or_laser_step(self.h, K, dxi, dt, k0, scheme, int(circular), _p(chi), _p(a), _p(an), _p(pond)))
+        return an, pond
+
+    def laser_sums(self, a, xi_min, dxi):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        out = np.zeros(5)
+        check(lib().or_laser_sums(self.h, a.shape[0], xi_min, dxi, _p(a), _p(out)))
+        return out
 
     def uniform_deposit(self, ppc, ntheta=1, mode1=False):
         out = np.zeros(self.M)

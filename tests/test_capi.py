@@ -194,3 +194,42 @@ def test_batch_plan_on_known_estimates():
         assert "no plan" in str(e)
     else:
         raise AssertionError("an unplannable batch was accepted")
+
+
+def test_laser_entry_points_reject_bad_arguments():
+    """The laser envelope's entry points answer null handles, bad extents, schemes and carriers with an error code and
+    a message, before any device work."""
+    import ctypes as C
+    import numpy as np
+    import quarz_amd
+    L = quarz_amd.lib()
+    nodes = np.linspace(0.0, 1.0, 11)
+    DP = C.POINTER(C.c_double)
+    out = C.c_void_p()
+    np_ = nodes.ctypes.data_as(DP)
+    assert L.qz_laser_create(None, 11, 10, 0.0, 0.1, 1.0, 10.0, 1, 0, C.byref(out)) == -1
+    assert "null" in quarz_amd.last_error()
+    assert L.qz_laser_create(np_, 2, 10, 0.0, 0.1, 1.0, 10.0, 1, 0, C.byref(out)) == -1
+    assert L.qz_laser_create(np_, 11, 10, 0.0, 0.1, 1.0, 10.0, 3, 0, C.byref(out)) == -1
+    assert "scheme" in quarz_amd.last_error()
+    assert L.qz_laser_create(np_, 11, 10, 0.0, 0.1, 1.0, 0.5, 1, 0, C.byref(out)) == -1
+    assert "underdense" in quarz_amd.last_error()
+    assert L.qz_laser_create(np_, 11, 10, 0.0, 0.0, 1.0, 10.0, 1, 0, C.byref(out)) == -1
+    big = np.linspace(0.0, 1.0, 2100)
+    assert L.qz_laser_create(big.ctypes.data_as(DP), 2100, 10, 0.0, 0.1, 1.0, 10.0, 1, 0, C.byref(out)) == -2
+    assert L.qz_laser_step(None, None, None) == -1 and "null" in quarz_amd.last_error()
+    assert L.qz_laser_gaussian(None, 0.1, 1.0, 1.0, 0.0, 0.0) == -1
+    assert L.qz_laser_sums(None, None) == -1
+    assert L.qz_laser_get(None, 0, None) == -1
+
+
+def test_laser_deck_refused_by_the_step():
+    """A deck with a laser (validation/laser_vacuum.in's keys) is refused by qz_sim_create before any device work:
+    the step has no ponderomotive coupling, so it does not run such a deck without it."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    deck = open(__file__.replace("test_capi.py", "decks/gain.in")).read() + "\nlaser.k0 = 20\nlaser.a0 = 0.1\n"
+    h = C.c_void_p()
+    assert L.qz_sim_create(deck.encode(), b"", C.byref(h)) == -1
+    assert "ponderomotive" in quarz_amd.last_error()

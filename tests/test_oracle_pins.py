@@ -484,3 +484,88 @@ def test_ion_motion_validation_quick(oracle, decks, tmp_path):
         for a, b, p in zip(series(d), R0, vals):
             err = 100 * abs(a - b).max() / abs(b).max()
             assert _fmt_matches(err, p), (name, err, p)
+
+
+# ---------------------------------------------------------------------------------- laser envelope
+@pytest.mark.parametrize("scheme", ["bdf2", "trapezoid"])
+def test_laser_vacuum_diffraction_validation(oracle, scheme):
+    """validation/reference_quick.txt:59-60 and 69-70 (run_validation.sh section 11, laser_vacuum.in checked by
+    laser_check.py vacuum): a Gaussian pulse through its focus against the Gaussian beam w(t) = w0 sqrt(1 + s^2),
+    a = a0 / sqrt(1 + s^2); spot size, peak amplitude, energy and the centroid's slip at the group velocity agree with
+    the printed line digit for digit.  A wrong sign of k0 or of the diffraction term, a lost factor of the xi
+    difference or a wrong carry changes these by orders of magnitude."""
+    from laser_checks import PRINTED, VAC, diag_row, vacuum_line
+    v = VAC
+    S = oracle.Solver(oracle.make_uniform(v["R"], v["dr"]))
+    code = {"trapezoid": 0, "bdf2": 1}[scheme]
+    a = S.laser_gaussian(v["K"], v["xi_min"], v["dxi"], v["a0"], v["w0"], v["L0"], v["xi0"], v["focus"], v["k0"])
+    chi = np.full((v["K"], S.M), v["chi"])
+    rows = []
+    for n in range(v["rows"]):
+        rows.append(diag_row(n, v["dt"] * n, S.laser_sums(a, v["xi_min"], v["dxi"])))
+        if n + 1 < v["rows"]:
+            a, _ = S.laser_step(a, v["dxi"], v["dt"], v["k0"], code, False, chi)
+    assert vacuum_line(rows) == PRINTED[scheme]
+
+
+def test_laser_pulse_closed_form(oracle):
+    """The Gaussian pulse of Laser::Laser against its closed form a0 / q exp(-r^2 / (w0^2 q)) exp(-(xi - xi0)^2 / L0^2),
+    q = 1 - i focus / Z_R, on a stretched grid (zero at the wall)."""
+    nodes = oracle.make_regions(10.0, [(2.0, 0.02), (10.0, 0.1)], 1.05)
+    S = oracle.Solver(nodes)
+    r = np.asarray(nodes)
+    K, dxi, a0, w0, L0, xi0, zf, k0 = 120, 0.07, 0.7, 2.5, 1.3, 4.0, 30.0, 12.0
+    a = S.laser_gaussian(K, 0.5, dxi, a0, w0, L0, xi0, zf, k0)
+    xi = 0.5 + np.arange(K) * dxi
+    q = 1 - 1j * zf / (k0 * w0 ** 2 / 2)
+    th = a0 / q * np.exp(-r[None, :] ** 2 / (w0 ** 2 * q)) * np.exp(-(xi[:, None] - xi0) ** 2 / L0 ** 2)[:, :1]
+    th[:, -1] = 0
+    got = a[..., 0] + 1j * a[..., 1]
+    assert np.abs(got - th).max() <= 1e-14 * a0
+
+
+def test_laser_ponderomotive_derivatives(oracle):
+    """prepare_slice: <a^2> = |a^|^2 / 2 (linear), |a^|^2 (circular); d<a^2>/dr and the backward d<a^2>/dxi against
+    the derivatives of the pulse at the focus, <a^2> = a0^2/2 exp(-2 r^2/w0^2 - 2 (xi - xi0)^2/L0^2), to their
+    second-order truncation errors."""
+    dr, dxi = 0.01, 0.01
+    S = oracle.Solver(oracle.make_uniform(8.0, dr))
+    r = np.asarray(S.nodes)
+    K, a0, w0, L0, xi0 = 600, 0.4, 2.0, 1.5, 3.0
+    a = S.laser_gaussian(K, 0.0, dxi, a0, w0, L0, xi0, 0.0, 10.0)
+    xi = np.arange(K) * dxi
+    aa = 0.5 * a0 ** 2 * np.exp(-2 * r[None, :] ** 2 / w0 ** 2 - 2 * (xi[:, None] - xi0) ** 2 / L0 ** 2)
+    aa[:, -1] = 0
+    for circular, fac in ((False, 1.0), (True, 2.0)):
+        _, pond = S.laser_step(a, dxi, 0.0, 10.0, 1, circular)
+        assert np.abs(pond[..., 0] - fac * aa).max() <= 1e-15
+        inner = slice(1, S.M - 20)
+        daa_r = aa * (-4 * r[None, :] / w0 ** 2)
+        assert np.abs(pond[:, inner, 1] - fac * daa_r[:, inner]).max() <= 5e-5 * fac * np.abs(daa_r).max()
+        daa_x = aa * (-4 * (xi[:, None] - xi0) / L0 ** 2)
+        assert np.abs(pond[2:, inner, 2] - fac * daa_x[2:, inner]).max() <= 5e-4 * fac * np.abs(daa_x).max()
+        assert np.allclose(pond[0, :, 2], 1.5 * pond[0, :, 0] / dxi, rtol=1e-14, atol=0)   # the head: m1 = m2 = 0
+
+
+@pytest.mark.parametrize("scheme,code", [("trapezoid", 0), ("bdf2", 1), ("euler", 2)])
+def test_laser_energy_in_plasma(oracle, scheme, code):
+    """With a real susceptibility H = L0 - chi stays self-adjoint in the volume-weighted product, so the trapezoid in
+    xi (the box scheme, neutral) keeps int |a|^2 dV to rounding, while BDF2 and backward Euler take energy out every
+    step, Euler (first order) much more; a parabolic channel on a stretched grid."""
+    nodes = oracle.make_regions(12.0, [(3.0, 0.02), (12.0, 0.08)], 1.05)
+    S = oracle.Solver(nodes)
+    r = np.asarray(nodes)
+    K, dxi = 301, 0.05
+    chi = np.tile(0.2 * (1 + r ** 2 / 16.0), (K, 1))
+    a = S.laser_gaussian(K, 0.0, dxi, 0.5, 3.0, 2.0, 7.0, 10.0, 10.0)
+    E = [S.laser_sums(a, 0.0, dxi)[0]]
+    for _ in range(20):
+        a, _ = S.laser_step(a, dxi, 2.5, 10.0, code, False, chi)
+        E.append(S.laser_sums(a, 0.0, dxi)[0])
+    E = np.array(E)
+    if code == 0:
+        assert np.abs(E / E[0] - 1).max() < 1e-12
+    else:
+        assert np.all(np.diff(E) < 0)
+        loss = 1 - E[-1] / E[0]
+        assert (1e-7 < loss < 1e-5) if code == 1 else (1e-4 < loss < 1e-2)
