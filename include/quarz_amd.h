@@ -430,6 +430,11 @@ int qz_laser_device(qz_laser* L, int what, double** ptr);
  * stream: NULL runs on the laser's own stream and waits for it; otherwise the call is asynchronous on `stream`,
  * and the caller synchronizes `stream` before the next qz_laser_* call. */
 int qz_laser_step(qz_laser* L, const double* chi, void* stream);
+/* The same step for n distinct boxes at once (not in the reference): each box's march is one workgroup, so the
+ * boxes march side by side; box i's results are those of qz_laser_step(Ls[i], chis[i], ...).  chis: n device
+ * arrays (or NULL entries, or a NULL array: vacuum).  The call first waits for earlier work on its stream (NULL:
+ * the first box's own stream, waited for at the end too).  Boxes may differ in grid, box, scheme and dt. */
+int qz_laser_step_batched(qz_laser* const* Ls, int n, const double* const* chis, void* stream);
 /* Laser::sums (src/Laser.cpp:421-438) of the current envelope, summed on the host in slice and node order:
  * {sum |a|^2 dV, sum |a|^2 xi dV (the user's xi = -xi), sum |a|^2 r^2 dV, max |a|, sum |a|^2 xi^2 dV} */
 int qz_laser_sums(qz_laser* L, double* out5);

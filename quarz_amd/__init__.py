@@ -143,6 +143,7 @@ def lib():
         L.qz_laser_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.qz_laser_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.qz_laser_sums.argtypes = [C.c_void_p, DP]
+        L.qz_laser_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_void_p]
         L.qz_sweep_adjoint_workspace_doubles.argtypes = [C.c_int, C.c_int]
         L.qz_sweep_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.qz_beam_spread_adjoint.argtypes = [C.c_int, C.POINTER(BeamSoAC), C.c_double, C.c_double, C.c_double,
@@ -555,6 +556,21 @@ class Laser:
             self.close()
         except Exception:
             pass
+
+
+def laser_step_batched(lasers, chis=None, stream=None):
+    """One step of several distinct Laser boxes in one launch per kernel (each box's march is one workgroup);
+    chis: None or one float64 device tensor (K, M) or None per box."""
+    n = len(lasers)
+    hs = (C.c_void_p * n)(*[x.h.value for x in lasers])
+    cs = None
+    if chis is not None:
+        assert len(chis) == n
+        for x, c in zip(lasers, chis):
+            if c is not None:
+                assert c.dtype.itemsize == 8 and tuple(c.shape) == (x.K, x.M) and c.is_contiguous()
+        cs = (C.c_void_p * n)(*[c.data_ptr() if c is not None else None for c in chis])
+    check(lib().qz_laser_step_batched(hs, n, cs, stream))
 
 
 # ---------------------------------------------------------------------------- kernel-level entry points

@@ -56,14 +56,9 @@ __device__ __forceinline__ Cx march_alpha(double dxi, double dt, double k0, int 
     return Cx{s1, -s2 * k0};
 }
 
-__global__ __launch_bounds__(kPondThreads) void laser_pond_kernel(const double* __restrict__ a,
-                                                                  double* __restrict__ pond,
-                                                                  const double* __restrict__ gm,
-                                                                  const double* __restrict__ g0,
-                                                                  const double* __restrict__ gp, int K, int M,
-                                                                  double fac, double idxi) {
-    const long i = static_cast<long>(blockIdx.x) * kPondThreads + threadIdx.x;
-    if (i >= static_cast<long>(K) * M) return;
+__device__ __forceinline__ void pond_node(long i, const double* __restrict__ a, double* __restrict__ pond,
+                                          const double* __restrict__ gm, const double* __restrict__ g0,
+                                          const double* __restrict__ gp, int M, double fac, double idxi) {
     const int k = static_cast<int>(i / M), j = static_cast<int>(i - static_cast<long>(k) * M);
     const int N = M - 1;
     auto mag = [&](int l, int jj) {   // <a^2> / fac at (slice l, node jj)
@@ -89,14 +84,10 @@ __global__ __launch_bounds__(kPondThreads) void laser_pond_kernel(const double* 
 // susceptibility and the operator, not on the carry, so only the right-hand side's recurrence stays in the
 // march.  fac: (K, N, 4) doubles, m.re, m.im, c'.re, c'.im of (slice k, node j): slice-major, so that the march's
 // workgroup reads a slice's rows with neighbouring addresses.
-__global__ __launch_bounds__(kFactorThreads) void laser_factor_kernel(const double* __restrict__ chi,
-                                                                      const double* __restrict__ la,
-                                                                      const double* __restrict__ lb,
-                                                                      const double* __restrict__ lc,
-                                                                      double* __restrict__ fac, int K, int M,
-                                                                      double dxi, double dt, double k0, int scheme) {
-    const int k = blockIdx.x * kFactorThreads + threadIdx.x;
-    if (k >= K) return;
+__device__ __forceinline__ void factor_slice(int k, const double* __restrict__ chi, const double* __restrict__ la,
+                                             const double* __restrict__ lb, const double* __restrict__ lc,
+                                             double* __restrict__ fac, int M, double dxi, double dt, double k0,
+                                             int scheme) {
     const int N = M - 1;
     const Cx alpha = march_alpha(dxi, dt, k0, scheme);
     const double* ch = chi ? chi + static_cast<long>(k) * M : nullptr;
@@ -114,15 +105,12 @@ __global__ __launch_bounds__(kFactorThreads) void laser_factor_kernel(const doub
     }
 }
 
-// One workgroup marches the K slices.  Every barrier is reached by all threads in every slice.
-__global__ __launch_bounds__(kMarchThreads) void laser_march_kernel(const double* __restrict__ a,
-                                                                    const double* __restrict__ chi,
-                                                                    double* __restrict__ an,
-                                                                    const double* __restrict__ la,
-                                                                    const double* __restrict__ lb,
-                                                                    const double* __restrict__ lc,
-                                                                    const double* __restrict__ fac, int K, int M,
-                                                                    double dxi, double dt, double k0, int scheme) {
+// One workgroup marches the K slices of one box.  Every barrier is reached by all threads in every slice.
+__device__ __forceinline__ void march_box(const double* __restrict__ a, const double* __restrict__ chi,
+                                          double* __restrict__ an, const double* __restrict__ la,
+                                          const double* __restrict__ lb, const double* __restrict__ lc,
+                                          const double* __restrict__ fac, int K, int M, double dxi, double dt,
+                                          double k0, int scheme) {
     extern __shared__ double lds[];
     Cx* cpl = reinterpret_cast<Cx*>(lds);   // c' of the slice
     Cx* ml = cpl + M;                       // pivots m of the slice
@@ -230,6 +218,42 @@ __global__ __launch_bounds__(kMarchThreads) void laser_march_kernel(const double
     }
 }
 
+// One box of a step, the kernels' argument: its arrays, operator rows and constants.  active = 0: a box whose
+// envelope is not advanced (dt <= 0); only its ponderomotive arrays are formed.
+struct LaserBox {
+    const double* a;
+    const double* chi;
+    double *an, *pond, *fac;
+    const double *la, *lb, *lc, *gm, *g0, *gp;
+    int K, M, scheme, active;
+    double dxi, dt, k0, fac2, idxi;   // fac2: <a^2> / |a^|^2 (1 circular, 1/2 linear); idxi = 1 / dxi
+};
+
+// kBatch = false: one box, passed by value; true: box blockIdx.y (pond, factor) or blockIdx.x (march) of `many`
+template <bool kBatch>
+__global__ __launch_bounds__(kPondThreads) void laser_pond_kernel(LaserBox one, const LaserBox* __restrict__ many) {
+    const LaserBox B = kBatch ? many[blockIdx.y] : one;
+    const long i = static_cast<long>(blockIdx.x) * kPondThreads + threadIdx.x;
+    if (i >= static_cast<long>(B.K) * B.M) return;
+    pond_node(i, B.a, B.pond, B.gm, B.g0, B.gp, B.M, B.fac2, B.idxi);
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kFactorThreads) void laser_factor_kernel(LaserBox one,
+                                                                      const LaserBox* __restrict__ many) {
+    const LaserBox B = kBatch ? many[blockIdx.y] : one;
+    const int k = blockIdx.x * kFactorThreads + threadIdx.x;
+    if (!B.active || k >= B.K) return;
+    factor_slice(k, B.chi, B.la, B.lb, B.lc, B.fac, B.M, B.dxi, B.dt, B.k0, B.scheme);
+}
+
+template <bool kBatch>
+__global__ __launch_bounds__(kMarchThreads) void laser_march_kernel(LaserBox one, const LaserBox* __restrict__ many) {
+    const LaserBox B = kBatch ? many[blockIdx.x] : one;
+    if (!B.active) return;   // the whole workgroup, before any barrier
+    march_box(B.a, B.chi, B.an, B.la, B.lb, B.lc, B.fac, B.K, B.M, B.dxi, B.dt, B.k0, B.scheme);
+}
+
 long march_lds(int M) { return static_cast<long>(kMarchLdsPerNode) * M * static_cast<long>(sizeof(double)); }
 
 // glibc exp / cos / sin called through pointers, so that no compiler folds or replaces them: the pulse is
@@ -248,6 +272,9 @@ struct qz_laser {
     double xi_min = 0, dxi = 0, dt = 0, k0 = 0;
     qz::DevBuf<Real> a, an, pond, fac;
     hipStream_t own = nullptr;
+    // a batched step led by this handle: the boxes' descriptors (kept until the next such step) and their copy
+    std::vector<LaserBox> boxes;
+    qz::DevBuf<char> dboxes;
     ~qz_laser() {
         if (own) (void)hipStreamDestroy(own);
     }
@@ -352,37 +379,98 @@ extern "C" int qz_laser_device(qz_laser* L, int what, double** ptr) {
     return QZ_OK;
 }
 
+static LaserBox box_of(qz_laser* L, const double* chi) {
+    LaserBox B{};
+    B.a = L->a.p; B.chi = chi; B.an = L->an.p; B.pond = L->pond.p; B.fac = L->fac.p;
+    B.la = L->solver.d_a[qz::FieldSolver::L0].p;
+    B.lb = L->solver.d_b[qz::FieldSolver::L0].p;
+    B.lc = L->solver.d_c[qz::FieldSolver::L0].p;
+    B.gm = L->solver.d_gm.p; B.g0 = L->solver.d_g0.p; B.gp = L->solver.d_gp.p;
+    B.K = L->K; B.M = L->M; B.scheme = L->scheme; B.active = L->dt > 0 ? 1 : 0;
+    B.dxi = L->dxi; B.dt = L->dt; B.k0 = L->k0;
+    B.fac2 = L->circular ? 1.0 : 0.5;
+    B.idxi = 1.0 / L->dxi;
+    return B;
+}
+
+static int set_march_lds(long lds) {
+    QZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(laser_march_kernel<false>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    QZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(laser_march_kernel<true>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    return QZ_OK;
+}
+
 // One time step of the envelope: prepare_slice of every slice (the ponderomotive arrays at level n), then the
 // march of advance_slice from the head of the box; the advanced envelope becomes the current one
 // (Laser::begin_step of the next step, src/Laser.cpp:258-262).  chi: (K, M) device array, or null for vacuum.
 extern "C" int qz_laser_step(qz_laser* L, const double* chi, void* stream) {
     if (!L) return qz_fail(QZ_EINVAL, "qz_laser_step: null argument");
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : L->own;
-    const int K = L->K, M = L->M;
-    const long n = static_cast<long>(K) * M;
-    const double fac = L->circular ? 1.0 : 0.5;
-    const double idxi = 1.0 / L->dxi;
-    const unsigned pond_blocks = static_cast<unsigned>((n + kPondThreads - 1) / kPondThreads);
-    hipLaunchKernelGGL(laser_pond_kernel, dim3(pond_blocks), dim3(kPondThreads), 0, st, L->a.p, L->pond.p,
-                       L->solver.d_gm.p, L->solver.d_g0.p, L->solver.d_gp.p, K, M, fac, idxi);
+    const LaserBox B = box_of(L, chi);
+    const long n = static_cast<long>(B.K) * B.M;
+    hipLaunchKernelGGL(laser_pond_kernel<false>, dim3(static_cast<unsigned>((n + kPondThreads - 1) / kPondThreads)),
+                       dim3(kPondThreads), 0, st, B, nullptr);
     QZ_HIP(hipGetLastError());
-    if (L->dt > 0) {
-        const size_t lds = static_cast<size_t>(march_lds(M));
-        QZ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(laser_march_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        const double* la = L->solver.d_a[qz::FieldSolver::L0].p;
-        const double* lb = L->solver.d_b[qz::FieldSolver::L0].p;
-        const double* lc = L->solver.d_c[qz::FieldSolver::L0].p;
-        hipLaunchKernelGGL(laser_factor_kernel, dim3((K + kFactorThreads - 1) / kFactorThreads), dim3(kFactorThreads),
-                           0, st, chi, la, lb, lc, L->fac.p, K, M, L->dxi, L->dt, L->k0, L->scheme);
+    if (B.active) {
+        const int rc = set_march_lds(march_lds(B.M));
+        if (rc != QZ_OK) return rc;
+        hipLaunchKernelGGL(laser_factor_kernel<false>, dim3((B.K + kFactorThreads - 1) / kFactorThreads),
+                           dim3(kFactorThreads), 0, st, B, nullptr);
         QZ_HIP(hipGetLastError());
-        hipLaunchKernelGGL(laser_march_kernel, dim3(1), dim3(kMarchThreads), lds, st, L->a.p, chi, L->an.p, la, lb,
-                           lc, L->fac.p, K, M, L->dxi, L->dt, L->k0, L->scheme);
+        hipLaunchKernelGGL(laser_march_kernel<false>, dim3(1), dim3(kMarchThreads), march_lds(B.M), st, B, nullptr);
         QZ_HIP(hipGetLastError());
         std::swap(L->a, L->an);
     }
     if (!stream) QZ_HIP(hipStreamSynchronize(st));
     return QZ_OK;
+}
+
+// Many boxes, one step each, in one launch per kernel: every box's march is one workgroup, so the boxes march
+// at once.  Each box's results are those of qz_laser_step.
+extern "C" int qz_laser_step_batched(qz_laser* const* Ls, int n, const double* const* chis, void* stream) {
+    if (!Ls || n < 1) return qz_fail(QZ_EINVAL, "qz_laser_step_batched: no boxes");
+    if (n > 65535) return qz_fail(QZ_ERANGE, "qz_laser_step_batched: at most 65535 boxes per call");
+    for (int i = 0; i < n; ++i) {
+        if (!Ls[i]) return qz_fail(QZ_EINVAL, "qz_laser_step_batched: null box");
+        for (int j = 0; j < i; ++j)
+            if (Ls[j] == Ls[i]) return qz_fail(QZ_EINVAL, "qz_laser_step_batched: a box appears twice");
+    }
+    qz_laser* L0 = Ls[0];
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : L0->own;
+    QZ_LASER_TRY({
+        // the descriptors of an earlier batched step on this stream may still be read: wait for it
+        laser_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        L0->boxes.resize(static_cast<size_t>(n));
+        long maxn = 1, lds = 0;
+        int maxk = 1;
+        for (int i = 0; i < n; ++i) {
+            L0->boxes[i] = box_of(Ls[i], chis ? chis[i] : nullptr);
+            const LaserBox& B = L0->boxes[i];
+            maxn = std::max(maxn, static_cast<long>(B.K) * B.M);
+            maxk = std::max(maxk, B.K);
+            if (B.active) lds = std::max(lds, march_lds(B.M));
+        }
+        const size_t bytes = sizeof(LaserBox) * static_cast<size_t>(n);
+        if (L0->dboxes.n < bytes) L0->dboxes.alloc(bytes);
+        laser_check(hipMemcpyAsync(L0->dboxes.p, L0->boxes.data(), bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+        const LaserBox* dev = reinterpret_cast<const LaserBox*>(L0->dboxes.p);
+        hipLaunchKernelGGL(laser_pond_kernel<true>,
+                           dim3(static_cast<unsigned>((maxn + kPondThreads - 1) / kPondThreads), n),
+                           dim3(kPondThreads), 0, st, LaserBox{}, dev);
+        laser_check(hipGetLastError(), "laser_pond_kernel");
+        if (lds > 0) {
+            if (set_march_lds(lds) != QZ_OK) throw std::runtime_error(qz_last_error());
+            hipLaunchKernelGGL(laser_factor_kernel<true>, dim3((maxk + kFactorThreads - 1) / kFactorThreads, n),
+                               dim3(kFactorThreads), 0, st, LaserBox{}, dev);
+            laser_check(hipGetLastError(), "laser_factor_kernel");
+            hipLaunchKernelGGL(laser_march_kernel<true>, dim3(n), dim3(kMarchThreads), lds, st, LaserBox{}, dev);
+            laser_check(hipGetLastError(), "laser_march_kernel");
+            for (int i = 0; i < n; ++i)
+                if (L0->boxes[i].active) std::swap(Ls[i]->a, Ls[i]->an);
+        }
+        if (!stream) laser_check(hipStreamSynchronize(st), "hipStreamSynchronize");
+    })
 }
 
 // Laser::sums (src/Laser.cpp:421-438) of the current envelope, on the host in slice and node order:

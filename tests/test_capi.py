@@ -233,3 +233,13 @@ def test_laser_deck_refused_by_the_step():
     h = C.c_void_p()
     assert L.qz_sim_create(deck.encode(), b"", C.byref(h)) == -1
     assert "ponderomotive" in quarz_amd.last_error()
+
+
+def test_laser_batched_step_rejects_bad_arguments():
+    """qz_laser_step_batched answers an empty batch and a null box with QZ_EINVAL before any device work."""
+    import ctypes as C
+    import quarz_amd
+    L = quarz_amd.lib()
+    assert L.qz_laser_step_batched(None, 0, None, None) == -1 and "no boxes" in quarz_amd.last_error()
+    hs = (C.c_void_p * 2)(None, None)
+    assert L.qz_laser_step_batched(hs, 2, None, None) == -1 and "null box" in quarz_amd.last_error()

@@ -128,3 +128,65 @@ def test_laser_long_box_bit_exact_and_timed(qa, oracle):
             assert np.array_equal(L.ponderomotive(), pond), n
         print(f"laser step M={S.M} K={K}: {ms:.2f} ms ({ms * 1e3 / K:.2f} us per slice)")
         L.close()
+
+
+def test_laser_batched_equals_single(qa, oracle):
+    """Four different boxes stepped together (qz_laser_step_batched): grids, box lengths, schemes, polarisations,
+    a vacuum box and one that is not advanced (dt = 0) -- each equals the oracle's step of that box alone, twice."""
+    import torch
+    specs = [  # (nodes, K, dxi, dt, k0, scheme, circular, chi?)
+        (oracle.make_uniform(14.0, 0.05), 201, 0.05, 2.0, 20.0, "bdf2", False, True),
+        (oracle.make_regions(12.0, [(3.0, 0.02), (12.0, 0.08)], 1.05), 257, 0.05, 2.5, 10.0, "trapezoid", True, True),
+        (oracle.make_uniform(8.0, 8.0 / 512), 129, 0.01, 1.0, 12.0, "euler", False, False),
+        (oracle.make_uniform(6.0, 0.03), 97, 0.04, 0.0, 5.0, "bdf2", False, True),
+    ]
+    code = {"trapezoid": 0, "bdf2": 1, "euler": 2}
+    rng = np.random.default_rng(21)
+    boxes, lasers, chis, chi_h, a = [], [], [], [], []
+    for nodes, K, dxi, dt, k0, scheme, circ, has_chi in specs:
+        S = oracle.Solver(nodes)
+        r = np.asarray(nodes)
+        c = 0.2 * (1 + r[None, :] ** 2 / 16.0) * (1 + 0.05 * rng.standard_normal((K, S.M))) if has_chi else None
+        L = qa.Laser(nodes, K, 0.0, dxi, dt, k0, scheme=scheme, circular=circ)
+        a0 = S.laser_gaussian(K, 0.0, dxi, 0.3, 2.0, 1.5, K * dxi / 2, 5.0, k0)
+        L.set_envelope(a0)
+        boxes.append(S); lasers.append(L); chi_h.append(c); a.append(a0)
+        chis.append(torch.from_numpy(c).cuda() if c is not None else None)
+    for step in range(2):
+        qa.laser_step_batched(lasers, chis)
+        for i, (nodes, K, dxi, dt, k0, scheme, circ, _) in enumerate(specs):
+            a[i], pond = boxes[i].laser_step(a[i], dxi, dt, k0, code[scheme], circ, chi_h[i])
+            assert np.array_equal(lasers[i].envelope(), a[i]), (step, i)
+            assert np.array_equal(lasers[i].ponderomotive(), pond), (step, i)
+    for L in lasers:
+        L.close()
+
+
+def test_laser_batched_throughput(qa, oracle):
+    """256 boxes of 4001 slices x 513 nodes in one batched step (one march workgroup per box, 46 GB of HBM): two
+    boxes checked against the oracle, the time per box printed against one box stepped alone."""
+    import time
+    import torch
+    nodes = oracle.make_uniform(8.0, 8.0 / 512)
+    S = oracle.Solver(nodes)
+    K, dxi, nb = 4001, 0.005, 256
+    r = np.asarray(nodes)
+    a0 = S.laser_gaussian(K, 0.0, dxi, 0.5, 2.0, 1.0, 10.0, 0.0, 10.0)
+    chi_h = np.tile(0.1 * (1 + r ** 2 / 9.0), (K, 1))
+    chi = torch.from_numpy(chi_h).cuda()
+    lasers = []
+    for _ in range(nb):
+        L = qa.Laser(nodes, K, 0.0, dxi, 1.0, 10.0, scheme="bdf2")
+        L.set_envelope(a0)
+        lasers.append(L)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qa.laser_step_batched(lasers, [chi] * nb)
+    ms = (time.perf_counter() - t0) * 1e3
+    a1, pond = S.laser_step(a0, dxi, 1.0, 10.0, 1, False, chi_h)
+    for i in (0, nb - 1):
+        assert np.array_equal(lasers[i].envelope(), a1), i
+        assert np.array_equal(lasers[i].ponderomotive(), pond), i
+    print(f"laser batched step: {nb} boxes of {K} x {S.M} in {ms:.1f} ms, {ms / nb:.2f} ms per box")
+    for L in lasers:
+        L.close()
