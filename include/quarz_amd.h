@@ -121,6 +121,9 @@ typedef struct qz_sweep_args {
     double* traj;              /* optional, one workgroup: (k_end - k_begin) x 6 x np doubles, np = the species' counts
                                   summed: x, y, px, py, Delta, w of every particle (species 0 then 1) at the start
                                   of each slice of the launch (the adjoint's recomputed segments) */
+    double* chi_out;           /* optional: (K, M) doubles; row k receives slice k's plasma susceptibility chi_0 (the
+                                  mode-0 chi of the B+ solves) as the sweep forms it, the chi that advance_slice of
+                                  the laser envelope takes (src/Simulation.cpp:735) */
 } qz_sweep_args;
 
 const char* qz_last_error(void);
@@ -324,6 +327,12 @@ int qz_sim_spread_grad(qz_sim* s, int b, int every, double* J, double* grad, int
 int qz_sim_adjoint_ticks(qz_sim* s, double* out);
 /* Number of values of beam b's weight profile (<beam>.pwl_weights; 0 without one), or a negative error. */
 int qz_sim_profile_size(qz_sim* s, int b);
+/* Record chi_0 of every slice of the coming sweeps (qz_sweep_args.chi_out) into a (K, M) device array of the run:
+ * the plasma susceptibility that the laser envelope's advance_slice takes (src/Simulation.cpp:735), ready to pass
+ * to qz_laser_step.  on = 0 stops and frees it.  qz_sim_get(s, "chi", ...) copies it to the host;
+ * qz_sim_chi_device gives its device address (NULL when not recording).  Set before a qz_batch is captured. */
+int qz_sim_record_chi(qz_sim* s, int on);
+int qz_sim_chi_device(qz_sim* s, double** ptr);
 int qz_sim_snapshot(qz_sim* s);
 int qz_sim_rewind(qz_sim* s);
 /* times of the last sweep and beam push (ms, HIP events on the simulation's stream) */

@@ -1564,6 +1564,8 @@ struct Sim {
     MVector gz;
     int M = 0, K = 0, NCF = 0, NCB = 0, NCD = 0;
     std::vector<Real> fld, bsrc, diag;
+    bool record_chi = false;
+    std::vector<Real> chi_rec;   // (K, M): chi_0 of every slice of the last sweep, when record_chi
     long lost_last = 0;
     double t_sweep = 0, t_beam = 0;
     double particle_slices = 0;   // live mobile particles summed over the slices of the last sweep
@@ -1884,6 +1886,8 @@ struct Sim {
             for (auto& sp : species)
                 if (sp.mobile) sp.template deposit<PAR>(src);
             combine_sources(kl);
+            if (record_chi)   // chi_0 of the slice: what advance_slice of the laser takes (src/Simulation.cpp:735)
+                std::copy(src.chi.a0.begin(), src.chi.a0.end(), chi_rec.begin() + static_cast<size_t>(kl) * M);
             solve_slice_fields<PAR>();
             store_slice(kl);
             if (k < Kg - 1)
@@ -2272,6 +2276,7 @@ long or_sim_get(void* h, const char* what, double* out, long cap) {
     const std::vector<double>* v = nullptr;
     std::string w(what);
     if (w == "fld") v = &s.fld;
+    else if (w == "chi") v = &s.chi_rec;
     else if (w == "diag") v = &s.diag;
     else if (w == "bsrc") v = &s.bsrc;
     else if (w == "nodes") v = &s.grid.r;
@@ -2343,6 +2348,13 @@ long or_sim_adjoint_get(void* h, int which, double* out, long cap) {
     const std::vector<double>& v = which == 0 ? s.adj_lf : which == 1 ? s.adj_lb : s.adj_lw;
     if (out && cap >= static_cast<long>(v.size())) std::copy(v.begin(), v.end(), out);
     return static_cast<long>(v.size());
+}
+// record chi_0 of every slice in the coming sweeps (or_sim_get "chi")
+int or_sim_record_chi(void* h, int on) {
+    Sim& s = *static_cast<Sim*>(h);
+    s.record_chi = on != 0;
+    s.chi_rec.assign(on ? static_cast<size_t>(s.box.nloc) * s.M : 0, 0.0);
+    return 0;
 }
 int or_sim_write_fields(void* h, const char* path) { OR_TRY(write_fields(*static_cast<Sim*>(h), path)) }
 int or_sim_write_axis(void* h, const char* path) { OR_TRY(write_axis(*static_cast<Sim*>(h), path)) }

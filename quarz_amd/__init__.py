@@ -143,6 +143,8 @@ def lib():
         L.qz_laser_device.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
         L.qz_laser_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.qz_laser_sums.argtypes = [C.c_void_p, DP]
+        L.qz_sim_record_chi.argtypes = [C.c_void_p, C.c_int]
+        L.qz_sim_chi_device.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
         L.qz_laser_step_batched.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p), C.c_void_p]
         L.qz_sweep_adjoint_workspace_doubles.argtypes = [C.c_int, C.c_int]
         L.qz_sweep_adjoint.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
@@ -333,6 +335,20 @@ class Simulation:
     @property
     def fld(self):
         return self.get("fld").reshape(self.K, self.M, self.NCF)
+
+    def record_chi(self, on=True):
+        """Record chi_0 of every slice of the coming sweeps (the plasma susceptibility a Laser step takes)."""
+        check(lib().qz_sim_record_chi(self.h, int(bool(on))))
+
+    def chi(self):
+        """The recorded chi_0, (K, M), host copy."""
+        return self.get("chi").reshape(self.K, self.M)
+
+    def chi_device(self):
+        """The device address of the recorded chi_0 (K, M), for Laser.step; 0 when not recording."""
+        p = C.c_void_p()
+        check(lib().qz_sim_chi_device(self.h, C.byref(p)))
+        return p.value or 0
 
     @property
     def diag(self):
@@ -536,10 +552,16 @@ class Laser:
         return out
 
     def step(self, chi=None, stream=None):
-        """One time step; chi: a float64 device tensor (K, M) or None."""
-        if chi is not None:
+        """One time step; chi: a float64 device tensor (K, M), the device address of such an array (for example
+        Simulation.chi_device() of a run on the same grid and box), or None (vacuum)."""
+        if isinstance(chi, int):
+            ptr = C.c_void_p(chi) if chi else None
+        elif chi is not None:
             assert chi.dtype.itemsize == 8 and tuple(chi.shape) == (self.K, self.M) and chi.is_contiguous()
-        check(lib().qz_laser_step(self.h, C.c_void_p(chi.data_ptr()) if chi is not None else None, stream))
+            ptr = C.c_void_p(chi.data_ptr())
+        else:
+            ptr = None
+        check(lib().qz_laser_step(self.h, ptr, stream))
 
     def sums(self):
         out = np.zeros(5)

@@ -1095,6 +1095,7 @@ void Simulation::prepare_sweep(bool timed) {
     qz_check(qz_zero_async(lost.p, sizeof(int) * lost.n, stream));
     // ---- the sweep
     qz_sweep_args A{};
+    A.chi_out = chi_rec.p;   // null unless recording
     A.grid = g;
     A.modes = m1 ? 1 : 0;
     A.picard = picard;
@@ -2471,7 +2472,10 @@ extern "C" long qz_sim_get(qz_sim* h, const char* what, double* out, long cap) {
         const std::string w(what);
         std::vector<double> v;
         if (w == "fld") s.fld.download(v, s.stream);
-        else if (w == "diag") s.diag.download(v, s.stream);
+        else if (w == "chi") {
+            if (!s.chi_rec.p) return -1;
+            s.chi_rec.download(v, s.stream);
+        } else if (w == "diag") s.diag.download(v, s.stream);
         else if (w == "bsrc") s.bsrc.download(v, s.stream);
         else if (w == "nodes") v = s.grid.r;
         else if (w.rfind("beam", 0) == 0) {
@@ -2532,6 +2536,31 @@ extern "C" int qz_sim_adjoint_ticks(qz_sim* h, double* out) {
     for (int i = 0; i < QZ_ADJ_NPHASE; ++i) out[i] = h->s->adj ? h->s->adj->last_ticks[i] : 0.0;
     return QZ_OK;
 }
+// chi_0 of every slice of the coming sweeps into a (K, M) device array (the plasma susceptibility the laser
+// envelope's advance_slice takes, src/Simulation.cpp:735); on = 0 stops and frees it
+extern "C" int qz_sim_record_chi(qz_sim* h, int on) {
+    QZ_NEED(h, "qz_sim_record_chi");
+    QZ_SIM_TRY({
+        auto& s = *h->s;
+        if (on) {
+            if (!s.chi_rec.p) {
+                s.chi_rec.alloc(static_cast<size_t>(s.K) * s.M);
+                qz::qz_check(qz_zero_async(s.chi_rec.p, sizeof(double) * s.chi_rec.n, s.stream));
+                qz::hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+            }
+        } else {
+            qz::hip_check(hipStreamSynchronize(s.stream), "hipStreamSynchronize");
+            s.chi_rec.release();
+        }
+    })
+}
+// the device address of that array (null when not recording), valid until recording stops
+extern "C" int qz_sim_chi_device(qz_sim* h, double** ptr) {
+    QZ_NEED(h && ptr, "qz_sim_chi_device");
+    *ptr = h->s->chi_rec.p;
+    return QZ_OK;
+}
+
 extern "C" int qz_sim_profile_size(qz_sim* h, int b) {
     if (!h) return qz_fail(QZ_EINVAL, "qz_sim_profile_size: null run");
     if (b < 0 || b >= static_cast<int>(h->s->beams.size())) return qz_fail(QZ_EINVAL, "qz_sim_profile_size: no such beam");

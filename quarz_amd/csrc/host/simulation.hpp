@@ -198,6 +198,7 @@ public:
     std::vector<std::unique_ptr<Beam>> beams;
     int M = 0, K = 0, NCF = 0, NCB = 0, NCD = 0;
     DevBuf<Real> fld, bsrc, diag, bg, ws;
+    DevBuf<Real> chi_rec;   // (K, M): chi_0 of every slice of the sweeps while recording (qz_sim_record_chi)
     DevBuf<int> lost;
     DevBuf<unsigned long long> counters;   // [0] particle-slices, [1..16] sweep phase ticks, [20] beam deposited, [21] pushed
     DevBuf<char> dep_work;

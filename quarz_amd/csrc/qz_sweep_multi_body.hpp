@@ -511,6 +511,7 @@ __device__ __forceinline__ void sweep_multi_run(const qz_sweep_args* __restrict_
                 for (int j = tid; j < M; j += mBS) {
                     BA(0)[j] = A.op_a[OP_L1D][j]; BA(1)[j] = A.op_b[OP_L1D][j]; BA(2)[j] = A.op_c[OP_L1D][j];
                     BA(3)[j] = WS(W_CHI0)[j];
+                    if (A.chi_out) A.chi_out[static_cast<size_t>(k) * M + j] = WS(W_CHI0)[j];   // for the laser march
                     if (M1) {
                         BA(4)[j] = A.op_a[OP_L0][j]; BA(5)[j] = A.op_b[OP_L0][j]; BA(6)[j] = A.op_c[OP_L0][j];
                         BA(7)[j] = A.op_a[OP_L2D][j]; BA(8)[j] = A.op_b[OP_L2D][j]; BA(9)[j] = A.op_c[OP_L2D][j];

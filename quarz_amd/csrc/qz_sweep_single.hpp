@@ -518,6 +518,7 @@ __device__ __forceinline__ void sweep_run(const qz_sweep_args& A) {
                 BA(LY::L1B)[j] = A.op_b[OP_L1D][j];
                 BA(LY::L1C)[j] = A.op_c[OP_L1D][j];
                 BA(LY::CHI0)[j] = WS(W_CHI0)[j];
+                if (A.chi_out) A.chi_out[static_cast<size_t>(k) * M + j] = WS(W_CHI0)[j];   // for the laser march
                 if (M1) {
                     BA(LY::L1M)[j] = A.fac_l1_m[j];
                     BA(LY::L1CP)[j] = A.fac_l1_cp[j];

@@ -40,6 +40,7 @@ def lib():
         L.or_thomas.argtypes = [DP, DP, DP, DP, DP, C.c_int]
         L.or_make_uniform.argtypes = [C.c_double, C.c_double, DP, C.c_int]
         L.or_make_regions.argtypes = [C.c_double, DP, DP, C.c_int, C.c_double, DP, C.c_int]
+        L.or_sim_record_chi.argtypes = [C.c_void_p, C.c_int]
         L.or_laser_gaussian.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 8 + [DP]
         L.or_laser_step.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                     DP, DP, DP, DP]

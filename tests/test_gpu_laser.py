@@ -190,3 +190,40 @@ def test_laser_batched_throughput(qa, oracle):
     print(f"laser batched step: {nb} boxes of {K} x {S.M} in {ms:.1f} ms, {ms / nb:.2f} ms per box")
     for L in lasers:
         L.close()
+
+
+M1_MOBILE = {"modes": 1, "xi.min": -6, "grid.dr": 0.04, "driver.n0": 0.5, "driver.x0": 0.05,
+             "driver.nparticles": 40000, "driver.sigma_r": 0.5, "electrons.ppc": 1}
+
+
+@pytest.mark.parametrize("deck,ov", [
+    ("blowout.in", {"xi.min": -10}),                          # m = 0, one workgroup
+    ("linear.in", M1_MOBILE),                                 # m = 1, one workgroup
+    ("linear.in", dict(M1_MOBILE, **{"sweep.groups": 3})),    # m = 1, three cooperating workgroups
+])
+def test_sweep_chi_feeds_the_laser(qa, oracle, decks, deck, ov):
+    """The sweep hands chi_0 of every slice to the laser march (qz_sweep_args.chi_out, qz_sim_record_chi): the
+    recorded chi equals the oracle's chi_0 of each slice (src/Simulation.cpp:735), the step's fields are unchanged by
+    the recording, and a laser step on the run's grid and box (xi.step 0.01) fed from the device array equals the
+    oracle's laser step fed from the oracle's chi."""
+    g = qa.Simulation(f"{decks}/{deck}", ov)
+    o = oracle.Sim(f"{decks}/{deck}", ov)
+    g.record_chi()
+    oracle.lib().or_sim_record_chi(o.h, 1)
+    g.step()
+    o.step()
+    assert np.array_equal(g.fld, o.fld)
+    chi_o = o.get("chi").reshape(g.K, g.M)
+    assert np.array_equal(g.chi(), chi_o)
+    nodes = g.get("nodes")
+    S = oracle.Solver(nodes)
+    K, dxi = g.K, 0.01
+    L = qa.Laser(nodes, K, 0.0, dxi, 2.0, 10.0, scheme="bdf2")
+    a0 = S.laser_gaussian(K, 0.0, dxi, 0.2, 2.0, 1.0, K * dxi / 2, 0.0, 10.0)
+    L.set_envelope(a0)
+    L.step(g.chi_device())
+    a1, pond = S.laser_step(a0, dxi, 2.0, 10.0, 1, False, chi_o)
+    assert np.array_equal(L.envelope(), a1) and np.array_equal(L.ponderomotive(), pond)
+    L.close()
+    g.record_chi(False)
+    assert g.chi_device() == 0
