@@ -330,7 +330,8 @@ int qz_sim_profile_size(qz_sim* s, int b);
 /* Record chi_0 of every slice of the coming sweeps (qz_sweep_args.chi_out) into a (K, M) device array of the run:
  * the plasma susceptibility that the laser envelope's advance_slice takes (src/Simulation.cpp:735), ready to pass
  * to qz_laser_step.  on = 0 stops and frees it.  qz_sim_get(s, "chi", ...) copies it to the host;
- * qz_sim_chi_device gives its device address (NULL when not recording).  Set before a qz_batch is captured. */
+ * qz_sim_chi_device gives its device address (NULL when not recording).  Set before a qz_batch is captured.  In
+ * pipelined runs a step sweeps each range after the step before it, so the array ends with the last step's chi_0. */
 int qz_sim_record_chi(qz_sim* s, int on);
 int qz_sim_chi_device(qz_sim* s, double** ptr);
 int qz_sim_snapshot(qz_sim* s);
